@@ -1,0 +1,174 @@
+/*
+ * mose_gpu.h -- C ABI of the MI355X-native time-step residual path of MOSE-GPU.
+ *
+ * The reference drives this path through Fortran module procedures called by Explicit_Step
+ * (src/lib/numerics/time/explicit/Mod_Explicit.f90:155-294) on module-global device arrays
+ * (src/lib/gpu/MOSE_Field_m.f90:42-72).  This header is the drop-in boundary that a Fortran host
+ * binds with ISO_C_BINDING (mose-gpu_amd/fortran/mose_gpu_c.f90); every entry point cites the
+ * reference procedure(s) it replaces.
+ *
+ * Conventions
+ *   - every function returns an int status: MOSE_OK (0) or a negative MOSE_ERR_*; the process is
+ *     never aborted.  mose_last_error() gives the message of the last failure of the context.
+ *   - the context owns all device memory; host arrays are borrowed for the duration of a call.
+ *   - one host thread per context; streams and graphs are internal.
+ *   - FP64 throughout.  Indices in records are the reference's 1-based (i,j,k), ghosts 1-2..n+2.
+ *
+ * Host layouts (the reference's flat arrays, MOSE_Field_m.f90:42-57 and Lib_Metrics.f90:107-133)
+ *   arena rows   NC = sum over blocks of (n1+4)(n2+4)(n3+4); block b starts at row boff[b] (blocks in
+ *                order); row(b,i,j,k) = boff + (i+1) + (j+1)*(n1+4) + (k+1)*(n1+4)*(n2+4)
+ *   state        P[s*NC + row], s = species 0..nsc-1, u, v, w, p, then the RANS variables (var-last)
+ *   cell geometry dl[d*NC+row] (cell lengths), m[(a+3b)*NC+row] (metric dxi_a/dx_b), vol[row], yn[row]
+ *   face geometry fm1[q*NF1 + f1] with q = nx, ny, nz, area and f1 = f1off[b] + i + (n1+1)*((j-1)+n2*(k-1)),
+ *                i = 0..n1 (fm1(4,0:n1,n2,n3) of the reference); fm2 (j = 0..n2), fm3 (k = 0..n3) alike
+ *   nodes        per block (n1+1)(n2+1)(n3+1) points, x,y,z interleaved, i fastest (the mesh file order)
+ *   tables       cp, h, mu, k per species on T = 0..10000 K (10001 rows, FLINT's Tmin = 0 layout,
+ *                lib/FLINT/src/lib/Load_ThermoTransport.f90:80-96): tab[s*10001 + T]
+ */
+#ifndef MOSE_GPU_H
+#define MOSE_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MOSE_GC 2
+#define MOSE_NTAB 10001
+#define MOSE_MAX_SPECIES 16
+
+enum {
+  MOSE_OK = 0,
+  MOSE_ERR_ARG = -1,       /* invalid argument / unsupported option */
+  MOSE_ERR_HIP = -2,       /* a HIP runtime call failed */
+  MOSE_ERR_NOMEM = -3,     /* device memory exhausted (or above the case's HBM budget) */
+  MOSE_ERR_MESH = -4,      /* degenerate cell (det = 0 or volume <= 0) */
+  MOSE_ERR_STATE = -5,     /* p < 0 or non-finite state after an update (State_Check_blk) */
+  MOSE_ERR_THERMO = -6,    /* T outside (1, 9990) K (the thermo guard) */
+  MOSE_ERR_NODEV = -7,     /* no GPU */
+  MOSE_ERR_ODE = -8        /* a chemistry integration returned idid < 0 */
+};
+
+/* boundary record types (internal ids of Mod_BC.f90; MOSE types: 201/1xx -> 1, 400 -> 11, 301 -> 5, 302 -> 6, 407 -> 4) */
+enum {
+  MOSE_BC_CONNECTION = 1,
+  MOSE_BC_FARFIELD = 4,
+  MOSE_BC_WALL_ADIABATIC = 5,
+  MOSE_BC_WALL_ISOTHERMAL = 6,
+  MOSE_BC_EXTRAPOLATION = 11
+};
+
+/* limiter ids (Lib_Limiters.f90:21-61) and Riemann solver ids (Lib_Riemann.f90:23-107) */
+enum { MOSE_LIM_MINMOD = 1, MOSE_LIM_VANLEER = 2, MOSE_LIM_VANALBADA = 3, MOSE_LIM_MC = 4, MOSE_LIM_SUPERBEE = 5, MOSE_LIM_FIRST = 6 };
+enum { MOSE_RS_HLLC = 1, MOSE_RS_AUSMP = 4, MOSE_RS_LMROE = 12 };
+
+/* one boundary cell-face: bc_bi/bc_bj/bc_bk/bc_bf/bc_typ of Mod_BC.f90, the connection donor of
+ * Lib_Extrapolation.f90:1023-1135 (conn_is/js/ks/fs, conn_d11..d22) and the wall / far-field data */
+typedef struct {
+  int32_t b, i, j, k, f, typ;      /* block (0-based), cell, face 1..6, type */
+  int32_t bs, is, js, ks, fs;      /* connection donor block (0-based), cell and face */
+  int32_t d11, d12, d21, d22;      /* transverse axis map (identity: 1,0,0,1) */
+  int32_t periodic;                /* MOSE 201: ghost nodes translated by one period (metrics only) */
+  double tw;                       /* isothermal wall temperature */
+  double farfield[8];              /* far field: rho(species sum), u, v, w, p, nu~ ... (type 4) */
+} mose_bc_record;
+
+typedef struct {
+  int32_t nsc;          /* species (nsc >= 1) */
+  int32_t nrans;        /* 0, or 1 for Spalart-Allmaras */
+  int32_t limiter;      /* MOSE_LIM_* */
+  int32_t riemann;      /* MOSE_RS_* */
+  int32_t viscous;      /* 0 euler, 1 navier-stokes */
+  int32_t n_rk;         /* 1..3 (SSP-RK3 = 3) */
+  int32_t ndir;         /* 1, 2 or 3 computational directions */
+  int32_t point_implicit;
+  int32_t thermo_guard; /* 1: T outside (1, 9990) K is an error (MOSE_ERR_THERMO) */
+  int32_t reserved[7];
+  double mco;           /* riemann-options-Mco (LMRoe) */
+  double cfl, vnn;
+  double prandtl;       /* > 0: k = mu cp / Pr */
+  double sc, sct, prt;  /* Schmidt, turbulent Schmidt and Prandtl numbers */
+  double reserved_d[8];
+} mose_scheme;
+
+typedef struct {
+  int32_t nsc;
+  int32_t pad;
+  const double* wm;     /* [nsc] molecular weights kg/kmol */
+  const double* cp;     /* [nsc*MOSE_NTAB] */
+  const double* h;
+  const double* mu;     /* NULL when inviscid */
+  const double* k;
+} mose_tables;
+
+typedef struct {
+  double dt;            /* the global time step of the last step */
+  double time;          /* accumulated time of the case */
+  int64_t steps;        /* steps done by the case */
+  int32_t bad_cells;    /* cells failing the state check / thermo guard in the last call */
+  int32_t first_bad_block, first_bad_i, first_bad_j, first_bad_k;
+  int32_t status;
+} mose_report;
+
+typedef struct mose_ctx mose_ctx;
+
+/* Context: device selection and the FLINT tables (flint_acc_upload_thermo, Lib_Radau5_dev.f90:118-159;
+ * cuf_init_tables, Mod_Convective.f90:59-76). */
+int mose_ctx_create(int device, const mose_tables* tables, mose_ctx** out);
+int mose_ctx_destroy(mose_ctx* ctx);
+const char* mose_last_error(const mose_ctx* ctx);
+int mose_device_count(int* n);
+
+/* Geometry of a multi-block mesh from its nodes (Setup_Metrics, Lib_Metrics.f90:15-136).  Host only.
+ * yn may be NULL; ndir as Check_Mesh_Type (Lib_Metrics.f90:139-196). */
+int mose_metrics_build(int nb, const int* dims, const double* nodes, int nrec, const mose_bc_record* rec, int ndir,
+                       double* dl, double* m, double* vol, double* yn, double* fm1, double* fm2, double* fm3);
+
+/* A case: blocks, geometry and boundary records made device resident (Setup_Device_Residency,
+ * Mod_Explicit.f90:404-482; Setup_BC_Device).  Records must be grouped by block, then by face 1..6
+ * (the launch order of BC_Fluxes, Mod_BC.f90:1042-1062).  Returns the case id in *case_id. */
+int mose_case_create(mose_ctx* ctx, const mose_scheme* scheme, int nb, const int* dims, const double* dl,
+                     const double* m, const double* vol, const double* yn, const double* fm1, const double* fm2,
+                     const double* fm3, int nrec, const mose_bc_record* rec, int* case_id);
+int mose_case_destroy(mose_ctx* ctx, int case_id);
+/* bytes of device memory a case of these blocks takes (the serving loop admits by this figure) */
+int mose_case_bytes(const mose_scheme* scheme, int nb, const int* dims, int nrec, int64_t* bytes);
+
+/* State in and out (pack_P_to_flat / unpack_flat_to_P, MOSE_Field_m.f90:171-194,334-369). */
+int mose_state_upload(mose_ctx* ctx, int case_id, const double* P);
+int mose_state_download(mose_ctx* ctx, int case_id, double* P);
+
+/* nsteps explicit steps of one case (Explicit_Step, Mod_Explicit.f90:52-397, time accurate without
+ * chemistry): per step the global dt (Compute_dt, Mod_dt.f90:15-243), PO <- P, then n_RK stages of
+ * ghosts + halo (Fill_Ghost_Cell, Lib_Extrapolation.f90:1370-1524), fluxes (Fluxes, Mod_Fluxes.f90:10;
+ * BC_Fluxes, Mod_BC.f90:646) and the conservative update (Newstate_Conservative_blk, Lib_Newstate.f90:257).
+ * The call is asynchronous on the context's stream except for the report. */
+int mose_case_step(mose_ctx* ctx, int case_id, int nsteps, mose_report* report);
+
+/* The same on several cases at once: one launch per kernel over all the blocks of all the cases
+ * (the block pack of MOSE_Pack_m.f90:70-251 as a persistent work queue); each case keeps its own dt. */
+int mose_cases_step(mose_ctx* ctx, const int* case_ids, int ncases, int nsteps, mose_report* reports);
+
+/* Residual of the current state (ghosts, fluxes and boundary fluxes of one stage, no update): R[s*NC+row]
+ * on the interior rows.  For checks against Fluxes + BC_Fluxes. */
+int mose_case_residual(mose_ctx* ctx, int case_id, double* R);
+
+/* Global dt of the current state (Compute_dt + the min reduction). */
+int mose_case_dt(mose_ctx* ctx, int case_id, double* dt);
+
+/* Sum over the interior cells of (sum of species, u, v, w, p, RANS)^2 differences between the current
+ * state and the step base (Compute_Residual_blk, Mod_Diagnostic.f90:63-106): out[nprim]. */
+int mose_case_residual_norm(mose_ctx* ctx, int case_id, double* out);
+
+/* Order-independent hash of the interior state (field_state_hash, MOSE_Field_m.f90:254-284). */
+int mose_case_state_hash(mose_ctx* ctx, int case_id, uint64_t* h_lo, uint64_t* h_hi);
+
+/* Average device time of the last timed step call per kernel family (ms): residual kernel, halo, dt. */
+int mose_case_timing(mose_ctx* ctx, int case_id, double* ms_residual, double* ms_halo, double* ms_other);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

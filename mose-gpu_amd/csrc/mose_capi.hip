@@ -1,0 +1,571 @@
+// Host side of the C ABI (include/mose_gpu.h): contexts, cases, the step sequencer and the launches.
+//
+// The sequencer is Explicit_Step (src/lib/numerics/time/explicit/Mod_Explicit.f90:52-397) for a time-accurate
+// step without chemistry: dt of the state at the start of the step, then n_RK stages.  The stage buffers are
+// rotated instead of copied (field_copy_P_flat_to_PO_flat, MOSE_Field_m.f90:321-330, is not needed): stage 1
+// reads the step base P0 and writes PA, stage 2 reads PA and writes PB, the last stage reads its input and
+// writes the new state over P0 (a cell-local read-then-write of the base).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "mose_gpu.h"
+#include "mose_internal.h"
+#include "mose_kernels.hpp"
+
+using namespace mose;
+
+namespace {
+
+struct Case {
+  mose_scheme sc{};
+  int nb = 0, nprim = 0;
+  std::vector<mose_blk> blk;
+  std::vector<mose_bc_record> rec;
+  int64_t NC = 0, NF1 = 0, NF2 = 0, NF3 = 0, NFI[3] = {0, 0, 0}, NBP = 0, ninterior = 0;
+  // device memory (one slab)
+  void* slab = nullptr;
+  size_t slab_bytes = 0;
+  double *P0 = nullptr, *PA = nullptr, *PB = nullptr;
+  double *dl = nullptr, *m = nullptr, *vol = nullptr, *yn = nullptr, *fm1 = nullptr, *fm2 = nullptr, *fm3 = nullptr;
+  double *Fc[3] = {nullptr, nullptr, nullptr}, *Fd[3] = {nullptr, nullptr, nullptr};
+  double* binc = nullptr;
+  mose_blk* dblk = nullptr;
+  mose_bc_record* drec = nullptr;
+  double* dt = nullptr;          // [1] current dt
+  double* dthist = nullptr;      // [hist_cap]
+  int hist_cap = 0;
+  unsigned long long* dtmin = nullptr;
+  unsigned int* bad = nullptr;   // [2]
+  unsigned long long* first_bad = nullptr;
+  double time = 0.0, last_dt = 0.0;
+  int64_t steps = 0;
+  bool valid = false;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  float ms_res = 0.f, ms_halo = 0.f, ms_other = 0.f;
+};
+
+}  // namespace
+
+struct mose_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int nsc = 0;
+  bool viscous_tables = false;
+  double *dcp = nullptr, *dh = nullptr, *dmi = nullptr, *dk = nullptr;
+  mose_dtab htab{};
+  mose_dtab* dtab = nullptr;
+  std::vector<std::unique_ptr<Case>> cases;
+  std::string err;
+};
+
+#define HIPCHK(ctx, call)                                                             \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess) {                                                           \
+      (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);                 \
+      return MOSE_ERR_HIP;                                                            \
+    }                                                                                 \
+  } while (0)
+
+static int fail(mose_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+extern "C" int mose_device_count(int* n) {
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+  if (n) *n = c;
+  return MOSE_OK;
+}
+
+extern "C" const char* mose_last_error(const mose_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+extern "C" int mose_ctx_create(int device, const mose_tables* t, mose_ctx** out) {
+  if (!out || !t || !t->wm || !t->cp || !t->h || t->nsc < 1 || t->nsc > MOSE_MAX_SPECIES) return MOSE_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return MOSE_ERR_NODEV;
+  if (device < 0 || device >= ndev) return MOSE_ERR_ARG;
+  auto ctx = std::make_unique<mose_ctx>();
+  ctx->device = device;
+  ctx->nsc = t->nsc;
+  HIPCHK(ctx.get(), hipSetDevice(device));
+  HIPCHK(ctx.get(), hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  size_t n = (size_t)t->nsc * MOSE_NTAB;
+  HIPCHK(ctx.get(), hipMalloc(&ctx->dcp, 4 * n * sizeof(double)));
+  ctx->dh = ctx->dcp + n;
+  ctx->dmi = ctx->dh + n;
+  ctx->dk = ctx->dmi + n;
+  HIPCHK(ctx.get(), hipMemcpy(ctx->dcp, t->cp, n * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(ctx.get(), hipMemcpy(ctx->dh, t->h, n * sizeof(double), hipMemcpyHostToDevice));
+  std::vector<double> zero(n, 0.0);
+  HIPCHK(ctx.get(), hipMemcpy(ctx->dmi, t->mu ? t->mu : zero.data(), n * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(ctx.get(), hipMemcpy(ctx->dk, t->k ? t->k : zero.data(), n * sizeof(double), hipMemcpyHostToDevice));
+  ctx->viscous_tables = (t->mu != nullptr && t->k != nullptr);
+  mose_dtab& T = ctx->htab;
+  T.cp = ctx->dcp; T.h = ctx->dh; T.mi = ctx->dmi; T.k = ctx->dk;
+  // Ri_tab = Runiv / wm_tab and the Wilke pair constants (Load_ThermoTransport.f90:51-67)
+  for (int s = 0; s < t->nsc; ++s) { T.wm[s] = t->wm[s]; T.Ri[s] = MOSE_RUNIV / t->wm[s]; }
+  for (int sj = 0; sj < t->nsc; ++sj)
+    for (int si = 0; si < t->nsc; ++si) {
+      double ratio = t->wm[si] / t->wm[sj];
+      T.mm025[si][sj] = std::pow(ratio, -0.25);
+      T.is8[si][sj] = 1.0 / std::sqrt(8.0 * (1.0 + ratio));
+    }
+  HIPCHK(ctx.get(), hipMalloc(&ctx->dtab, sizeof(mose_dtab)));
+  HIPCHK(ctx.get(), hipMemcpy(ctx->dtab, &T, sizeof(mose_dtab), hipMemcpyHostToDevice));
+  *out = ctx.release();
+  return MOSE_OK;
+}
+
+static void case_free(Case* c) {
+  if (!c) return;
+  if (c->slab) (void)hipFree(c->slab);
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  c->slab = nullptr;
+  c->valid = false;
+}
+
+extern "C" int mose_ctx_destroy(mose_ctx* ctx) {
+  if (!ctx) return MOSE_ERR_ARG;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (auto& c : ctx->cases) case_free(c.get());
+  if (ctx->dcp) (void)hipFree(ctx->dcp);
+  if (ctx->dtab) (void)hipFree(ctx->dtab);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return MOSE_OK;
+}
+
+// block table, arena sizes and validation of the scheme / records
+static int plan_case(const mose_scheme* sc, int nb, const int* dims, int nrec, const mose_bc_record* rec, Case& c,
+                     std::string& why) {
+  if (!sc || nb <= 0 || !dims) { why = "bad arguments"; return MOSE_ERR_ARG; }
+  if (sc->nsc < 1 || sc->nsc > MOSE_MAX_SPECIES) { why = "nsc out of range"; return MOSE_ERR_ARG; }
+  if (sc->nrans < 0 || sc->nrans > 1) { why = "nrans must be 0 or 1 (Spalart-Allmaras)"; return MOSE_ERR_ARG; }
+  if (sc->limiter < 1 || sc->limiter > 6) { why = "limiter id must be 1..6"; return MOSE_ERR_ARG; }
+  if (!(sc->riemann == MOSE_RS_HLLC || sc->riemann == MOSE_RS_AUSMP || sc->riemann == MOSE_RS_LMROE)) {
+    why = "riemann solver must be HLLC (1), AUSM+ (4) or LMRoe (12)";
+    return MOSE_ERR_ARG;
+  }
+  if (sc->n_rk < 1 || sc->n_rk > 3) { why = "n_rk must be 1..3"; return MOSE_ERR_ARG; }
+  if (sc->ndir < 1 || sc->ndir > 3) { why = "ndir must be 1..3"; return MOSE_ERR_ARG; }
+  c.sc = *sc;
+  c.nb = nb;
+  c.nprim = sc->nsc + 4 + sc->nrans;
+  c.blk.resize(nb);
+  int64_t NC = 0, NF1 = 0, NF2 = 0, NF3 = 0, NFI[3] = {0, 0, 0}, NBP = 0, nin = 0;
+  for (int b = 0; b < nb; ++b) {
+    int n1 = dims[3 * b], n2 = dims[3 * b + 1], n3 = dims[3 * b + 2];
+    if (n1 < 1 || n2 < 1 || n3 < 1) { why = "block dims must be >= 1"; return MOSE_ERR_ARG; }
+    mose_blk& B = c.blk[b];
+    B.n1 = n1; B.n2 = n2; B.n3 = n3; B.pi = n1 + 2 * MOSE_GC; B.pj = n2 + 2 * MOSE_GC; B.pad = 0;
+    B.boff = NC; B.f1off = NF1; B.f2off = NF2; B.f3off = NF3;
+    B.ifoff[0] = NFI[0]; B.ifoff[1] = NFI[1]; B.ifoff[2] = NFI[2];
+    int64_t pl[6] = {(int64_t)n2 * n3, (int64_t)n2 * n3, (int64_t)n1 * n3, (int64_t)n1 * n3, (int64_t)n1 * n2, (int64_t)n1 * n2};
+    for (int f = 0; f < 6; ++f) { B.bpoff[f] = NBP; NBP += pl[f]; }
+    B.cell0 = nin;
+    NC += (int64_t)B.pi * B.pj * (n3 + 2 * MOSE_GC);
+    NF1 += (int64_t)(n1 + 1) * n2 * n3;
+    NF2 += (int64_t)n1 * (n2 + 1) * n3;
+    NF3 += (int64_t)n1 * n2 * (n3 + 1);
+    NFI[0] += (int64_t)(n1 - 1) * n2 * n3;
+    NFI[1] += (int64_t)n1 * (n2 - 1) * n3;
+    NFI[2] += (int64_t)n1 * n2 * (n3 - 1);
+    nin += (int64_t)n1 * n2 * n3;
+  }
+  c.NC = NC; c.NF1 = NF1; c.NF2 = NF2; c.NF3 = NF3; c.NBP = NBP; c.ninterior = nin;
+  for (int d = 0; d < 3; ++d) c.NFI[d] = std::max<int64_t>(NFI[d], 1);
+  // records: every boundary cell-face exactly once, grouped by block then face
+  std::vector<uint8_t> seen((size_t)NBP, 0);
+  int lastb = -1, lastf = 0;
+  for (int e = 0; e < nrec; ++e) {
+    const mose_bc_record& r = rec[e];
+    if (r.b < 0 || r.b >= nb || r.f < 1 || r.f > 6) { why = "record " + std::to_string(e) + ": bad block or face"; return MOSE_ERR_ARG; }
+    if (r.b < lastb || (r.b == lastb && r.f < lastf)) { why = "records must be grouped by block, then face 1..6"; return MOSE_ERR_ARG; }
+    lastb = r.b; lastf = r.f;
+    const mose_blk& B = c.blk[r.b];
+    int n[3] = {B.n1, B.n2, B.n3};
+    int dir = (r.f + 1) / 2 - 1;
+    int idx[3] = {r.i, r.j, r.k};
+    for (int a = 0; a < 3; ++a)
+      if (idx[a] < 1 || idx[a] > n[a]) { why = "record " + std::to_string(e) + ": cell outside the block"; return MOSE_ERR_ARG; }
+    int want = (r.f % 2 == 1) ? 1 : n[dir];
+    if (idx[dir] != want) { why = "record " + std::to_string(e) + ": cell is not on face " + std::to_string(r.f); return MOSE_ERR_ARG; }
+    switch (r.typ) {
+      case MOSE_BC_CONNECTION: {
+        if (r.bs < 0 || r.bs >= nb || r.fs < 1 || r.fs > 6) { why = "record " + std::to_string(e) + ": bad donor"; return MOSE_ERR_ARG; }
+        const mose_blk& S = c.blk[r.bs];
+        int ns_[3] = {S.n1, S.n2, S.n3}, sidx[3] = {r.is, r.js, r.ks};
+        for (int a = 0; a < 3; ++a)
+          if (sidx[a] < 1 || sidx[a] > ns_[a]) { why = "record " + std::to_string(e) + ": donor outside its block"; return MOSE_ERR_ARG; }
+        int sdir = (r.fs + 1) / 2 - 1;
+        if (ns_[sdir] < MOSE_GC) { why = "record " + std::to_string(e) + ": donor block thinner than the halo"; return MOSE_ERR_ARG; }
+        // transverse donor neighbours must stay inside the padded donor block
+        int d1[3] = {0, 0, 0}, d2[3] = {0, 0, 0}, t1 = 0, t2 = 0;
+        for (int a = 0; a < 3; ++a) if (a != sdir) { if (t1 == 0) { d1[a] = r.d11; d2[a] = r.d21; t1 = 1; } else { d1[a] = r.d12; d2[a] = r.d22; t2 = 1; } }
+        (void)t2;
+        for (int a = 0; a < 3; ++a) {
+          int lo = sidx[a] - std::abs(d1[a]) - std::abs(d2[a]), hi = sidx[a] + std::abs(d1[a]) + std::abs(d2[a]);
+          if (lo < 1 - MOSE_GC || hi > ns_[a] + MOSE_GC) { why = "record " + std::to_string(e) + ": bad transverse map"; return MOSE_ERR_ARG; }
+        }
+        break;
+      }
+      case MOSE_BC_EXTRAPOLATION:
+        if (n[dir] == 2) { why = "extrapolation faces need a block extent of 1 or >= 3 along the face normal"; return MOSE_ERR_ARG; }
+        break;
+      default:
+        why = "record " + std::to_string(e) + ": boundary type " + std::to_string(r.typ) + " is not supported yet";
+        return MOSE_ERR_ARG;
+    }
+    int64_t pos = B.bpoff[r.f - 1] + (r.f <= 2 ? (r.j - 1) + (int64_t)B.n2 * (r.k - 1)
+                                     : r.f <= 4 ? (r.i - 1) + (int64_t)B.n1 * (r.k - 1)
+                                                : (r.i - 1) + (int64_t)B.n1 * (r.j - 1));
+    if (seen[pos]) { why = "record " + std::to_string(e) + ": boundary face listed twice"; return MOSE_ERR_ARG; }
+    seen[pos] = 1;
+  }
+  for (int64_t q = 0; q < NBP; ++q)
+    if (!seen[q]) { why = "a boundary cell-face has no record (every block face needs one record per cell)"; return MOSE_ERR_ARG; }
+  c.rec.assign(rec, rec + nrec);
+  return MOSE_OK;
+}
+
+static size_t case_bytes(const Case& c) {
+  size_t d = sizeof(double);
+  size_t b = 0;
+  b += 3 * (size_t)c.NC * c.nprim * d;                     // P0 PA PB
+  b += (size_t)c.NC * (3 + 9 + 1 + 1) * d;                 // dl m vol yn
+  b += (size_t)(c.NF1 + c.NF2 + c.NF3) * 4 * d;            // fm1..3
+  for (int q = 0; q < 3; ++q) b += 2 * (size_t)c.NFI[q] * c.nprim * d;   // face buffers
+  b += 2 * (size_t)std::max<int64_t>(c.NBP, 1) * c.nprim * d;           // boundary increments
+  b += c.blk.size() * sizeof(mose_blk) + c.rec.size() * sizeof(mose_bc_record) + 4096 + 64 * 1024;
+  return b;
+}
+
+extern "C" int mose_case_bytes(const mose_scheme* sc, int nb, const int* dims, int nrec, int64_t* bytes) {
+  Case c;
+  std::string why;
+  if (!bytes) return MOSE_ERR_ARG;
+  // sizing does not need the records themselves
+  if (!sc || nb <= 0 || !dims) return MOSE_ERR_ARG;
+  c.nprim = sc->nsc + 4 + sc->nrans;
+  int64_t NC = 0, NF = 0, NFI[3] = {0, 0, 0}, NBP = 0;
+  for (int b = 0; b < nb; ++b) {
+    int64_t n1 = dims[3 * b], n2 = dims[3 * b + 1], n3 = dims[3 * b + 2];
+    NC += (n1 + 4) * (n2 + 4) * (n3 + 4);
+    NF += (n1 + 1) * n2 * n3 + n1 * (n2 + 1) * n3 + n1 * n2 * (n3 + 1);
+    NFI[0] += (n1 - 1) * n2 * n3; NFI[1] += n1 * (n2 - 1) * n3; NFI[2] += n1 * n2 * (n3 - 1);
+    NBP += 2 * (n2 * n3 + n1 * n3 + n1 * n2);
+  }
+  c.NC = NC; c.NF1 = NF; c.NF2 = 0; c.NF3 = 0; c.NBP = NBP;
+  for (int d = 0; d < 3; ++d) c.NFI[d] = std::max<int64_t>(NFI[d], 1);
+  c.blk.resize(nb);
+  c.rec.resize(std::max(nrec, 0));
+  *bytes = (int64_t)case_bytes(c);
+  return MOSE_OK;
+}
+
+static int alloc_case(mose_ctx* ctx, Case& c) {
+  size_t d = sizeof(double);
+  c.hist_cap = 4096;
+  size_t total = case_bytes(c) + (size_t)c.hist_cap * d;
+  HIPCHK(ctx, hipMalloc(&c.slab, total));
+  c.slab_bytes = total;
+  char* p = (char*)c.slab;
+  auto take = [&](size_t nbytes) {
+    char* q = p;
+    p += (nbytes + 255) & ~(size_t)255;
+    return q;
+  };
+  size_t st = (size_t)c.NC * c.nprim * d;
+  c.P0 = (double*)take(st);
+  c.PA = (double*)take(st);
+  c.PB = (double*)take(st);
+  c.dl = (double*)take(3 * (size_t)c.NC * d);
+  c.m = (double*)take(9 * (size_t)c.NC * d);
+  c.vol = (double*)take((size_t)c.NC * d);
+  c.yn = (double*)take((size_t)c.NC * d);
+  c.fm1 = (double*)take(4 * (size_t)c.NF1 * d);
+  c.fm2 = (double*)take(4 * (size_t)c.NF2 * d);
+  c.fm3 = (double*)take(4 * (size_t)c.NF3 * d);
+  for (int q = 0; q < 3; ++q) {
+    c.Fc[q] = (double*)take((size_t)c.NFI[q] * c.nprim * d);
+    c.Fd[q] = (double*)take((size_t)c.NFI[q] * c.nprim * d);
+  }
+  c.binc = (double*)take(2 * (size_t)std::max<int64_t>(c.NBP, 1) * c.nprim * d);
+  c.dblk = (mose_blk*)take(c.blk.size() * sizeof(mose_blk));
+  c.drec = (mose_bc_record*)take(std::max<size_t>(c.rec.size(), 1) * sizeof(mose_bc_record));
+  c.dt = (double*)take(64);
+  c.dthist = (double*)take((size_t)c.hist_cap * d);
+  c.dtmin = (unsigned long long*)take(64);
+  c.bad = (unsigned int*)take(64);
+  c.first_bad = (unsigned long long*)take(64);
+  if ((size_t)(p - (char*)c.slab) > total) return fail(ctx, MOSE_ERR_NOMEM, "internal: slab overflow");
+  for (auto& e : c.ev) HIPCHK(ctx, hipEventCreate(&e));
+  return MOSE_OK;
+}
+
+static Case* get_case(mose_ctx* ctx, int id) {
+  if (!ctx || id < 0 || id >= (int)ctx->cases.size()) return nullptr;
+  Case* c = ctx->cases[id].get();
+  return (c && c->valid) ? c : nullptr;
+}
+
+extern "C" int mose_case_create(mose_ctx* ctx, const mose_scheme* sc, int nb, const int* dims, const double* dl,
+                                const double* m, const double* vol, const double* yn, const double* fm1,
+                                const double* fm2, const double* fm3, int nrec, const mose_bc_record* rec, int* case_id) {
+  if (!ctx || !case_id || !dl || !m || !vol || !fm1 || !fm2 || !fm3 || (nrec > 0 && !rec)) return MOSE_ERR_ARG;
+  auto c = std::make_unique<Case>();
+  std::string why;
+  int rc = plan_case(sc, nb, dims, nrec, rec, *c, why);
+  if (rc != MOSE_OK) return fail(ctx, rc, why);
+  if (sc->nsc != ctx->nsc) return fail(ctx, MOSE_ERR_ARG, "the scheme's nsc differs from the context tables");
+  if (sc->viscous && !ctx->viscous_tables) return fail(ctx, MOSE_ERR_ARG, "viscous case needs mu and k tables");
+  if (!((sc->nsc == 1 && sc->nrans == 0) || (sc->nsc == 1 && sc->nrans == 1)))
+    return fail(ctx, MOSE_ERR_ARG, "this build instantiates the residual kernels for nsc = 1 (nrans 0 or 1)");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  rc = alloc_case(ctx, *c);
+  if (rc != MOSE_OK) { case_free(c.get()); return rc; }
+  size_t d = sizeof(double);
+  hipStream_t s = ctx->stream;
+  HIPCHK(ctx, hipMemcpyAsync(c->dl, dl, 3 * (size_t)c->NC * d, hipMemcpyHostToDevice, s));
+  HIPCHK(ctx, hipMemcpyAsync(c->m, m, 9 * (size_t)c->NC * d, hipMemcpyHostToDevice, s));
+  HIPCHK(ctx, hipMemcpyAsync(c->vol, vol, (size_t)c->NC * d, hipMemcpyHostToDevice, s));
+  if (yn) HIPCHK(ctx, hipMemcpyAsync(c->yn, yn, (size_t)c->NC * d, hipMemcpyHostToDevice, s));
+  else HIPCHK(ctx, hipMemsetAsync(c->yn, 0, (size_t)c->NC * d, s));
+  HIPCHK(ctx, hipMemcpyAsync(c->fm1, fm1, 4 * (size_t)c->NF1 * d, hipMemcpyHostToDevice, s));
+  HIPCHK(ctx, hipMemcpyAsync(c->fm2, fm2, 4 * (size_t)c->NF2 * d, hipMemcpyHostToDevice, s));
+  HIPCHK(ctx, hipMemcpyAsync(c->fm3, fm3, 4 * (size_t)c->NF3 * d, hipMemcpyHostToDevice, s));
+  HIPCHK(ctx, hipMemcpyAsync(c->dblk, c->blk.data(), c->blk.size() * sizeof(mose_blk), hipMemcpyHostToDevice, s));
+  if (!c->rec.empty())
+    HIPCHK(ctx, hipMemcpyAsync(c->drec, c->rec.data(), c->rec.size() * sizeof(mose_bc_record), hipMemcpyHostToDevice, s));
+  HIPCHK(ctx, hipMemsetAsync(c->P0, 0, 3 * (size_t)c->NC * c->nprim * d, s));
+  HIPCHK(ctx, hipStreamSynchronize(s));
+  c->valid = true;
+  ctx->cases.push_back(std::move(c));
+  *case_id = (int)ctx->cases.size() - 1;
+  return MOSE_OK;
+}
+
+extern "C" int mose_case_destroy(mose_ctx* ctx, int id) {
+  Case* c = get_case(ctx, id);
+  if (!c) return MOSE_ERR_ARG;
+  (void)hipSetDevice(ctx->device);
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  case_free(c);
+  return MOSE_OK;
+}
+
+extern "C" int mose_state_upload(mose_ctx* ctx, int id, const double* P) {
+  Case* c = get_case(ctx, id);
+  if (!c || !P) return MOSE_ERR_ARG;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  size_t n = (size_t)c->NC * c->nprim * sizeof(double);
+  HIPCHK(ctx, hipMemcpyAsync(c->P0, P, n, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return MOSE_OK;
+}
+
+extern "C" int mose_state_download(mose_ctx* ctx, int id, double* P) {
+  Case* c = get_case(ctx, id);
+  if (!c || !P) return MOSE_ERR_ARG;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  size_t n = (size_t)c->NC * c->nprim * sizeof(double);
+  HIPCHK(ctx, hipMemcpyAsync(P, c->P0, n, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return MOSE_OK;
+}
+
+// ----------------------------------------------------------------------------------------------------------------
+// launches
+// ----------------------------------------------------------------------------------------------------------------
+static CaseDev make_dev(mose_ctx* ctx, Case& c, const double* Pin, double* Pout) {
+  CaseDev D{};
+  D.Pin = Pin; D.Pout = Pout; D.PO = c.P0; D.Pg = const_cast<double*>(Pin);
+  D.NC = c.NC; D.NF1 = c.NF1; D.NF2 = c.NF2; D.NF3 = c.NF3;
+  D.dl = c.dl; D.m = c.m; D.vol = c.vol; D.yn = c.yn; D.fm1 = c.fm1; D.fm2 = c.fm2; D.fm3 = c.fm3;
+  for (int q = 0; q < 3; ++q) { D.Fc[q] = c.Fc[q]; D.Fd[q] = c.Fd[q]; D.NFI[q] = c.NFI[q]; }
+  D.binc = c.binc; D.NBP = std::max<int64_t>(c.NBP, 1);
+  D.blk = c.dblk; D.rec = c.drec; D.nrec = (int)c.rec.size();
+  D.lim = c.sc.limiter; D.rs = c.sc.riemann; D.visc = c.sc.viscous; D.guard = c.sc.thermo_guard; D.mco = c.sc.mco;
+  D.vc.sc = c.sc.sc; D.vc.sct = c.sc.sct; D.vc.prt = c.sc.prt; D.vc.prl = c.sc.prandtl;
+  D.tab = ctx->dtab;
+  D.dt = c.dt; D.dtmin = c.dtmin; D.bad = c.bad; D.first_bad = c.first_bad;
+  D.cfl = c.sc.cfl; D.vnn = c.sc.vnn; D.ndir = c.sc.ndir;
+  return D;
+}
+
+static inline unsigned nblocks(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+template <int NS, int NR>
+static void launch_dt(Case& c, const CaseDev& D, hipStream_t s, int step_index) {
+  k_dt_init<<<1, 64, 0, s>>>(c.dtmin);
+  for (int b = 0; b < c.nb; ++b) {
+    int64_t n = (int64_t)c.blk[b].n1 * c.blk[b].n2 * c.blk[b].n3;
+    k_dt<NS, NR><<<nblocks(n, 256), 256, 0, s>>>(D, b);
+  }
+  k_dt_final<<<1, 64, 0, s>>>(c.dtmin, c.dt, step_index >= 0 ? c.dthist : nullptr, step_index);
+}
+
+template <int NS, int NR>
+static void launch_flux(Case& c, const CaseDev& D, hipStream_t s) {
+  int nrec = (int)c.rec.size();
+  if (nrec > 0) {
+    k_ghost<NS, NR><<<nblocks(nrec, 256), 256, 0, s>>>(D);
+    k_bcrec<NS, NR><<<nblocks(nrec, 128), 128, 0, s>>>(D);
+  }
+  for (int b = 0; b < c.nb; ++b) {
+    const mose_blk& B = c.blk[b];
+    int64_t n0 = (int64_t)(B.n1 - 1) * B.n2 * B.n3, n1_ = (int64_t)B.n1 * (B.n2 - 1) * B.n3;
+    int64_t n2_ = (int64_t)B.n1 * B.n2 * (B.n3 - 1);
+    if (n0 > 0) k_face<NS, NR, 0><<<nblocks(n0, 128), 128, 0, s>>>(D, b);
+    if (n1_ > 0) k_face<NS, NR, 1><<<nblocks(n1_, 128), 128, 0, s>>>(D, b);
+    if (n2_ > 0) k_face<NS, NR, 2><<<nblocks(n2_, 128), 128, 0, s>>>(D, b);
+  }
+}
+
+template <int NS, int NR>
+static int run_steps(mose_ctx* ctx, Case& c, int nsteps) {
+  hipStream_t s = ctx->stream;
+  static const double rk[3][3] = {{1.0, 0.0, 0.0}, {1.0, 0.5, 0.0}, {1.0, 0.25, 2.0 / 3.0}};   // Lib_RK.f90:6-9
+  int nrk = c.sc.n_rk;
+  HIPCHK(ctx, hipMemsetAsync(c.bad, 0, 2 * sizeof(unsigned), s));
+  HIPCHK(ctx, hipMemsetAsync(c.first_bad, 0xff, sizeof(unsigned long long), s));
+  int hist0 = 0;
+  for (int n = 0; n < nsteps; ++n) {
+    CaseDev D0 = make_dev(ctx, c, c.P0, c.PA);
+    launch_dt<NS, NR>(c, D0, s, (n < c.hist_cap) ? n : -1);
+    for (int irk = 1; irk <= nrk; ++irk) {
+      double* in = irk == 1 ? c.P0 : (irk == 2 ? c.PA : c.PB);
+      double* out = irk == nrk ? c.P0 : (irk == 1 ? c.PA : c.PB);
+      CaseDev D = make_dev(ctx, c, in, out);
+      launch_flux<NS, NR>(c, D, s);
+      for (int b = 0; b < c.nb; ++b) {
+        int64_t ncell = (int64_t)c.blk[b].n1 * c.blk[b].n2 * c.blk[b].n3;
+        k_update<NS, NR><<<nblocks(ncell, 256), 256, 0, s>>>(D, b, rk[nrk - 1][irk - 1]);
+      }
+    }
+  }
+  (void)hist0;
+  HIPCHK(ctx, hipGetLastError());
+  return MOSE_OK;
+}
+
+extern "C" int mose_case_step(mose_ctx* ctx, int id, int nsteps, mose_report* rep) {
+  Case* c = get_case(ctx, id);
+  if (!c || nsteps < 0) return MOSE_ERR_ARG;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  int rc;
+  if (c->sc.nrans == 0) rc = run_steps<1, 0>(ctx, *c, nsteps);
+  else rc = run_steps<1, 1>(ctx, *c, nsteps);
+  if (rc != MOSE_OK) return rc;
+  std::vector<double> hist(std::min(nsteps, c->hist_cap));
+  unsigned bad[2] = {0, 0};
+  unsigned long long fb = ~0ull;
+  if (!hist.empty())
+    HIPCHK(ctx, hipMemcpyAsync(hist.data(), c->dthist, hist.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(bad, c->bad, sizeof bad, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(&fb, c->first_bad, sizeof fb, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  for (double v : hist) c->time += v;
+  if (!hist.empty()) c->last_dt = hist.back();
+  c->steps += nsteps;
+  int status = MOSE_OK;
+  if (bad[1] > 0) status = MOSE_ERR_THERMO;
+  if (bad[0] > 0) status = MOSE_ERR_STATE;
+  if (rep) {
+    rep->dt = c->last_dt;
+    rep->time = c->time;
+    rep->steps = c->steps;
+    rep->bad_cells = (int)(bad[0] + bad[1]);
+    rep->first_bad_block = fb == ~0ull ? -1 : (int)(fb >> 48);
+    rep->first_bad_k = fb == ~0ull ? -1 : (int)((fb >> 32) & 0xffff);
+    rep->first_bad_j = fb == ~0ull ? -1 : (int)((fb >> 16) & 0xffff);
+    rep->first_bad_i = fb == ~0ull ? -1 : (int)(fb & 0xffff);
+    rep->status = status;
+  }
+  if (status != MOSE_OK) {
+    char msg[256];
+    snprintf(msg, sizeof msg, "%s in %u cells, first at block %d (i,j,k) = (%d,%d,%d)",
+             status == MOSE_ERR_STATE ? "p < 0 or non-finite state" : "T outside (1, 9990) K", bad[0] + bad[1],
+             fb == ~0ull ? -1 : (int)(fb >> 48), (int)(fb & 0xffff), (int)((fb >> 16) & 0xffff), (int)((fb >> 32) & 0xffff));
+    ctx->err = msg;
+  }
+  return status;
+}
+
+extern "C" int mose_cases_step(mose_ctx* ctx, const int* ids, int n, int nsteps, mose_report* reps) {
+  if (!ctx || !ids || n < 0) return MOSE_ERR_ARG;
+  int worst = MOSE_OK;
+  for (int q = 0; q < n; ++q) {
+    int rc = mose_case_step(ctx, ids[q], nsteps, reps ? &reps[q] : nullptr);
+    if (rc != MOSE_OK && worst == MOSE_OK) worst = rc;
+  }
+  return worst;
+}
+
+template <int NS, int NR>
+static int residual_impl(mose_ctx* ctx, Case& c, double* R) {
+  hipStream_t s = ctx->stream;
+  CaseDev D = make_dev(ctx, c, c.P0, c.PA);
+  launch_flux<NS, NR>(c, D, s);
+  double* dR = c.PB;   // scratch
+  HIPCHK(ctx, hipMemsetAsync(dR, 0, (size_t)c.NC * c.nprim * sizeof(double), s));
+  for (int b = 0; b < c.nb; ++b) {
+    int64_t ncell = (int64_t)c.blk[b].n1 * c.blk[b].n2 * c.blk[b].n3;
+    k_residual_out<NS, NR><<<nblocks(ncell, 256), 256, 0, s>>>(D, b, dR);
+  }
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipMemcpyAsync(R, dR, (size_t)c.NC * c.nprim * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(ctx, hipStreamSynchronize(s));
+  return MOSE_OK;
+}
+
+extern "C" int mose_case_residual(mose_ctx* ctx, int id, double* R) {
+  Case* c = get_case(ctx, id);
+  if (!c || !R) return MOSE_ERR_ARG;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  return c->sc.nrans == 0 ? residual_impl<1, 0>(ctx, *c, R) : residual_impl<1, 1>(ctx, *c, R);
+}
+
+extern "C" int mose_case_dt(mose_ctx* ctx, int id, double* dt) {
+  Case* c = get_case(ctx, id);
+  if (!c || !dt) return MOSE_ERR_ARG;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  CaseDev D = make_dev(ctx, *c, c->P0, c->PA);
+  if (c->sc.nrans == 0) launch_dt<1, 0>(*c, D, ctx->stream, -1);
+  else launch_dt<1, 1>(*c, D, ctx->stream, -1);
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipMemcpyAsync(dt, c->dt, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return MOSE_OK;
+}
+
+extern "C" int mose_case_residual_norm(mose_ctx* ctx, int id, double* out) {
+  (void)ctx; (void)id; (void)out;
+  return fail(ctx, MOSE_ERR_ARG, "mose_case_residual_norm: not implemented in this build");
+}
+
+extern "C" int mose_case_state_hash(mose_ctx* ctx, int id, uint64_t* lo, uint64_t* hi) {
+  (void)id; (void)lo; (void)hi;
+  return fail(ctx, MOSE_ERR_ARG, "mose_case_state_hash: not implemented in this build");
+}
+
+extern "C" int mose_case_timing(mose_ctx* ctx, int id, double* a, double* b, double* c2) {
+  Case* c = get_case(ctx, id);
+  if (!c) return MOSE_ERR_ARG;
+  if (a) *a = c->ms_res;
+  if (b) *b = c->ms_halo;
+  if (c2) *c2 = c->ms_other;
+  return MOSE_OK;
+}

@@ -1,0 +1,274 @@
+"""ctypes binding of the MI355X C ABI (include/mose_gpu.h).
+
+The library is the in-tree ``mose-gpu_amd/lib/libmosegpu.so``; every entry point raises ``MoseError`` on a
+negative status, with the context's message.  Nothing here falls back to a CPU path: without the library or a
+GPU the calls fail.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG_DIR = Path(__file__).resolve().parent.parent          # mose-gpu_amd/
+LIB_PATH = PKG_DIR / "lib" / "libmosegpu.so"
+
+GC = 2
+NTAB = 10001
+
+MOSE_OK = 0
+ERRORS = {
+    -1: "MOSE_ERR_ARG", -2: "MOSE_ERR_HIP", -3: "MOSE_ERR_NOMEM", -4: "MOSE_ERR_MESH",
+    -5: "MOSE_ERR_STATE", -6: "MOSE_ERR_THERMO", -7: "MOSE_ERR_NODEV", -8: "MOSE_ERR_ODE",
+}
+
+BC_CONNECTION, BC_FARFIELD, BC_WALL_ADIABATIC, BC_WALL_ISOTHERMAL, BC_EXTRAPOLATION = 1, 4, 5, 6, 11
+LIM = {"minmod": 1, "vanleer": 2, "vanalbada": 3, "mc": 4, "superbee": 5, "first-order": 6}
+RS = {"HLLC": 1, "AUSM+": 4, "AUSMp": 4, "LMRoe": 12, "Roe": 12}
+
+# one boundary cell-face (mose_bc_record)
+RECORD_DTYPE = np.dtype(
+    [("b", "<i4"), ("i", "<i4"), ("j", "<i4"), ("k", "<i4"), ("f", "<i4"), ("typ", "<i4"),
+     ("bs", "<i4"), ("is_", "<i4"), ("js", "<i4"), ("ks", "<i4"), ("fs", "<i4"),
+     ("d11", "<i4"), ("d12", "<i4"), ("d21", "<i4"), ("d22", "<i4"), ("periodic", "<i4"),
+     ("tw", "<f8"), ("farfield", "<f8", (8,))],
+    align=True,
+)
+assert RECORD_DTYPE.itemsize == 136
+
+
+class MoseError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Scheme(C.Structure):
+    _fields_ = [
+        ("nsc", C.c_int32), ("nrans", C.c_int32), ("limiter", C.c_int32), ("riemann", C.c_int32),
+        ("viscous", C.c_int32), ("n_rk", C.c_int32), ("ndir", C.c_int32), ("point_implicit", C.c_int32),
+        ("thermo_guard", C.c_int32), ("reserved", C.c_int32 * 7),
+        ("mco", C.c_double), ("cfl", C.c_double), ("vnn", C.c_double), ("prandtl", C.c_double),
+        ("sc", C.c_double), ("sct", C.c_double), ("prt", C.c_double), ("reserved_d", C.c_double * 8),
+    ]
+
+
+class Tables(C.Structure):
+    _fields_ = [("nsc", C.c_int32), ("pad", C.c_int32), ("wm", C.POINTER(C.c_double)),
+                ("cp", C.POINTER(C.c_double)), ("h", C.POINTER(C.c_double)),
+                ("mu", C.POINTER(C.c_double)), ("k", C.POINTER(C.c_double))]
+
+
+class Report(C.Structure):
+    _fields_ = [("dt", C.c_double), ("time", C.c_double), ("steps", C.c_int64), ("bad_cells", C.c_int32),
+                ("first_bad_block", C.c_int32), ("first_bad_i", C.c_int32), ("first_bad_j", C.c_int32),
+                ("first_bad_k", C.c_int32), ("status", C.c_int32)]
+
+
+def dptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def iptr(a: np.ndarray):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+_lib_handle = None
+
+
+def lib() -> C.CDLL:
+    """The loaded C ABI.  Raises if the in-tree library is not built."""
+    global _lib_handle
+    if _lib_handle is not None:
+        return _lib_handle
+    if not LIB_PATH.exists():
+        raise FileNotFoundError(f"{LIB_PATH} is not built (run __graft_entry__.build())")
+    L = C.CDLL(str(LIB_PATH))
+    vp, ip, dp = C.c_void_p, C.c_int, C.POINTER(C.c_double)
+    L.mose_last_error.restype = C.c_char_p
+    L.mose_last_error.argtypes = [vp]
+    L.mose_device_count.argtypes = [C.POINTER(C.c_int)]
+    L.mose_ctx_create.argtypes = [ip, C.POINTER(Tables), C.POINTER(vp)]
+    L.mose_ctx_destroy.argtypes = [vp]
+    L.mose_metrics_build.argtypes = [ip, C.POINTER(C.c_int), dp, ip, C.c_void_p, ip, dp, dp, dp, dp, dp, dp, dp]
+    L.mose_case_create.argtypes = [vp, C.POINTER(Scheme), ip, C.POINTER(C.c_int), dp, dp, dp, dp, dp, dp, dp, ip,
+                                   C.c_void_p, C.POINTER(C.c_int)]
+    L.mose_case_destroy.argtypes = [vp, ip]
+    L.mose_case_bytes.argtypes = [C.POINTER(Scheme), ip, C.POINTER(C.c_int), ip, C.POINTER(C.c_int64)]
+    L.mose_state_upload.argtypes = [vp, ip, dp]
+    L.mose_state_download.argtypes = [vp, ip, dp]
+    L.mose_case_step.argtypes = [vp, ip, ip, C.POINTER(Report)]
+    L.mose_cases_step.argtypes = [vp, C.POINTER(C.c_int), ip, ip, C.POINTER(Report)]
+    L.mose_case_residual.argtypes = [vp, ip, dp]
+    L.mose_case_dt.argtypes = [vp, ip, dp]
+    L.mose_case_residual_norm.argtypes = [vp, ip, dp]
+    L.mose_case_state_hash.argtypes = [vp, ip, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    L.mose_case_timing.argtypes = [vp, ip, dp, dp, dp]
+    for name in ("mose_ctx_create", "mose_ctx_destroy", "mose_metrics_build", "mose_case_create",
+                 "mose_case_destroy", "mose_case_bytes", "mose_state_upload", "mose_state_download",
+                 "mose_case_step", "mose_cases_step", "mose_case_residual", "mose_case_dt",
+                 "mose_case_residual_norm", "mose_case_state_hash", "mose_case_timing", "mose_device_count"):
+        getattr(L, name).restype = C.c_int
+    _lib_handle = L
+    return L
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    lib().mose_device_count(C.byref(n))
+    return n.value
+
+
+# ------------------------------------------------------------------------------------------------------------
+# geometry
+# ------------------------------------------------------------------------------------------------------------
+class Layout:
+    """Arena sizes of a list of blocks (rows of the padded cells, face counts)."""
+
+    def __init__(self, dims):
+        self.dims = [tuple(int(x) for x in d) for d in dims]
+        self.nb = len(self.dims)
+        self.boff, self.f1off, self.f2off, self.f3off = [], [], [], []
+        NC = NF1 = NF2 = NF3 = 0
+        for n1, n2, n3 in self.dims:
+            self.boff.append(NC); self.f1off.append(NF1); self.f2off.append(NF2); self.f3off.append(NF3)
+            NC += (n1 + 2 * GC) * (n2 + 2 * GC) * (n3 + 2 * GC)
+            NF1 += (n1 + 1) * n2 * n3
+            NF2 += n1 * (n2 + 1) * n3
+            NF3 += n1 * n2 * (n3 + 1)
+        self.NC, self.NF1, self.NF2, self.NF3 = NC, NF1, NF2, NF3
+        self.dims_arr = np.ascontiguousarray(np.array(self.dims, dtype=np.int32).reshape(-1))
+
+    def block_view(self, arr: np.ndarray, b: int, nvar: int | None = None):
+        """View of block b of an arena array [nvar, NC] as [nvar, n3+4, n2+4, n1+4] (k, j, i)."""
+        n1, n2, n3 = self.dims[b]
+        shp = (n3 + 2 * GC, n2 + 2 * GC, n1 + 2 * GC)
+        npad = shp[0] * shp[1] * shp[2]
+        if arr.ndim == 1:
+            return arr[self.boff[b]:self.boff[b] + npad].reshape(shp)
+        return arr[:, self.boff[b]:self.boff[b] + npad].reshape((arr.shape[0],) + shp)
+
+    def interior(self, arr: np.ndarray, b: int):
+        v = self.block_view(arr, b)
+        return v[..., GC:-GC, GC:-GC, GC:-GC]
+
+
+class Geometry:
+    def __init__(self, layout: Layout, dl, m, vol, yn, fm1, fm2, fm3):
+        self.layout = layout
+        self.dl, self.m, self.vol, self.yn, self.fm1, self.fm2, self.fm3 = dl, m, vol, yn, fm1, fm2, fm3
+
+
+def metrics_build(layout: Layout, nodes: list[np.ndarray], records: np.ndarray, ndir: int) -> Geometry:
+    """Setup_Metrics (Lib_Metrics.f90:15-136) through mose_metrics_build.  nodes[b]: (n3+1, n2+1, n1+1, 3)."""
+    flat = np.ascontiguousarray(np.concatenate([np.ascontiguousarray(n, dtype=np.float64).reshape(-1) for n in nodes]))
+    NC = layout.NC
+    dl = np.zeros(3 * NC); m = np.zeros(9 * NC); vol = np.zeros(NC); yn = np.zeros(NC)
+    fm1 = np.zeros(4 * layout.NF1); fm2 = np.zeros(4 * layout.NF2); fm3 = np.zeros(4 * layout.NF3)
+    rec = np.ascontiguousarray(records, dtype=RECORD_DTYPE)
+    rc = lib().mose_metrics_build(layout.nb, iptr(layout.dims_arr), dptr(flat), len(rec),
+                                  rec.ctypes.data_as(C.c_void_p), ndir, dptr(dl), dptr(m), dptr(vol), dptr(yn),
+                                  dptr(fm1), dptr(fm2), dptr(fm3))
+    if rc != MOSE_OK:
+        raise MoseError(rc, "mose_metrics_build failed (degenerate cell or bad record)")
+    return Geometry(layout, dl, m, vol, yn, fm1, fm2, fm3)
+
+
+# ------------------------------------------------------------------------------------------------------------
+# context and cases
+# ------------------------------------------------------------------------------------------------------------
+class ThermoTables:
+    """FLINT per-Kelvin tables on T = 0..10000 K (Load_ThermoTransport.f90:7-189): cp, h, mu, k [nsc, 10001]."""
+
+    def __init__(self, wm, cp, h, mu=None, k=None, names=None):
+        self.wm = np.ascontiguousarray(np.atleast_1d(np.asarray(wm, dtype=np.float64)))
+        self.nsc = self.wm.size
+        self.cp = np.ascontiguousarray(np.asarray(cp, dtype=np.float64).reshape(self.nsc, NTAB))
+        self.h = np.ascontiguousarray(np.asarray(h, dtype=np.float64).reshape(self.nsc, NTAB))
+        self.mu = None if mu is None else np.ascontiguousarray(np.asarray(mu, dtype=np.float64).reshape(self.nsc, NTAB))
+        self.k = None if k is None else np.ascontiguousarray(np.asarray(k, dtype=np.float64).reshape(self.nsc, NTAB))
+        self.names = names or [f"s{q}" for q in range(self.nsc)]
+
+    def as_struct(self) -> Tables:
+        t = Tables()
+        t.nsc = self.nsc
+        t.wm = dptr(self.wm); t.cp = dptr(self.cp); t.h = dptr(self.h)
+        t.mu = dptr(self.mu); t.k = dptr(self.k)
+        return t
+
+
+class Context:
+    def __init__(self, tables: ThermoTables, device: int = 0):
+        self.tables = tables
+        self._t = tables.as_struct()
+        h = C.c_void_p()
+        rc = lib().mose_ctx_create(device, C.byref(self._t), C.byref(h))
+        if rc != MOSE_OK:
+            raise MoseError(rc, "mose_ctx_create failed")
+        self.h = h
+
+    def check(self, rc, what):
+        if rc != MOSE_OK:
+            raise MoseError(rc, f"{what}: {lib().mose_last_error(self.h).decode()}")
+        return rc
+
+    def close(self):
+        if self.h:
+            lib().mose_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceCase:
+    """A case resident on the device (mose_case_create + upload/step/download)."""
+
+    def __init__(self, ctx: Context, case):
+        self.ctx = ctx
+        self.case = case
+        g = case.geom
+        cid = C.c_int(-1)
+        ctx.check(lib().mose_case_create(ctx.h, C.byref(case.scheme), case.layout.nb, iptr(case.layout.dims_arr),
+                                         dptr(g.dl), dptr(g.m), dptr(g.vol), dptr(g.yn), dptr(g.fm1), dptr(g.fm2),
+                                         dptr(g.fm3), len(case.records),
+                                         case.records.ctypes.data_as(C.c_void_p), C.byref(cid)), "mose_case_create")
+        self.id = cid.value
+
+    def upload(self, P: np.ndarray):
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        self.ctx.check(lib().mose_state_upload(self.ctx.h, self.id, dptr(P)), "mose_state_upload")
+
+    def download(self) -> np.ndarray:
+        P = np.zeros((self.case.nprim, self.case.layout.NC))
+        self.ctx.check(lib().mose_state_download(self.ctx.h, self.id, dptr(P)), "mose_state_download")
+        return P
+
+    def step(self, nsteps: int) -> Report:
+        rep = Report()
+        self.ctx.check(lib().mose_case_step(self.ctx.h, self.id, nsteps, C.byref(rep)), "mose_case_step")
+        return rep
+
+    def residual(self) -> np.ndarray:
+        R = np.zeros((self.case.nprim, self.case.layout.NC))
+        self.ctx.check(lib().mose_case_residual(self.ctx.h, self.id, dptr(R)), "mose_case_residual")
+        return R
+
+    def dt(self) -> float:
+        v = C.c_double(0.0)
+        self.ctx.check(lib().mose_case_dt(self.ctx.h, self.id, C.byref(v)), "mose_case_dt")
+        return v.value
+
+    def destroy(self):
+        if self.id >= 0:
+            lib().mose_case_destroy(self.ctx.h, self.id)
+            self.id = -1

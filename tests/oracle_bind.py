@@ -1,0 +1,102 @@
+"""ctypes binding of the CPU oracle (oracle/libmoseoracle.so) for the tests -- the checker, never the product."""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+REPO = Path(__file__).resolve().parent.parent
+ORACLE_DIR = REPO / "oracle"
+ORACLE_LIB = ORACLE_DIR / "libmoseoracle.so"
+
+_L = None
+
+
+def oracle_lib():
+    global _L
+    if _L is not None:
+        return _L
+    if not ORACLE_LIB.exists():
+        subprocess.run(["make", "-C", str(ORACLE_DIR)], check=True, capture_output=True)
+    L = C.CDLL(str(ORACLE_LIB))
+    vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.c_int
+    L.oa_create.restype = vp
+    L.oa_create.argtypes = [vp, vp, ip, C.POINTER(C.c_int), dp, dp, dp, dp, dp, dp, dp, ip, vp, ip]
+    L.oa_destroy.argtypes = [vp]
+    L.oa_ncell.restype = C.c_int64
+    L.oa_ncell.argtypes = [vp]
+    L.oa_fill_ghosts.argtypes = [vp, dp]
+    L.oa_residual.argtypes = [vp, dp, dp]
+    L.oa_dt.restype = C.c_double
+    L.oa_dt.argtypes = [vp, dp]
+    L.oa_newstate.argtypes = [vp, dp, dp, dp, C.c_double, C.c_double]
+    L.oa_step.restype = C.c_int
+    L.oa_step.argtypes = [vp, dp, ip, dp]
+    L.oa_limiter.restype = C.c_double
+    L.oa_limiter.argtypes = [ip, C.c_double, C.c_double]
+    L.oa_conv_face.argtypes = [vp, dp, dp, dp, dp, dp, dp, C.c_double, dp]
+    L.oa_riemann.argtypes = [vp, ip, dp, dp, dp, dp]
+    L.oa_sound_speed.restype = C.c_double
+    L.oa_sound_speed.argtypes = [vp, dp, C.c_double]
+    _L = L
+    return L
+
+
+def _d(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class OracleCase:
+    """The oracle instantiated on the same scheme, tables, geometry and records as a product Case."""
+
+    def __init__(self, case, nthreads: int = 8):
+        L = oracle_lib()
+        self.case = case
+        self._t = case.tables.as_struct()
+        g = case.geom
+        self._keep = (g,)
+        self.h = L.oa_create(C.cast(C.pointer(case.scheme), C.c_void_p), C.cast(C.pointer(self._t), C.c_void_p),
+                             case.layout.nb, case.layout.dims_arr.ctypes.data_as(C.POINTER(C.c_int)),
+                             _d(g.dl), _d(g.m), _d(g.vol), _d(g.yn), _d(g.fm1), _d(g.fm2), _d(g.fm3),
+                             len(case.records), case.records.ctypes.data_as(C.c_void_p), nthreads)
+        assert self.h, "oa_create failed"
+
+    def step(self, P: np.ndarray, nsteps: int):
+        dts = np.zeros(max(nsteps, 1))
+        oracle_lib().oa_step(self.h, _d(P), nsteps, _d(dts))
+        return dts[:nsteps]
+
+    def residual(self, P: np.ndarray) -> np.ndarray:
+        R = np.zeros_like(P)
+        oracle_lib().oa_residual(self.h, _d(P), _d(R))
+        return R
+
+    def dt(self, P: np.ndarray) -> float:
+        return oracle_lib().oa_dt(self.h, _d(P))
+
+    def fill_ghosts(self, P: np.ndarray):
+        oracle_lib().oa_fill_ghosts(self.h, _d(P))
+
+    def riemann(self, rs: int, qL, qR, n):
+        F = np.zeros(5)
+        oracle_lib().oa_riemann(self.h, rs, _d(np.ascontiguousarray(qL, dtype=float)),
+                                _d(np.ascontiguousarray(qR, dtype=float)), _d(np.ascontiguousarray(n, dtype=float)),
+                                _d(F))
+        return F
+
+    def conv_face(self, pm1, p0, pp1, pp2, dlc, nrm, area):
+        res = np.zeros(len(p0))
+        a = [np.ascontiguousarray(x, dtype=float) for x in (pm1, p0, pp1, pp2, dlc, nrm)]
+        oracle_lib().oa_conv_face(self.h, *[_d(x) for x in a], float(area), _d(res))
+        return res
+
+    def sound_speed(self, rhoi, p):
+        return oracle_lib().oa_sound_speed(self.h, _d(np.ascontiguousarray(rhoi, dtype=float)), float(p))
+
+    def __del__(self):
+        try:
+            oracle_lib().oa_destroy(self.h)
+        except Exception:
+            pass

@@ -1,0 +1,106 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle on the same inputs.
+
+Bar: relative L-infinity <= 1e-12 on the conservative variables after N steps (the tolerance of the residual path);
+both sides are built without FMA contraction and keep the reference's rounding order, so the results are expected
+to be bitwise equal, which the tests also report."""
+import numpy as np
+import pytest
+
+from mosegpu import Context, DeviceCase, cases
+from oracle_bind import OracleCase
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def conservative(case, P):
+    out = []
+    for b in range(case.layout.nb):
+        V = case.layout.interior(P, b).reshape(case.nprim, -1)
+        ns = case.scheme.nsc
+        rho = V[:ns].sum(axis=0)
+        u, v, w, p = V[ns], V[ns + 1], V[ns + 2], V[ns + 3]
+        # E with the table h = cp T of the air cases: e = cv T = p / (rho (gamma - 1)) up to the table
+        E = p / (case.meta.get("gamma", 1.4) - 1.0) + 0.5 * rho * (u * u + v * v + w * w)
+        out.append(np.vstack([V[:ns], rho * u, rho * v, rho * w, E, V[ns + 4:]]))
+    return np.hstack(out)
+
+
+def rel_linf(a, b):
+    d = np.abs(a - b).max(axis=1)
+    s = np.abs(b).max(axis=1)
+    s[s == 0] = 1.0
+    return (d / s).max()
+
+
+def run_pair(case, nsteps):
+    ctx = Context(case.tables)
+    dc = DeviceCase(ctx, case)
+    dc.upload(case.P)
+    rep = dc.step(nsteps)
+    Pg = dc.download()
+    Po = case.P.copy()
+    dts = OracleCase(case).step(Po, nsteps)
+    return Pg, Po, rep, dts, ctx
+
+
+def test_sod_parity(gpu_ready):
+    case = cases.sod(1000, 200)
+    case.meta["gamma"] = 1004.5 / (1004.5 - 8314.46261815324 / 28.970418)
+    Pg, Po, rep, dts, _ = run_pair(case, 200)
+    assert abs(rep.time - dts.sum()) <= 1e-13 * dts.sum()
+    err = rel_linf(conservative(case, Pg), conservative(case, Po))
+    nbit = int(np.sum(case.layout.interior(Pg, 0) != case.layout.interior(Po, 0)))
+    print(f"sod 1000 x 200 steps: rel Linf {err:.3e}, cells differing bitwise {nbit}")
+    assert err <= TOL
+
+
+def test_sod_residual_and_dt(gpu_ready):
+    case = cases.sod(200, 0)
+    ctx = Context(case.tables)
+    dc = DeviceCase(ctx, case)
+    dc.upload(case.P)
+    Rg = dc.residual()
+    Po = case.P.copy()
+    o = OracleCase(case)
+    Ro = o.residual(Po)
+    Ig, Io = case.layout.interior(Rg, 0), case.layout.interior(Ro, 0)
+    assert np.abs(Ig - Io).max() <= TOL * np.abs(Io).max()
+    assert dc.dt() == o.dt(case.P.copy())
+
+
+@pytest.mark.parametrize("n,steps", [(16, 10), (24, 4)])
+def test_taylor_green_parity(gpu_ready, n, steps):
+    case = cases.taylor_green(n, steps)
+    case.meta["gamma"] = 1004.5 / (1004.5 - 8314.46261815324 / 28.970418)
+    Pg, Po, rep, dts, _ = run_pair(case, steps)
+    assert abs(rep.time - dts.sum()) <= 1e-13 * dts.sum()
+    err = rel_linf(conservative(case, Pg), conservative(case, Po))
+    nbit = int(np.sum(case.layout.interior(Pg, 0) != case.layout.interior(Po, 0)))
+    print(f"taylor-green {n}^3 x {steps} steps: rel Linf {err:.3e}, cells differing bitwise {nbit}")
+    assert err <= TOL
+
+
+def test_taylor_green_residual(gpu_ready):
+    case = cases.taylor_green(20, 0, dims=(20, 12, 16))
+    ctx = Context(case.tables)
+    dc = DeviceCase(ctx, case)
+    dc.upload(case.P)
+    Rg = dc.residual()
+    Ro = OracleCase(case).residual(case.P.copy())
+    Ig, Io = case.layout.interior(Rg, 0), case.layout.interior(Ro, 0)
+    for s in range(5):
+        assert np.abs(Ig[s] - Io[s]).max() <= TOL * np.abs(Io[s]).max(), s
+
+
+def test_gpu_errors_are_reported(gpu_ready):
+    case = cases.sod(64, 1)
+    ctx = Context(case.tables)
+    dc = DeviceCase(ctx, case)
+    P = case.P.copy()
+    case.layout.interior(P, 0)[4, 0, 0, 40] = -5.0   # negative pressure in one cell: the update fails the state check
+    dc.upload(P)
+    with pytest.raises(Exception) as ei:
+        dc.step(1)
+    assert "MOSE_ERR_STATE" in str(ei.value)
