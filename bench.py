@@ -1,0 +1,128 @@
+#!/usr/bin/env python3
+"""Benchmark of the residual path on one MI355X: the 3D compressible Taylor-Green vortex, 256^3 periodic cells,
+M = 0.1, Re = 1600, Roe (LMRoe, Mco = 1) + van Albada MUSCL, central viscous fluxes with Sutherland viscosity,
+SSP-RK3, FP64.  One "step" = one SSP-RK3 step of all 16.8 M cells (three fused stage kernels).
+
+Prints one JSON line: value = cell-updates/s (cells x steps / s) with the state resident in HBM; roofline of the
+fused stage kernel from HIP events on its own stream (algorithmic bytes per cell-stage 8 (3 nprim + 25) = 320 B);
+cpu_baseline = the CPU oracle (a C restatement of the reference's double-precision path) on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+REPO = Path(__file__).resolve().parent
+sys.path.insert(0, str(REPO / "mose-gpu_amd"))
+
+METRIC = "FP64 cell-updates/s & %HBM roofline; served cases/s + p50/p99 at ≤1e-12 err"
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes_per_cell_stage(nprim: int, viscous: bool, rans: bool) -> int:
+    # read the stage input, read the step base, write the stage output (3 nprim) + geometry read once per cell:
+    # 12 face normals/areas + 3 lengths + 1 volume (+ 9 metric if viscous, + 1 wall distance if RANS)
+    G = 16 + (9 if viscous else 0) + (1 if rans else 0)
+    return 8 * (3 * nprim + G)
+
+
+def cpu_baseline(n: int, steps: int, threads: int):
+    sys.path.insert(0, str(REPO / "tests"))
+    from mosegpu import cases
+    from oracle_bind import OracleCase
+    case = cases.taylor_green(n, steps)
+    o = OracleCase(case, nthreads=threads)
+    P = case.P.copy()
+    o.step(P, 1)                       # warm-up step
+    t0 = time.perf_counter()
+    o.step(P, steps)
+    dt = time.perf_counter() - t0
+    return {"value": n ** 3 * steps / dt, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"taylor-green {n}^3, {steps} SSP-RK3 steps after 1 warm-up step, CPU oracle "
+                      f"(gcc -O2 -ffp-contract=off, OpenMP {threads} threads), {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--path", choices=["fused", "plain"], default="fused")
+    ap.add_argument("--tile", default="auto", help="TIxTJxTK or auto")
+    ap.add_argument("--cpu-n", type=int, default=48)
+    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--probe-bw", action="store_true", help="only run the stream-copy probe (1 GiB buffers)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (the multi-GPU sharding "
+              f"of the residual path is a design note, not code)", file=sys.stderr)
+        sys.exit(2)
+
+    import mosegpu
+    from mosegpu import cases
+
+    if args.probe_bw:
+        ctx = mosegpu.Context(cases.air_tables())
+        gbs = ctx.bandwidth_probe(1 << 30, 20)
+        print(json.dumps({"probe": "stream copy", "bytes_per_launch_read": 1 << 30, "bytes_per_launch_written": 1 << 30,
+                          "launches": 21, "GBs": gbs}))
+        return
+
+    t_setup = time.perf_counter()
+    case = cases.taylor_green(args.n, args.steps)
+    ctx = mosegpu.Context(case.tables)
+    dc = mosegpu.DeviceCase(ctx, case)
+    dc.upload(case.P)
+    if args.tile == "auto":
+        tile = (0, 0, 0)
+    else:
+        tile = tuple(int(x) for x in args.tile.split("x"))
+    path = mosegpu.PATH_FUSED if args.path == "fused" else mosegpu.PATH_PLAIN
+    chosen = dc.tune(path, tile, timing=False)
+    t_setup = time.perf_counter() - t_setup
+    print(f"bench: case built and tuned in {t_setup:.1f} s, tile {chosen}", file=sys.stderr, flush=True)
+    if args.warmup > 0:
+        dc.step(args.warmup)
+    dc.tune(path, chosen if path == mosegpu.PATH_FUSED else (0, 0, 0), timing=True)
+    dc.step(0)                          # drains the stream
+    t0 = time.perf_counter()
+    rep = dc.step(args.steps)           # returns after the device has finished (the report is read back)
+    elapsed = time.perf_counter() - t0
+    ms_avg, ms_tot, nlaunch = dc.timing()
+    print(f"bench: {args.steps} steps in {elapsed:.3f} s, stage kernel {ms_avg:.3f} ms avg", file=sys.stderr, flush=True)
+    cells = case.interior_cells()
+    value = cells * args.steps / elapsed
+    B = algorithmic_bytes_per_cell_stage(case.nprim, True, False)
+    achieved = (B * cells) / (ms_avg * 1e-3) / 1e9 if ms_avg > 0 else None
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+                "kernel": "k_stage (fused residual + RK stage update)" if path == mosegpu.PATH_FUSED
+                          else "face kernels + cell update (one stage)",
+                "kernel_ms_avg": ms_avg, "launches": nlaunch, "bytes_per_cell_stage": B}
+    out = {
+        "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"taylor-green {args.n}^3 periodic, M 0.1, Re 1600, LMRoe(Mco=1) + van Albada, "
+                               f"viscous (Sutherland table), SSP-RK3", "cells": cells, "path": args.path,
+                   "tile": list(chosen), "parallelism": "single GPU"},
+        "roofline": roofline,
+        "sim_time": rep.time, "dt_last": rep.dt, "setup_s": t_setup, "device": ctx.info(),
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_steps, min(16, os.cpu_count() or 1))
+    else:
+        out["cpu_baseline"] = None
+    dc.destroy()
+    ctx.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

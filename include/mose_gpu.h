@@ -120,6 +120,9 @@ int mose_ctx_create(int device, const mose_tables* tables, mose_ctx** out);
 int mose_ctx_destroy(mose_ctx* ctx);
 const char* mose_last_error(const mose_ctx* ctx);
 int mose_device_count(int* n);
+/* Device facts the host sizes work by: HBM bytes (total, free), compute units, LDS bytes a workgroup may take.
+ * No reference counterpart (the serving loop admits cases by HBM budget). */
+int mose_ctx_info(mose_ctx* ctx, int64_t* hbm_total, int64_t* hbm_free, int* cu_count, int* lds_per_block);
 
 /* Geometry of a multi-block mesh from its nodes (Setup_Metrics, Lib_Metrics.f90:15-136).  Host only.
  * yn may be NULL; ndir as Check_Mesh_Type (Lib_Metrics.f90:139-196). */
@@ -165,8 +168,27 @@ int mose_case_residual_norm(mose_ctx* ctx, int case_id, double* out);
 /* Order-independent hash of the interior state (field_state_hash, MOSE_Field_m.f90:254-284). */
 int mose_case_state_hash(mose_ctx* ctx, int case_id, uint64_t* h_lo, uint64_t* h_hi);
 
-/* Average device time of the last timed step call per kernel family (ms): residual kernel, halo, dt. */
-int mose_case_timing(mose_ctx* ctx, int case_id, double* ms_residual, double* ms_halo, double* ms_other);
+/* Execution path and tile shape of a case.  path: MOSE_PATH_FUSED (default) runs one fused stage kernel per RK
+ * stage over the tiles of every block; MOSE_PATH_PLAIN runs the per-block face kernels + cell update (the
+ * per-block launch structure of the reference, kept as the comparison path).  ti = tj = tk = 0 autotunes the
+ * tile (each candidate timed on the case itself, the state left unchanged).  timing = 1 records HIP events around
+ * every stage kernel of the following step calls (mose_case_timing).  No reference counterpart (launch tuning). */
+enum { MOSE_PATH_FUSED = 0, MOSE_PATH_PLAIN = 1 };
+typedef struct {
+  int32_t path;
+  int32_t ti, tj, tk;
+  int32_t timing;
+  int32_t reserved[3];
+} mose_tuning;
+int mose_case_tune(mose_ctx* ctx, int case_id, const mose_tuning* tuning, mose_tuning* chosen);
+
+/* Device time of the stage kernels of the last timed step call: average per launch, total, launch count. */
+int mose_case_timing(mose_ctx* ctx, int case_id, double* ms_stage_avg, double* ms_stage_total, int64_t* launches);
+
+/* Stream-copy probe: copies `bytes` (read `bytes` + write `bytes`) `reps` times with the library's copy kernel and
+ * returns the bandwidth in GB/s (read + written bytes over the kernel time).  Calibrates the HBM traffic counters
+ * against a kernel of known traffic and measures the box's achievable bandwidth.  No reference counterpart. */
+int mose_bandwidth_probe(mose_ctx* ctx, int64_t bytes, int reps, double* gbs);
 
 #ifdef __cplusplus
 }

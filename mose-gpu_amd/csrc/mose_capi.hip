@@ -18,6 +18,7 @@
 #include "mose_gpu.h"
 #include "mose_internal.h"
 #include "mose_kernels.hpp"
+#include "mose_stage.hpp"
 
 using namespace mose;
 
@@ -48,13 +49,25 @@ struct Case {
   int64_t steps = 0;
   bool valid = false;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-  float ms_res = 0.f, ms_halo = 0.f, ms_other = 0.f;
+  // fused path
+  int path = MOSE_PATH_FUSED;
+  int TI = 32, TJ = 4, TK = 2;
+  Tile* dtiles = nullptr;
+  int ntiles = 0, tiles_cap = 0;
+  unsigned long long* dtnext = nullptr;
+  bool dt_valid = false;
+  // timing
+  bool timing = false;
+  std::vector<hipEvent_t> tev;
+  double ms_stage_total = 0.0;
+  int64_t stage_launches = 0;
 };
 
 }  // namespace
 
 struct mose_ctx {
   int device = 0;
+  int max_lds = 65536;       // dynamic LDS a workgroup may take on this device
   hipStream_t stream = nullptr;
   int nsc = 0;
   bool viscous_tables = false;
@@ -86,6 +99,20 @@ extern "C" int mose_device_count(int* n) {
   return MOSE_OK;
 }
 
+extern "C" int mose_ctx_info(mose_ctx* ctx, int64_t* total, int64_t* freeb, int* cus, int* lds) {
+  if (!ctx) return MOSE_ERR_ARG;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  size_t f = 0, t = 0;
+  HIPCHK(ctx, hipMemGetInfo(&f, &t));
+  int n = 0;
+  HIPCHK(ctx, hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, ctx->device));
+  if (total) *total = (int64_t)t;
+  if (freeb) *freeb = (int64_t)f;
+  if (cus) *cus = n;
+  if (lds) *lds = ctx->max_lds;
+  return MOSE_OK;
+}
+
 extern "C" const char* mose_last_error(const mose_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
 
 extern "C" int mose_ctx_create(int device, const mose_tables* t, mose_ctx** out) {
@@ -99,6 +126,11 @@ extern "C" int mose_ctx_create(int device, const mose_tables* t, mose_ctx** out)
   ctx->nsc = t->nsc;
   HIPCHK(ctx.get(), hipSetDevice(device));
   HIPCHK(ctx.get(), hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && v > 0)
+      ctx->max_lds = v;
+  }
   size_t n = (size_t)t->nsc * MOSE_NTAB;
   HIPCHK(ctx.get(), hipMalloc(&ctx->dcp, 4 * n * sizeof(double)));
   ctx->dh = ctx->dcp + n;
@@ -129,6 +161,11 @@ extern "C" int mose_ctx_create(int device, const mose_tables* t, mose_ctx** out)
 static void case_free(Case* c) {
   if (!c) return;
   if (c->slab) (void)hipFree(c->slab);
+  if (c->dtiles) (void)hipFree(c->dtiles);
+  c->dtiles = nullptr;
+  for (auto& e : c->tev)
+    if (e) (void)hipEventDestroy(e);
+  c->tev.clear();
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   c->slab = nullptr;
@@ -248,7 +285,7 @@ static size_t case_bytes(const Case& c) {
   b += (size_t)(c.NF1 + c.NF2 + c.NF3) * 4 * d;            // fm1..3
   for (int q = 0; q < 3; ++q) b += 2 * (size_t)c.NFI[q] * c.nprim * d;   // face buffers
   b += 2 * (size_t)std::max<int64_t>(c.NBP, 1) * c.nprim * d;           // boundary increments
-  b += c.blk.size() * sizeof(mose_blk) + c.rec.size() * sizeof(mose_bc_record) + 4096 + 64 * 1024;
+  b += c.blk.size() * sizeof(mose_blk) + c.rec.size() * sizeof(mose_bc_record) + 8192 + 64 * 1024;
   return b;
 }
 
@@ -310,6 +347,7 @@ static int alloc_case(mose_ctx* ctx, Case& c) {
   c.dtmin = (unsigned long long*)take(64);
   c.bad = (unsigned int*)take(64);
   c.first_bad = (unsigned long long*)take(64);
+  c.dtnext = (unsigned long long*)take(64);
   if ((size_t)(p - (char*)c.slab) > total) return fail(ctx, MOSE_ERR_NOMEM, "internal: slab overflow");
   for (auto& e : c.ev) HIPCHK(ctx, hipEventCreate(&e));
   return MOSE_OK;
@@ -319,6 +357,21 @@ static Case* get_case(mose_ctx* ctx, int id) {
   if (!ctx || id < 0 || id >= (int)ctx->cases.size()) return nullptr;
   Case* c = ctx->cases[id].get();
   return (c && c->valid) ? c : nullptr;
+}
+
+static size_t stage_lds_(int nprim, bool visc, int TI, int TJ, int TK);
+// the first tile of the preference list that fits the device's LDS and does not straddle a one-cell direction
+static void choose_default_tile(mose_ctx* ctx, Case& c) {
+  static const int pref[][3] = {{8, 8, 4}, {16, 4, 4}, {32, 4, 2}, {16, 8, 2}, {32, 8, 1}, {64, 4, 1}, {16, 16, 1},
+                                {128, 2, 1}, {256, 1, 1}};
+  int mx[3] = {0, 0, 0};
+  for (auto& B : c.blk) { mx[0] = std::max(mx[0], B.n1); mx[1] = std::max(mx[1], B.n2); mx[2] = std::max(mx[2], B.n3); }
+  for (auto& t : pref) {
+    if ((t[1] > 1 && mx[1] == 1) || (t[2] > 1 && mx[2] == 1)) continue;
+    if (stage_lds_(c.nprim, c.sc.viscous != 0, t[0], t[1], t[2]) > (size_t)ctx->max_lds) continue;
+    c.TI = t[0]; c.TJ = t[1]; c.TK = t[2];
+    return;
+  }
 }
 
 extern "C" int mose_case_create(mose_ctx* ctx, const mose_scheme* sc, int nb, const int* dims, const double* dl,
@@ -352,6 +405,7 @@ extern "C" int mose_case_create(mose_ctx* ctx, const mose_scheme* sc, int nb, co
   HIPCHK(ctx, hipMemsetAsync(c->P0, 0, 3 * (size_t)c->NC * c->nprim * d, s));
   HIPCHK(ctx, hipStreamSynchronize(s));
   c->valid = true;
+  choose_default_tile(ctx, *c);
   ctx->cases.push_back(std::move(c));
   *case_id = (int)ctx->cases.size() - 1;
   return MOSE_OK;
@@ -373,6 +427,7 @@ extern "C" int mose_state_upload(mose_ctx* ctx, int id, const double* P) {
   size_t n = (size_t)c->NC * c->nprim * sizeof(double);
   HIPCHK(ctx, hipMemcpyAsync(c->P0, P, n, hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  c->dt_valid = false;
   return MOSE_OK;
 }
 
@@ -434,30 +489,202 @@ static void launch_flux(Case& c, const CaseDev& D, hipStream_t s) {
   }
 }
 
+// ----------------------------------------------------------------------------------------------------------------
+// fused path: tiles, launch, autotune
+// ----------------------------------------------------------------------------------------------------------------
+static int tile_slots(int TI, int TJ, int TK) { return (TI + 1) * TJ * TK + TI * (TJ + 1) * TK + TI * TJ * (TK + 1); }
+
+static int hs_slots(int TI, int TJ, int TK) { return (TI + 2) * TJ * TK + TI * (TJ + 2) * TK + TI * TJ * (TK + 2); }
+
+// LDS doubles of region 0 (half-slopes, then the viscous cell cache) and the bytes of the whole tile
+static int stage_r0(const Case& c, int TI, int TJ, int TK) {
+  int r0 = hs_slots(TI, TJ, TK) * c.nprim;
+  if (c.sc.viscous) r0 = std::max(r0, 3 * (TI + 2) * (TJ + 2) * (TK + 2));
+  return (r0 + 31) & ~31;
+}
+static size_t stage_lds(const Case& c, int TI, int TJ, int TK) {
+  return ((size_t)stage_r0(c, TI, TJ, TK) + (size_t)tile_slots(TI, TJ, TK) * c.nprim) * sizeof(double);
+}
+static size_t stage_lds_(int nprim, bool visc, int TI, int TJ, int TK) {
+  Case c;
+  c.nprim = nprim;
+  c.sc.viscous = visc ? 1 : 0;
+  return stage_lds(c, TI, TJ, TK);
+}
+
+static int set_tiles(mose_ctx* ctx, Case& c, int TI, int TJ, int TK) {
+  if (TI < 1 || TJ < 1 || TK < 1 || TI * TJ * TK != 256) return fail(ctx, MOSE_ERR_ARG, "tile must hold 256 cells");
+  if (stage_lds(c, TI, TJ, TK) > (size_t)ctx->max_lds)
+    return fail(ctx, MOSE_ERR_ARG, "tile " + std::to_string(TI) + "x" + std::to_string(TJ) + "x" + std::to_string(TK) +
+                                       " needs " + std::to_string(stage_lds(c, TI, TJ, TK)) + " B of LDS, the device allows " +
+                                       std::to_string(ctx->max_lds));
+  std::vector<Tile> t;
+  for (int b = 0; b < c.nb; ++b) {
+    const mose_blk& B = c.blk[b];
+    for (int k0 = 1; k0 <= B.n3; k0 += TK)
+      for (int j0 = 1; j0 <= B.n2; j0 += TJ)
+        for (int i0 = 1; i0 <= B.n1; i0 += TI) t.push_back(Tile{b, i0, j0, k0});
+  }
+  if ((int)t.size() > c.tiles_cap) {
+    if (c.dtiles) HIPCHK(ctx, hipFree(c.dtiles));
+    c.dtiles = nullptr;
+    HIPCHK(ctx, hipMalloc(&c.dtiles, t.size() * sizeof(Tile)));
+    c.tiles_cap = (int)t.size();
+  }
+  HIPCHK(ctx, hipMemcpyAsync(c.dtiles, t.data(), t.size() * sizeof(Tile), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  c.ntiles = (int)t.size();
+  c.TI = TI; c.TJ = TJ; c.TK = TK;
+  return MOSE_OK;
+}
+
+template <int NS, int NR>
+static void launch_stage(Case& c, const CaseDev& D, hipStream_t s, double rkc, bool last) {
+  StageArgs A;
+  A.tiles = c.dtiles; A.ntiles = c.ntiles; A.TI = c.TI; A.TJ = c.TJ; A.TK = c.TK;
+  A.nslot = tile_slots(c.TI, c.TJ, c.TK);
+  A.nhs = hs_slots(c.TI, c.TJ, c.TK);
+  A.ncache = (c.TI + 2) * (c.TJ + 2) * (c.TK + 2);
+  A.r0 = stage_r0(c, c.TI, c.TJ, c.TK);
+  A.last = last ? 1 : 0; A.rkc = rkc; A.dtnext = c.dtnext;
+  size_t lds = stage_lds(c, c.TI, c.TJ, c.TK);
+  unsigned grid = (unsigned)(((c.ntiles + 7) / 8) * 8);
+  if (c.sc.viscous) k_stage<NS, NR, true><<<grid, 256, lds, s>>>(D, A);
+  else k_stage<NS, NR, false><<<grid, 256, lds, s>>>(D, A);
+}
+
+__global__ void k_step_begin(const double* dt, double* hist, int n) {
+  if (threadIdx.x == 0 && blockIdx.x == 0 && hist) hist[n] = *dt;
+}
+
 template <int NS, int NR>
 static int run_steps(mose_ctx* ctx, Case& c, int nsteps) {
   hipStream_t s = ctx->stream;
   static const double rk[3][3] = {{1.0, 0.0, 0.0}, {1.0, 0.5, 0.0}, {1.0, 0.25, 2.0 / 3.0}};   // Lib_RK.f90:6-9
   int nrk = c.sc.n_rk;
+  bool fused = (c.path == MOSE_PATH_FUSED);
+  if (fused && c.ntiles == 0) {
+    int rc = set_tiles(ctx, c, c.TI, c.TJ, c.TK);
+    if (rc != MOSE_OK) return rc;
+  }
+  if (fused && stage_lds(c, c.TI, c.TJ, c.TK) > (size_t)ctx->max_lds) return fail(ctx, MOSE_ERR_ARG, "tile does not fit");
   HIPCHK(ctx, hipMemsetAsync(c.bad, 0, 2 * sizeof(unsigned), s));
   HIPCHK(ctx, hipMemsetAsync(c.first_bad, 0xff, sizeof(unsigned long long), s));
-  int hist0 = 0;
+  size_t nev = c.timing ? (size_t)2 * nsteps * nrk : 0;
+  while (c.tev.size() < nev) {
+    hipEvent_t e;
+    HIPCHK(ctx, hipEventCreate(&e));
+    c.tev.push_back(e);
+  }
+  size_t iev = 0;
   for (int n = 0; n < nsteps; ++n) {
     CaseDev D0 = make_dev(ctx, c, c.P0, c.PA);
-    launch_dt<NS, NR>(c, D0, s, (n < c.hist_cap) ? n : -1);
+    if (!fused || !c.dt_valid) launch_dt<NS, NR>(c, D0, s, -1);
+    k_step_begin<<<1, 64, 0, s>>>(c.dt, n < c.hist_cap ? c.dthist : nullptr, n < c.hist_cap ? n : 0);
     for (int irk = 1; irk <= nrk; ++irk) {
       double* in = irk == 1 ? c.P0 : (irk == 2 ? c.PA : c.PB);
       double* out = irk == nrk ? c.P0 : (irk == 1 ? c.PA : c.PB);
       CaseDev D = make_dev(ctx, c, in, out);
-      launch_flux<NS, NR>(c, D, s);
-      for (int b = 0; b < c.nb; ++b) {
-        int64_t ncell = (int64_t)c.blk[b].n1 * c.blk[b].n2 * c.blk[b].n3;
-        k_update<NS, NR><<<nblocks(ncell, 256), 256, 0, s>>>(D, b, rk[nrk - 1][irk - 1]);
+      double rkc = rk[nrk - 1][irk - 1];
+      if (fused) {
+        int nrec = (int)c.rec.size();
+        if (nrec > 0) {
+          k_ghost<NS, NR><<<nblocks(nrec, 256), 256, 0, s>>>(D);
+          k_bcrec<NS, NR><<<nblocks(nrec, 128), 128, 0, s>>>(D);
+        }
+        bool last = irk == nrk;
+        if (last) k_dt_init<<<1, 64, 0, s>>>(c.dtnext);
+        if (c.timing) HIPCHK(ctx, hipEventRecord(c.tev[iev++], s));
+        launch_stage<NS, NR>(c, D, s, rkc, last);
+        if (c.timing) HIPCHK(ctx, hipEventRecord(c.tev[iev++], s));
+        if (last) k_dt_final<<<1, 64, 0, s>>>(c.dtnext, c.dt, nullptr, 0);
+      } else {
+        launch_flux<NS, NR>(c, D, s);
+        if (c.timing) HIPCHK(ctx, hipEventRecord(c.tev[iev++], s));
+        for (int b = 0; b < c.nb; ++b) {
+          int64_t ncell = (int64_t)c.blk[b].n1 * c.blk[b].n2 * c.blk[b].n3;
+          k_update<NS, NR><<<nblocks(ncell, 256), 256, 0, s>>>(D, b, rkc);
+        }
+        if (c.timing) HIPCHK(ctx, hipEventRecord(c.tev[iev++], s));
       }
     }
+    c.dt_valid = fused;
   }
-  (void)hist0;
   HIPCHK(ctx, hipGetLastError());
+  if (c.timing && iev > 0) {
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    double tot = 0.0;
+    for (size_t q = 0; q + 1 < iev; q += 2) {
+      float ms = 0.f;
+      HIPCHK(ctx, hipEventElapsedTime(&ms, c.tev[q], c.tev[q + 1]));
+      tot += ms;
+    }
+    c.ms_stage_total = tot;
+    c.stage_launches = (int64_t)(iev / 2);
+  }
+  return MOSE_OK;
+}
+
+template <int NS, int NR>
+static int autotune(mose_ctx* ctx, Case& c, mose_tuning* chosen) {
+  // candidates of 256 cells; each runs the ghost, record and stage kernels of stage 1 (into the scratch buffer PA,
+  // the state P0 untouched) a few times, the fastest is kept
+  static const int cand[][3] = {{32, 4, 2}, {64, 2, 2}, {16, 4, 4}, {16, 8, 2}, {8, 8, 4}, {64, 4, 1}, {32, 8, 1},
+                                {128, 2, 1}, {256, 1, 1}};
+  hipStream_t s = ctx->stream;
+  hipEvent_t e0, e1;
+  HIPCHK(ctx, hipEventCreate(&e0));
+  HIPCHK(ctx, hipEventCreate(&e1));
+  int dims_max[3] = {0, 0, 0};
+  for (auto& B : c.blk) { dims_max[0] = std::max(dims_max[0], B.n1); dims_max[1] = std::max(dims_max[1], B.n2); dims_max[2] = std::max(dims_max[2], B.n3); }
+  CaseDev D0 = make_dev(ctx, c, c.P0, c.PA);
+  launch_dt<NS, NR>(c, D0, s, -1);
+  double best = 1e30;
+  int bi = 0;
+  for (int q = 0; q < (int)(sizeof cand / sizeof cand[0]); ++q) {
+    int TI = cand[q][0], TJ = cand[q][1], TK = cand[q][2];
+    if (stage_lds(c, TI, TJ, TK) > (size_t)ctx->max_lds) continue;
+    if ((TJ > 1 && dims_max[1] == 1) || (TK > 1 && dims_max[2] == 1)) continue;   // no tile across a 1-cell direction
+    if (set_tiles(ctx, c, TI, TJ, TK) != MOSE_OK) continue;
+    CaseDev D = make_dev(ctx, c, c.P0, c.PA);
+    int nrec = (int)c.rec.size();
+    if (nrec > 0) {
+      k_ghost<NS, NR><<<nblocks(nrec, 256), 256, 0, s>>>(D);
+      k_bcrec<NS, NR><<<nblocks(nrec, 128), 128, 0, s>>>(D);
+    }
+    launch_stage<NS, NR>(c, D, s, 1.0, false);   // warm
+    HIPCHK(ctx, hipEventRecord(e0, s));
+    for (int r = 0; r < 3; ++r) launch_stage<NS, NR>(c, D, s, 1.0, false);
+    HIPCHK(ctx, hipEventRecord(e1, s));
+    HIPCHK(ctx, hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(ctx, hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best) { best = ms; bi = q; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  HIPCHK(ctx, hipGetLastError());
+  int rc = set_tiles(ctx, c, cand[bi][0], cand[bi][1], cand[bi][2]);
+  if (rc != MOSE_OK) return rc;
+  c.dt_valid = false;
+  if (chosen) { chosen->ti = c.TI; chosen->tj = c.TJ; chosen->tk = c.TK; }
+  return MOSE_OK;
+}
+
+extern "C" int mose_case_tune(mose_ctx* ctx, int id, const mose_tuning* t, mose_tuning* chosen) {
+  Case* c = get_case(ctx, id);
+  if (!c || !t) return MOSE_ERR_ARG;
+  if (t->path != MOSE_PATH_FUSED && t->path != MOSE_PATH_PLAIN) return fail(ctx, MOSE_ERR_ARG, "unknown path");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  c->path = t->path;
+  c->timing = t->timing != 0;
+  c->dt_valid = false;
+  if (chosen) *chosen = *t;
+  if (c->path == MOSE_PATH_FUSED) {
+    if (t->ti == 0 && t->tj == 0 && t->tk == 0)
+      return c->sc.nrans == 0 ? autotune<1, 0>(ctx, *c, chosen) : autotune<1, 1>(ctx, *c, chosen);
+    return set_tiles(ctx, *c, t->ti, t->tj, t->tk);
+  }
   return MOSE_OK;
 }
 
@@ -561,11 +788,48 @@ extern "C" int mose_case_state_hash(mose_ctx* ctx, int id, uint64_t* lo, uint64_
   return fail(ctx, MOSE_ERR_ARG, "mose_case_state_hash: not implemented in this build");
 }
 
-extern "C" int mose_case_timing(mose_ctx* ctx, int id, double* a, double* b, double* c2) {
+extern "C" int mose_case_timing(mose_ctx* ctx, int id, double* avg, double* total, int64_t* launches) {
   Case* c = get_case(ctx, id);
   if (!c) return MOSE_ERR_ARG;
-  if (a) *a = c->ms_res;
-  if (b) *b = c->ms_halo;
-  if (c2) *c2 = c->ms_other;
+  if (avg) *avg = c->stage_launches > 0 ? c->ms_stage_total / (double)c->stage_launches : 0.0;
+  if (total) *total = c->ms_stage_total;
+  if (launches) *launches = c->stage_launches;
+  return MOSE_OK;
+}
+
+// ----------------------------------------------------------------------------------------------------------------
+// stream-copy probe
+// ----------------------------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_copy(const double2* __restrict__ a, double2* __restrict__ b, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) b[i] = a[i];
+}
+
+extern "C" int mose_bandwidth_probe(mose_ctx* ctx, int64_t bytes, int reps, double* gbs) {
+  if (!ctx || bytes < 4096 || reps < 1 || !gbs) return MOSE_ERR_ARG;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  int64_t n = bytes / 16;
+  double2 *a = nullptr, *b = nullptr;
+  HIPCHK(ctx, hipMalloc(&a, (size_t)n * 16));
+  if (hipMalloc(&b, (size_t)n * 16) != hipSuccess) { (void)hipFree(a); return fail(ctx, MOSE_ERR_NOMEM, "probe buffers"); }
+  hipStream_t s = ctx->stream;
+  (void)hipMemsetAsync(a, 0, (size_t)n * 16, s);
+  hipEvent_t e0, e1;
+  HIPCHK(ctx, hipEventCreate(&e0));
+  HIPCHK(ctx, hipEventCreate(&e1));
+  unsigned grid = 256 * 64;
+  k_copy<<<grid, 256, 0, s>>>(a, b, n);
+  HIPCHK(ctx, hipEventRecord(e0, s));
+  for (int r = 0; r < reps; ++r) k_copy<<<grid, 256, 0, s>>>(a, b, n);
+  HIPCHK(ctx, hipEventRecord(e1, s));
+  HIPCHK(ctx, hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIPCHK(ctx, hipEventElapsedTime(&ms, e0, e1));
+  *gbs = 2.0 * (double)n * 16.0 * reps / (ms * 1e-3) / 1e9;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
   return MOSE_OK;
 }

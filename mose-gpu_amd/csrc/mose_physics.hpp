@@ -386,6 +386,21 @@ __device__ __forceinline__ F5 riemann(const mose_dtab& T, int rs, double mco, bo
 // interior convective face: conv_physics.inc:4-179.  q* = the 4-cell stencil (cells i-1, i, i+1, i+2 of face
 // i+1/2), dlc = their lengths along the face direction, res = +Flux of the face.
 // ---------------------------------------------------------------------------------------------------------
+// The limited half-slope of one cell along one direction: lim(s(c,c+1), s(c-1,c)) * (dl_c / 2), with the slopes
+// s(a,b) = (q_b - q_a) * (1 / (0.5 (dl_a + dl_b))) of conv_physics.inc:5-17.  The left state of face c+1/2 is
+// q_c + delta_c and the right state of face c-1/2 is q_c - delta_c: both faces use the same product
+// `beta * lim(slope1, slope0) * (0.5 dl(0))` of the reference, so it is computed once per cell.
+__device__ __forceinline__ double half_slope(int lim, double qm, double q0, double qp, double idlm, double idlp,
+                                             double hdl) {
+  double s0 = (q0 - qm) * idlm, s1 = (qp - q0) * idlp;
+  return 1.0 * limiter(lim, s1, s0) * hdl;
+}
+
+// The flux of an interior face from its reconstructed states L (left) and R (right): conv_physics.inc:34-179
+template <int NS, int NR>
+__device__ __forceinline__ void conv_face_lr(const mose_dtab& T, int rs, double mco, double* L, double* R, double nx,
+                                             double ny, double nz, double area, double* res);
+
 template <int NS, int NR>
 __device__ __forceinline__ void conv_face(const mose_dtab& T, int lim, int rs, double mco, const double* qm,
                                           const double* q0, const double* q1, const double* q2, const double* dlc,
@@ -401,6 +416,13 @@ __device__ __forceinline__ void conv_face(const mose_dtab& T, int lim, int rs, d
     L[s] = q0[s] + 1.0 * limiter(lim, s1, s0) * lm;
     R[s] = q1[s] - 1.0 * limiter(lim, s2, s1) * lp;
   }
+  conv_face_lr<NS, NR>(T, rs, mco, L, R, nx, ny, nz, area, res);
+}
+
+template <int NS, int NR>
+__device__ __forceinline__ void conv_face_lr(const mose_dtab& T, int rs, double mco, double* L, double* R, double nx,
+                                             double ny, double nz, double area, double* res) {
+  using V = Vars<NS, NR>;
 #pragma unroll
   for (int s = 0; s < NS; ++s) { L[s] = dmax2(L[s], 1e-20); R[s] = dmax2(R[s], 1e-20); }
   double rL = rho_sum<NS>(L), rR = rho_sum<NS>(R), RL = 0.0, RR = 0.0;

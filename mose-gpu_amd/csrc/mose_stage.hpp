@@ -1,0 +1,316 @@
+// The fused stage kernel: one workgroup per tile of TI x TJ x TK cells of any block of the case.
+//
+//   phase A  every interior face of the tile (TI+1)TJTK + TI(TJ+1)TK + TITJ(TK+1): the convective flux into LDS;
+//            the per-cell (rho, 1/rho, T) of the tile and its one-cell halo into LDS (viscous cases)
+//   phase B  each cell sums its six convective faces in the fixed order of pass2_body.inc
+//   phase C  every interior face of the tile: the viscous flux into the same LDS buffer
+//   phase D  each cell subtracts its six viscous faces (pass2_body.inc), adds the boundary-record increments in face
+//            order 1..6, and applies the RK stage update (newstate_cons_body.inc) into the output buffer;
+//            on the last stage it also takes the CFL/VNN limit of its new state for the next step's dt.
+//
+// Each face is computed once per tile (the faces on a tile edge twice: once by each tile), no face buffer goes to
+// HBM, and every sum keeps the reference's order, so the result is bitwise that of the per-face kernels.
+// The tiles of a case are listed in device memory (block, i0, j0, k0); the workgroup -> tile map deals a
+// contiguous run of tiles to each XCD (workgroups are handed to the 8 XCDs round robin), so neighbouring tiles
+// share the XCD's L2.
+#pragma once
+
+#include "mose_kernels.hpp"
+
+namespace mose {
+
+struct Tile {
+  int b, i0, j0, k0;
+};
+
+struct StageArgs {
+  const Tile* tiles;
+  int ntiles;
+  int TI, TJ, TK;
+  int nslot;          // faces per tile (LDS slots)
+  int nhs;            // half-slope slots: (TI+2)TJTK + TI(TJ+2)TK + TITJ(TK+2)
+  int ncache;         // (TI+2)(TJ+2)(TK+2) for the viscous cell cache
+  int r0;             // doubles of LDS region 0 (half slopes, then the viscous cell cache)
+  int last;           // 1 on the last RK stage: take the dt limit of the new state
+  double rkc;
+  unsigned long long* dtnext;
+};
+
+__device__ __forceinline__ int xcd_tile(int wg, int n) {
+  // workgroup wg runs on XCD wg % 8; give XCD x the tiles [x*n/8, (x+1)*n/8) in order
+  int x = wg & 7, q = wg >> 3;
+  int per = n >> 3, rem = n & 7;
+  int base = x * per + (x < rem ? x : rem);
+  int cnt = per + (x < rem ? 1 : 0);
+  return q < cnt ? base + q : -1;
+}
+
+// One interior viscous face of direction D (diff_cuf_i/j/k + diff_physics.inc) with the cell values (rho, 1/rho, T)
+// taken from the tile's LDS cache; (i,j,k) = the face's left cell, (a,b,c) = its tile-local face coordinates.
+template <int NS, int NR, int D>
+__device__ __forceinline__ void visc_face(const CaseDev& C, const mose_blk& B, const Tile& tl, const StageArgs& A,
+                                          const double* cache, int i, int j, int k, int a, int b, int c, double* res) {
+  using V = Vars<NS, NR>;
+  constexpr int NP = V::nprim;
+  constexpr int ib = D == 0 ? 1 : 0, ic = D == 2 ? 1 : 2;
+  const double* P = C.Pin;
+  const int64_t NC = C.NC;
+  const mose_dtab& T = *C.tab;
+  const int CI = A.TI + 2, CJ = A.TJ + 2, NCa = A.ncache;
+  const int64_t sa[3] = {1, (int64_t)B.pi, (int64_t)B.pi * B.pj};
+  const int64_t st = sa[D], sb = sa[ib], sc = sa[ic];
+  const int ca[3] = {1, CI, CI * CJ};
+  const int kst = ca[D], kb = ca[ib], kc = ca[ic];
+  // cache index of the left cell: tile-local + 1 (halo), one back along D
+  int k1 = (a + 1 - (D == 0)) + CI * ((b + 1 - (D == 1)) + CJ * (c + 1 - (D == 2)));
+  int k2 = k1 + kst;
+  int64_t c0 = row(B, i, j, k), c1 = c0 + st;
+  double q0[NP], q1[NP], G[NP][3];
+#pragma unroll
+  for (int s = 0; s < NP; ++s) { q0[s] = P[s * NC + c0]; q1[s] = P[s * NC + c1]; }
+  double ir1 = cache[NCa + k1], ir2 = cache[NCa + k2];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) G[s][D] = q1[s] * ir2 - q0[s] * ir1;
+  G[V::np][D] = cache[2 * NCa + k2] - cache[2 * NCa + k1];
+#pragma unroll
+  for (int s = V::nu; s <= V::nw; ++s) G[s][D] = q1[s] - q0[s];
+#pragma unroll
+  for (int s = V::nt; s < NP; ++s) G[s][D] = q1[s] * ir2 - q0[s] * ir1;
+#pragma unroll
+  for (int pass = 0; pass < 2; ++pass) {
+    const int ax = pass ? ic : ib;
+    const int64_t sd = pass ? sc : sb;
+    const int kd = pass ? kc : kb;
+    int64_t a3 = c0 - sd, a4 = c0 + sd, a5 = c1 - sd, a6 = c1 + sd;
+    int k3 = k1 - kd, k4 = k1 + kd, k5 = k2 - kd, k6 = k2 + kd;
+    double i3 = cache[NCa + k3], i4 = cache[NCa + k4], i5 = cache[NCa + k5], i6 = cache[NCa + k6];
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+      G[s][ax] = (P[s * NC + a4] * i4 - P[s * NC + a3] * i3 + P[s * NC + a6] * i6 - P[s * NC + a5] * i5) * 0.25;
+    G[V::np][ax] = (cache[2 * NCa + k4] - cache[2 * NCa + k3] + cache[2 * NCa + k6] - cache[2 * NCa + k5]) * 0.25;
+#pragma unroll
+    for (int s = V::nu; s <= V::nw; ++s)
+      G[s][ax] = (P[s * NC + a4] - P[s * NC + a3] + P[s * NC + a6] - P[s * NC + a5]) * 0.25;
+#pragma unroll
+    for (int s = V::nt; s < NP; ++s)
+      G[s][ax] = (P[s * NC + a4] * i4 - P[s * NC + a3] * i3 + P[s * NC + a6] * i6 - P[s * NC + a5] * i5) * 0.25;
+  }
+  double Mav[3][3];
+#pragma unroll
+  for (int x = 0; x < 3; ++x)
+#pragma unroll
+    for (int y = 0; y < 3; ++y) Mav[x][y] = 0.5 * (C.m[(int64_t)(x + 3 * y) * NC + c0] + C.m[(int64_t)(x + 3 * y) * NC + c1]);
+  double Pf[NP], ps[NS];
+#pragma unroll
+  for (int s = 0; s < NP; ++s) Pf[s] = 0.5 * (q0[s] + q1[s]);
+#pragma unroll
+  for (int s = 0; s < NS; ++s) ps[s] = Pf[s];
+  const double* fm = D == 0 ? C.fm1 : (D == 1 ? C.fm2 : C.fm3);
+  const int64_t NFg = D == 0 ? C.NF1 : (D == 1 ? C.NF2 : C.NF3);
+  const int64_t fg = D == 0 ? B.f1off + i + (int64_t)(B.n1 + 1) * ((j - 1) + (int64_t)B.n2 * (k - 1))
+                   : D == 1 ? B.f2off + (i - 1) + (int64_t)B.n1 * (j + (int64_t)(B.n2 + 1) * (k - 1))
+                            : B.f3off + (i - 1) + (int64_t)B.n1 * ((j - 1) + (int64_t)B.n2 * k);
+  FaceThermo ft = face_thermo<NS>(T, ps, Pf[V::np], C.vc.prl);
+  diff_flux_interior<NS, NR>(T, C.vc, G, Mav, Pf, ft, fm[fg], fm[NFg + fg], fm[2 * NFg + fg], fm[3 * NFg + fg], res);
+  (void)tl;
+}
+
+template <int NS, int NR, bool VISC>
+__global__ void __launch_bounds__(256) k_stage(CaseDev C, StageArgs A) {
+  using V = Vars<NS, NR>;
+  constexpr int NP = V::nprim;
+  extern __shared__ double sm[];
+  const int ntile_wg = gridDim.x;
+  int tid = threadIdx.x;
+  int tidx = (ntile_wg == ((A.ntiles + 7) / 8) * 8) ? xcd_tile(blockIdx.x, A.ntiles) : (int)blockIdx.x;
+  if (tidx < 0 || tidx >= A.ntiles) return;
+  const Tile tl = A.tiles[tidx];
+  const mose_blk B = C.blk[tl.b];
+  const mose_dtab& T = *C.tab;
+  const int64_t NC = C.NC;
+  const double* P = C.Pin;
+  const int TI = A.TI, TJ = A.TJ, TK = A.TK;
+  const int NI = (TI + 1) * TJ * TK, NJ = TI * (TJ + 1) * TK;
+  const int NF = A.nslot;
+  double* Hs = sm;                       // region 0: [NP][nhs] half-slopes (phase A)
+  double* cache = sm;                    // region 0: [3][ncache] rho, 1/rho, T (phases B-D)
+  double* Fb = sm + A.r0;                // region 1: [NP][NF] face fluxes
+  const int64_t si = 1, sj = B.pi, sk = (int64_t)B.pi * B.pj;
+
+  // ---------------- phase A0: limited half-slopes of the tile's cells (+ one cell along each direction) --------
+  const int HI = (TI + 2) * TJ * TK, HJ = TI * (TJ + 2) * TK;
+  for (int q = tid; q < A.nhs; q += blockDim.x) {
+    int d, a, b, c;
+    if (q < HI) { d = 0; a = q % (TI + 2); int r = q / (TI + 2); b = r % TJ; c = r / TJ; }
+    else if (q < HI + HJ) { int g = q - HI; d = 1; a = g % TI; int r = g / TI; b = r % (TJ + 2); c = r / (TJ + 2); }
+    else { int g = q - HI - HJ; d = 2; a = g % TI; int r = g / TI; b = r % TJ; c = r / TJ; }
+    int i = tl.i0 + a - (d == 0), j = tl.j0 + b - (d == 1), k = tl.k0 + c - (d == 2);
+    int n_d = d == 0 ? B.n1 : (d == 1 ? B.n2 : B.n3);
+    int l_d = d == 0 ? i : (d == 1 ? j : k);
+    if (n_d < 2 || l_d < 1 || l_d > n_d || i > B.n1 || j > B.n2 || k > B.n3) continue;
+    int64_t st = d == 0 ? si : (d == 1 ? sj : sk);
+    int64_t c0 = row(B, i, j, k);
+    const double* dlp = C.dl + (int64_t)d * NC;
+    double dm = dlp[c0 - st], d0 = dlp[c0], dp = dlp[c0 + st];
+    double idm = 1.0 / (0.5 * (dm + d0)), idp = 1.0 / (0.5 * (d0 + dp)), hd = 0.5 * d0;
+#pragma unroll
+    for (int s = 0; s < NP; ++s)
+      Hs[s * A.nhs + q] = half_slope(C.lim, P[s * NC + c0 - st], P[s * NC + c0], P[s * NC + c0 + st], idm, idp, hd);
+  }
+  __syncthreads();
+
+  // ---------------- phase A1: convective faces from the half-slopes ----------------
+  for (int f = tid; f < NF; f += blockDim.x) {
+    int d, a, b, c;
+    if (f < NI) { d = 0; a = f % (TI + 1); int r = f / (TI + 1); b = r % TJ; c = r / TJ; }
+    else if (f < NI + NJ) { int g = f - NI; d = 1; a = g % TI; int r = g / TI; b = r % (TJ + 1); c = r / (TJ + 1); }
+    else { int g = f - NI - NJ; d = 2; a = g % TI; int r = g / TI; b = r % TJ; c = r / TJ; }
+    // the face's left cell (the reference's face index): the cell before local (a,b,c) along d
+    int i = tl.i0 + a - (d == 0), j = tl.j0 + b - (d == 1), k = tl.k0 + c - (d == 2);
+    int n_d = d == 0 ? B.n1 : (d == 1 ? B.n2 : B.n3);
+    int l_d = d == 0 ? i : (d == 1 ? j : k);
+    bool ok = (l_d >= 1 && l_d <= n_d - 1) && i <= B.n1 && j <= B.n2 && k <= B.n3;
+    if (!ok) continue;
+    int64_t st = d == 0 ? si : (d == 1 ? sj : sk);
+    int64_t c0 = row(B, i, j, k);
+    // half-slope slots of the left cell (local a-1 along d, +1 halo -> a) and of the right cell
+    int hl = d == 0 ? a + (TI + 2) * (b + TJ * c) : (d == 1 ? HI + a + TI * (b + (TJ + 2) * c) : HI + HJ + a + TI * (b + TJ * c));
+    int hr = hl + (d == 0 ? 1 : (d == 1 ? TI : TI * TJ));
+    double L[NP], R[NP], res[NP];
+#pragma unroll
+    for (int s = 0; s < NP; ++s) {
+      L[s] = P[s * NC + c0] + Hs[s * A.nhs + hl];
+      R[s] = P[s * NC + c0 + st] - Hs[s * A.nhs + hr];
+    }
+    const double* fm = d == 0 ? C.fm1 : (d == 1 ? C.fm2 : C.fm3);
+    int64_t NFg = d == 0 ? C.NF1 : (d == 1 ? C.NF2 : C.NF3);
+    int64_t fg = d == 0 ? B.f1off + i + (int64_t)(B.n1 + 1) * ((j - 1) + (int64_t)B.n2 * (k - 1))
+               : d == 1 ? B.f2off + (i - 1) + (int64_t)B.n1 * (j + (int64_t)(B.n2 + 1) * (k - 1))
+                        : B.f3off + (i - 1) + (int64_t)B.n1 * ((j - 1) + (int64_t)B.n2 * k);
+    conv_face_lr<NS, NR>(T, C.rs, C.mco, L, R, fm[fg], fm[NFg + fg], fm[2 * NFg + fg], fm[3 * NFg + fg], res);
+#pragma unroll
+    for (int s = 0; s < NP; ++s) Fb[s * NF + f] = res[s];
+  }
+  __syncthreads();
+  if (VISC) {
+    // the half-slopes are dead: region 0 now holds (rho, 1/rho, T) of the tile and its one-cell halo
+    const int CI = TI + 2, CJ = TJ + 2;
+    for (int q = tid; q < A.ncache; q += blockDim.x) {
+      int a = q % CI, r = q / CI, b = r % CJ, c = r / CJ;
+      int i = tl.i0 + a - 1, j = tl.j0 + b - 1, k = tl.k0 + c - 1;
+      if (i < 1 - GC || i > B.n1 + GC || j < 1 - GC || j > B.n2 + GC || k < 1 - GC || k > B.n3 + GC) continue;
+      int64_t r0 = row(B, i, j, k);
+      double y[NS];
+#pragma unroll
+      for (int s = 0; s < NS; ++s) y[s] = P[s * NC + r0];
+      double rho = rho_sum<NS>(y);
+      cache[q] = rho;
+      cache[A.ncache + q] = 1.0 / rho;
+      cache[2 * A.ncache + q] = P[V::np * NC + r0] / (rho * f_Rtot<NS>(T, y));
+    }
+  }
+
+  // ---------------- phase B: convective sums ----------------
+  int ca = tid % TI, cb = (tid / TI) % TJ, cc = tid / (TI * TJ);
+  int ci = tl.i0 + ca, cj = tl.j0 + cb, ck = tl.k0 + cc;
+  bool mine = (tid < TI * TJ * TK) && ci <= B.n1 && cj <= B.n2 && ck <= B.n3;
+  int sIm = ca + (TI + 1) * (cb + TJ * cc), sJm = NI + ca + TI * (cb + (TJ + 1) * cc), sKm = NI + NJ + ca + TI * (cb + TJ * cc);
+  int sIp = sIm + 1, sJp = sJm + TI, sKp = sKm + TI * TJ;
+  double tmp[NP];
+#pragma unroll
+  for (int s = 0; s < NP; ++s) tmp[s] = 0.0;
+  if (mine) {
+#pragma unroll
+    for (int s = 0; s < NP; ++s) {
+      double t = tmp[s];
+      if (ci < B.n1) t = t + Fb[s * NF + sIp];
+      if (ci > 1) t = t - Fb[s * NF + sIm];
+      if (cj < B.n2) t = t + Fb[s * NF + sJp];
+      if (cj > 1) t = t - Fb[s * NF + sJm];
+      if (ck < B.n3) t = t + Fb[s * NF + sKp];
+      if (ck > 1) t = t - Fb[s * NF + sKm];
+      tmp[s] = t;
+    }
+  }
+
+  // ---------------- phase C: viscous faces ----------------
+  if (VISC) {
+    __syncthreads();
+    for (int f = tid; f < NF; f += blockDim.x) {
+      int d, a, b, c;
+      if (f < NI) { d = 0; a = f % (TI + 1); int r = f / (TI + 1); b = r % TJ; c = r / TJ; }
+      else if (f < NI + NJ) { int g = f - NI; d = 1; a = g % TI; int r = g / TI; b = r % (TJ + 1); c = r / (TJ + 1); }
+      else { int g = f - NI - NJ; d = 2; a = g % TI; int r = g / TI; b = r % TJ; c = r / TJ; }
+      int i = tl.i0 + a - (d == 0), j = tl.j0 + b - (d == 1), k = tl.k0 + c - (d == 2);
+      int n_d = d == 0 ? B.n1 : (d == 1 ? B.n2 : B.n3);
+      int l_d = d == 0 ? i : (d == 1 ? j : k);
+      bool ok = (l_d >= 1 && l_d <= n_d - 1) && i <= B.n1 && j <= B.n2 && k <= B.n3;
+      if (!ok) continue;
+      // local cache coordinates of the left cell c1 (tile-local + 1)
+      double res[NP];
+      if (d == 0) visc_face<NS, NR, 0>(C, B, tl, A, cache, i, j, k, a, b, c, res);
+      else if (d == 1) visc_face<NS, NR, 1>(C, B, tl, A, cache, i, j, k, a, b, c, res);
+      else visc_face<NS, NR, 2>(C, B, tl, A, cache, i, j, k, a, b, c, res);
+#pragma unroll
+      for (int s = 0; s < NP; ++s) Fb[s * NF + f] = res[s];
+    }
+    __syncthreads();
+    if (mine) {
+#pragma unroll
+      for (int s = 0; s < NP; ++s) {
+        double t = tmp[s];
+        if (ci < B.n1) t = t - Fb[s * NF + sIp];
+        if (ci > 1) t = t + Fb[s * NF + sIm];
+        if (cj < B.n2) t = t - Fb[s * NF + sJp];
+        if (cj > 1) t = t + Fb[s * NF + sJm];
+        if (ck < B.n3) t = t - Fb[s * NF + sKp];
+        if (ck > 1) t = t + Fb[s * NF + sKm];
+        tmp[s] = t;
+      }
+    }
+  }
+
+  // ---------------- phase D: boundary increments, update, next dt ----------------
+  double dtl = 1e300;
+  if (mine) {
+    bool on[6] = {ci == 1, ci == B.n1, cj == 1, cj == B.n2, ck == 1, ck == B.n3};
+#pragma unroll
+    for (int f = 1; f <= 6; ++f) {
+      if (!on[f - 1]) continue;
+      int64_t pos = B.bpoff[f - 1] + plane_pos(B, f, ci, cj, ck);
+#pragma unroll
+      for (int s = 0; s < NP; ++s) {
+        tmp[s] = tmp[s] + C.binc[(int64_t)s * C.NBP + pos];
+        tmp[s] = tmp[s] + C.binc[(int64_t)(NP + s) * C.NBP + pos];
+      }
+    }
+    int64_t c0 = row(B, ci, cj, ck);
+    double p[NP], po[NP], out[NP];
+#pragma unroll
+    for (int s = 0; s < NP; ++s) { p[s] = P[s * NC + c0]; po[s] = C.PO[s * NC + c0]; }
+    double Tseed;
+    if (VISC) {
+      const int CI = TI + 2, CJ = TJ + 2;
+      Tseed = cache[2 * A.ncache + (ca + 1) + CI * ((cb + 1) + CJ * (cc + 1))];
+      if (C.guard && !(Tseed > 1.0 && Tseed < 9990.0)) {
+        flag_bad(C, tl.b, ci, cj, ck, 1);
+        if (Tseed != Tseed) Tseed = 300.0;
+        Tseed = dmin2(dmax2(Tseed, 1.0), 9990.0);
+      }
+    } else {
+      double rho = rho_sum<NS>(p), Rg = f_Rtot<NS>(T, p);
+      Tseed = p[V::np] / (rho * Rg);
+    }
+    update_cell<NS, NR>(T, p, po, tmp, C.vol[c0], *C.dt, A.rkc, Tseed, out);
+    bool good = out[V::np] >= 0.0;
+#pragma unroll
+    for (int s = 0; s < NP; ++s) good = good && isfinite(out[s]);
+    if (!good) flag_bad(C, tl.b, ci, cj, ck, 0);
+#pragma unroll
+    for (int s = 0; s < NP; ++s) C.Pout[s * NC + c0] = out[s];
+    if (A.last) dtl = cell_dt<NS, NR>(C, c0, out);
+  }
+  if (A.last) block_min_atomic(dtl, A.dtnext);
+}
+
+}  // namespace mose

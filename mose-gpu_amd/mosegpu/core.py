@@ -61,6 +61,14 @@ class Tables(C.Structure):
                 ("mu", C.POINTER(C.c_double)), ("k", C.POINTER(C.c_double))]
 
 
+PATH_FUSED, PATH_PLAIN = 0, 1
+
+
+class Tuning(C.Structure):
+    _fields_ = [("path", C.c_int32), ("ti", C.c_int32), ("tj", C.c_int32), ("tk", C.c_int32),
+                ("timing", C.c_int32), ("reserved", C.c_int32 * 3)]
+
+
 class Report(C.Structure):
     _fields_ = [("dt", C.c_double), ("time", C.c_double), ("steps", C.c_int64), ("bad_cells", C.c_int32),
                 ("first_bad_block", C.c_int32), ("first_bad_i", C.c_int32), ("first_bad_j", C.c_int32),
@@ -109,11 +117,15 @@ def lib() -> C.CDLL:
     L.mose_case_dt.argtypes = [vp, ip, dp]
     L.mose_case_residual_norm.argtypes = [vp, ip, dp]
     L.mose_case_state_hash.argtypes = [vp, ip, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
-    L.mose_case_timing.argtypes = [vp, ip, dp, dp, dp]
+    L.mose_case_timing.argtypes = [vp, ip, dp, dp, C.POINTER(C.c_int64)]
+    L.mose_case_tune.argtypes = [vp, ip, C.POINTER(Tuning), C.POINTER(Tuning)]
+    L.mose_bandwidth_probe.argtypes = [vp, C.c_int64, ip, dp]
+    L.mose_ctx_info.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     for name in ("mose_ctx_create", "mose_ctx_destroy", "mose_metrics_build", "mose_case_create",
                  "mose_case_destroy", "mose_case_bytes", "mose_state_upload", "mose_state_download",
                  "mose_case_step", "mose_cases_step", "mose_case_residual", "mose_case_dt",
-                 "mose_case_residual_norm", "mose_case_state_hash", "mose_case_timing", "mose_device_count"):
+                 "mose_case_residual_norm", "mose_case_state_hash", "mose_case_timing", "mose_device_count",
+                 "mose_case_tune", "mose_bandwidth_probe", "mose_ctx_info"):
         getattr(L, name).restype = C.c_int
     _lib_handle = L
     return L
@@ -218,6 +230,17 @@ class Context:
             raise MoseError(rc, f"{what}: {lib().mose_last_error(self.h).decode()}")
         return rc
 
+    def info(self) -> dict:
+        t, f, n, l = C.c_int64(), C.c_int64(), C.c_int(), C.c_int()
+        self.check(lib().mose_ctx_info(self.h, C.byref(t), C.byref(f), C.byref(n), C.byref(l)), "mose_ctx_info")
+        return {"hbm_total": t.value, "hbm_free": f.value, "cus": n.value, "lds_per_block": l.value}
+
+    def bandwidth_probe(self, nbytes: int = 1 << 30, reps: int = 20) -> float:
+        """GB/s (read + write) of the library's stream-copy kernel on `nbytes` buffers."""
+        v = C.c_double(0.0)
+        self.check(lib().mose_bandwidth_probe(self.h, int(nbytes), int(reps), C.byref(v)), "mose_bandwidth_probe")
+        return v.value
+
     def close(self):
         if self.h:
             lib().mose_ctx_destroy(self.h)
@@ -267,6 +290,19 @@ class DeviceCase:
         v = C.c_double(0.0)
         self.ctx.check(lib().mose_case_dt(self.ctx.h, self.id, C.byref(v)), "mose_case_dt")
         return v.value
+
+    def tune(self, path: int = PATH_FUSED, tile=(0, 0, 0), timing: bool = False):
+        """Execution path and tile (0,0,0 = autotune); returns the tile chosen."""
+        t, out = Tuning(), Tuning()
+        t.path = path; t.ti, t.tj, t.tk = tile; t.timing = int(timing)
+        self.ctx.check(lib().mose_case_tune(self.ctx.h, self.id, C.byref(t), C.byref(out)), "mose_case_tune")
+        return (out.ti, out.tj, out.tk)
+
+    def timing(self):
+        """(average ms per stage kernel, total ms, launches) of the last timed step call."""
+        a, t, n = C.c_double(), C.c_double(), C.c_int64()
+        self.ctx.check(lib().mose_case_timing(self.ctx.h, self.id, C.byref(a), C.byref(t), C.byref(n)), "timing")
+        return a.value, t.value, n.value
 
     def destroy(self):
         if self.id >= 0:

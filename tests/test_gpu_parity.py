@@ -104,3 +104,64 @@ def test_gpu_errors_are_reported(gpu_ready):
     with pytest.raises(Exception) as ei:
         dc.step(1)
     assert "MOSE_ERR_STATE" in str(ei.value)
+
+
+# --------------------------------------------------------------------------------------------------------------
+# the fused stage kernel (default path) against the oracle, on tile shapes that do and do not divide the block
+# --------------------------------------------------------------------------------------------------------------
+def run_fused(case, nsteps, tile):
+    import mosegpu
+    ctx = Context(case.tables)
+    dc = DeviceCase(ctx, case)
+    dc.upload(case.P)
+    chosen = dc.tune(mosegpu.PATH_FUSED, tile)
+    rep = dc.step(nsteps)
+    return dc.download(), rep, chosen
+
+
+@pytest.mark.parametrize("tile", [(32, 4, 2), (16, 4, 4), (64, 2, 2), (0, 0, 0)])
+def test_fused_taylor_green_parity(gpu_ready, tile):
+    case = cases.taylor_green(20, 6, dims=(20, 18, 21))
+    case.meta["gamma"] = 1004.5 / (1004.5 - 8314.46261815324 / 28.970418)
+    Pg, rep, chosen = run_fused(case, 6, tile)
+    Po = case.P.copy()
+    dts = OracleCase(case).step(Po, 6)
+    assert abs(rep.time - dts.sum()) <= 1e-13 * dts.sum()
+    err = rel_linf(conservative(case, Pg), conservative(case, Po))
+    nbit = int(np.sum(case.layout.interior(Pg, 0) != case.layout.interior(Po, 0)))
+    print(f"fused tile {chosen}: taylor-green 20x18x21 x 6 steps rel Linf {err:.3e}, bitwise-differing {nbit}")
+    assert err <= TOL
+
+
+def test_fused_sod_parity(gpu_ready):
+    case = cases.sod(1000, 200)
+    case.meta["gamma"] = 1004.5 / (1004.5 - 8314.46261815324 / 28.970418)
+    Pg, rep, chosen = run_fused(case, 200, (256, 1, 1))
+    Po = case.P.copy()
+    OracleCase(case).step(Po, 200)
+    err = rel_linf(conservative(case, Pg), conservative(case, Po))
+    print(f"fused sod 1000 x 200: rel Linf {err:.3e}")
+    assert err <= TOL
+
+
+def test_fused_equals_plain_at_bench_size(gpu_ready):
+    """At the benchmark's full size (256^3) the oracle is too slow: the fused kernel must give the bits of the
+    per-face kernels, and the total mass on the periodic box must stay constant (to rounding)."""
+    import mosegpu
+    case = cases.taylor_green(256, 2)
+    ctx = Context(case.tables)
+    out = {}
+    for name, path, tile in (("plain", mosegpu.PATH_PLAIN, (0, 0, 0)), ("fused", mosegpu.PATH_FUSED, (32, 4, 2))):
+        dc = DeviceCase(ctx, case)
+        dc.upload(case.P)
+        dc.tune(path, tile)
+        dc.step(2)
+        out[name] = dc.download()
+        dc.destroy()
+    a = case.layout.interior(out["plain"], 0)
+    b = case.layout.interior(out["fused"], 0)
+    assert np.array_equal(a, b)
+    vol = case.layout.interior(case.geom.vol, 0)
+    m0 = np.sum(case.layout.interior(case.P, 0)[0] * vol)
+    m1 = np.sum(b[0] * vol)
+    assert abs(m1 - m0) <= 1e-12 * m0
