@@ -54,8 +54,8 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--path", choices=["fused", "plain"], default="fused")
     ap.add_argument("--tile", default="auto", help="TIxTJxTK or auto")
-    ap.add_argument("--cpu-n", type=int, default=48)
-    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--cpu-n", type=int, default=64)
+    ap.add_argument("--cpu-steps", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--probe-bw", action="store_true", help="only run the stream-copy probe (1 GiB buffers)")
     args = ap.parse_args()

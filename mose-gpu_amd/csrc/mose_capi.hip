@@ -549,8 +549,16 @@ static void launch_stage(Case& c, const CaseDev& D, hipStream_t s, double rkc, b
   A.last = last ? 1 : 0; A.rkc = rkc; A.dtnext = c.dtnext;
   size_t lds = stage_lds(c, c.TI, c.TJ, c.TK);
   unsigned grid = (unsigned)(((c.ntiles + 7) / 8) * 8);
-  if (c.sc.viscous) k_stage<NS, NR, true><<<grid, 256, lds, s>>>(D, A);
-  else k_stage<NS, NR, false><<<grid, 256, lds, s>>>(D, A);
+  const int rs = c.sc.riemann;
+  if (c.sc.viscous) {
+    if (rs == MOSE_RS_LMROE) k_stage<NS, NR, true, MOSE_RS_LMROE><<<grid, 256, lds, s>>>(D, A);
+    else if (rs == MOSE_RS_AUSMP) k_stage<NS, NR, true, MOSE_RS_AUSMP><<<grid, 256, lds, s>>>(D, A);
+    else k_stage<NS, NR, true, MOSE_RS_HLLC><<<grid, 256, lds, s>>>(D, A);
+  } else {
+    if (rs == MOSE_RS_LMROE) k_stage<NS, NR, false, MOSE_RS_LMROE><<<grid, 256, lds, s>>>(D, A);
+    else if (rs == MOSE_RS_AUSMP) k_stage<NS, NR, false, MOSE_RS_AUSMP><<<grid, 256, lds, s>>>(D, A);
+    else k_stage<NS, NR, false, MOSE_RS_HLLC><<<grid, 256, lds, s>>>(D, A);
+  }
 }
 
 __global__ void k_step_begin(const double* dt, double* hist, int n) {

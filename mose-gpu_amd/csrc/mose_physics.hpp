@@ -366,12 +366,14 @@ __device__ __forceinline__ F5 lmroe(const mose_dtab& T, double mco, const double
 }
 
 // The Riemann dispatch (conv_physics.inc:103-183; Lib_Riemann.f90:114-136).  hllc_clamp: the interior
-// kernel clamps the table index of the HLLC Roe averages, the record kernels do not.
-template <int NS>
-__device__ __forceinline__ F5 riemann(const mose_dtab& T, int rs, double mco, bool hllc_clamp, const double* dl,
+// kernel clamps the table index of the HLLC Roe averages, the record kernels do not.  RS != 0 fixes the
+// solver at compile time (one kernel per solver keeps the register allocation of the other two out).
+template <int NS, int RS = 0>
+__device__ __forceinline__ F5 riemann(const mose_dtab& T, int rs_, double mco, bool hllc_clamp, const double* dl,
                                       double ul, double vl, double wl, double pl, double al, double dlt, const double* dr,
                                       double ur, double vr, double wr, double pr, double ar, double drt, double nx,
                                       double ny, double nz, double Rl, double Rr) {
+  const int rs = RS != 0 ? RS : rs_;
   if (rs == 12) {
     double rl = f_Rtot<NS>(T, dl), rr = f_Rtot<NS>(T, dr);
     Roe ro = roe_avg<NS>(T, true, dl, dr, dlt, drt, ul, vl, wl, ur, vr, wr, pl, pr, rl, rr);
@@ -397,7 +399,7 @@ __device__ __forceinline__ double half_slope(int lim, double qm, double q0, doub
 }
 
 // The flux of an interior face from its reconstructed states L (left) and R (right): conv_physics.inc:34-179
-template <int NS, int NR>
+template <int NS, int NR, int RS = 0>
 __device__ __forceinline__ void conv_face_lr(const mose_dtab& T, int rs, double mco, double* L, double* R, double nx,
                                              double ny, double nz, double area, double* res);
 
@@ -419,7 +421,7 @@ __device__ __forceinline__ void conv_face(const mose_dtab& T, int lim, int rs, d
   conv_face_lr<NS, NR>(T, rs, mco, L, R, nx, ny, nz, area, res);
 }
 
-template <int NS, int NR>
+template <int NS, int NR, int RS>
 __device__ __forceinline__ void conv_face_lr(const mose_dtab& T, int rs, double mco, double* L, double* R, double nx,
                                              double ny, double nz, double area, double* res) {
   using V = Vars<NS, NR>;
@@ -449,7 +451,7 @@ __device__ __forceinline__ void conv_face_lr(const mose_dtab& T, int rs, double 
     double g = cp / (cp - RR);
     aR = sqrt(g * RR * Tk);
   }
-  F5 F = riemann<NS>(T, rs, mco, true, L, L[V::nu], L[V::nv], L[V::nw], L[V::np], aL, rL, R, R[V::nu], R[V::nv],
+  F5 F = riemann<NS, RS>(T, rs, mco, true, L, L[V::nu], L[V::nv], L[V::nw], L[V::np], aL, rL, R, R[V::nu], R[V::nv],
                      R[V::nw], R[V::np], aR, rR, nx, ny, nz, RL, RR);
   double su = copysign(0.5, F.r), sp = 0.5 + su, sm = su - 0.5;
   double iL = 1.0 / rL, iR = 1.0 / rR;

@@ -115,7 +115,7 @@ __device__ __forceinline__ void visc_face(const CaseDev& C, const mose_blk& B, c
   (void)tl;
 }
 
-template <int NS, int NR, bool VISC>
+template <int NS, int NR, bool VISC, int RS>
 __global__ void __launch_bounds__(256) k_stage(CaseDev C, StageArgs A) {
   using V = Vars<NS, NR>;
   constexpr int NP = V::nprim;
@@ -187,7 +187,7 @@ __global__ void __launch_bounds__(256) k_stage(CaseDev C, StageArgs A) {
     int64_t fg = d == 0 ? B.f1off + i + (int64_t)(B.n1 + 1) * ((j - 1) + (int64_t)B.n2 * (k - 1))
                : d == 1 ? B.f2off + (i - 1) + (int64_t)B.n1 * (j + (int64_t)(B.n2 + 1) * (k - 1))
                         : B.f3off + (i - 1) + (int64_t)B.n1 * ((j - 1) + (int64_t)B.n2 * k);
-    conv_face_lr<NS, NR>(T, C.rs, C.mco, L, R, fm[fg], fm[NFg + fg], fm[2 * NFg + fg], fm[3 * NFg + fg], res);
+    conv_face_lr<NS, NR, RS>(T, C.rs, C.mco, L, R, fm[fg], fm[NFg + fg], fm[2 * NFg + fg], fm[3 * NFg + fg], res);
 #pragma unroll
     for (int s = 0; s < NP; ++s) Fb[s * NF + f] = res[s];
   }
