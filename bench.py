@@ -30,6 +30,17 @@ def algorithmic_bytes_per_cell_stage(nprim: int, viscous: bool, rans: bool) -> i
     return 8 * (3 * nprim + G)
 
 
+def profiled_traffic(n: int, nprim: int, viscous: bool, tile) -> int | None:
+    """HBM bytes per stage launch measured by rocprofv3 counters for this workload (profiles/*_traffic.txt), or None."""
+    key = ("k_stage", str(n), str(nprim), str(int(viscous)), "x".join(str(t) for t in tile))
+    for f in sorted((REPO / "profiles").glob("*_traffic.txt")):
+        for line in f.read_text().splitlines():
+            w = line.split()
+            if len(w) == 6 and not line.startswith("#") and tuple(w[:5]) == key:
+                return int(w[5])
+    return None
+
+
 def cpu_baseline(n: int, steps: int, threads: int):
     sys.path.insert(0, str(REPO / "tests"))
     from mosegpu import cases
@@ -55,7 +66,7 @@ def main():
     ap.add_argument("--path", choices=["fused", "plain"], default="fused")
     ap.add_argument("--tile", default="auto", help="TIxTJxTK or auto")
     ap.add_argument("--cpu-n", type=int, default=64)
-    ap.add_argument("--cpu-steps", type=int, default=24)
+    ap.add_argument("--cpu-steps", type=int, default=96)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--probe-bw", action="store_true", help="only run the stream-copy probe (1 GiB buffers)")
     args = ap.parse_args()
@@ -101,7 +112,8 @@ def main():
     B = algorithmic_bytes_per_cell_stage(case.nprim, True, False)
     achieved = (B * cells) / (ms_avg * 1e-3) / 1e9 if ms_avg > 0 else None
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+                "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+                "traffic": profiled_traffic(args.n, case.nprim, True, chosen) if path == mosegpu.PATH_FUSED else None,
                 "kernel": "k_stage (fused residual + RK stage update)" if path == mosegpu.PATH_FUSED
                           else "face kernels + cell update (one stage)",
                 "kernel_ms_avg": ms_avg, "launches": nlaunch, "bytes_per_cell_stage": B}
