@@ -71,9 +71,22 @@ typedef struct {
   int32_t bs, is, js, ks, fs;      /* connection donor block (0-based), cell and face */
   int32_t d11, d12, d21, d22;      /* transverse axis map (identity: 1,0,0,1) */
   int32_t periodic;                /* MOSE 201: ghost nodes translated by one period (metrics only) */
-  double tw;                       /* isothermal wall temperature */
-  double farfield[8];              /* far field: rho(species sum), u, v, w, p, nu~ ... (type 4) */
+  int32_t ff;                      /* type 4: row of the case's far-field table (mose_farfield) */
+  int32_t pad;
+  double tw;                       /* isothermal wall temperature (bc_tw) */
+  double tvec[3];                  /* type 4: unit tangent of the face, the node diagonal (bc_tvec, Mod_BC.f90:228-240);
+                                      mose_metrics_build fills it from the nodes */
 } mose_bc_record;
+
+/* The data of a far-field / inflow-outflow face (type 4; MOSE_open 407 -> legacy 4, IO_BC_Open.f90:302-311):
+ * bc_mach, bc_t0, bc_p0, bc_pamb, bc_alpha, bc_beta, bc_relfac, bc_ci (mass fractions), bc_rans (nu~ per unit
+ * density) of Mod_BC.f90:212-224.  mach >= 0: stagnation inlet (p0, T0) / back-pressure outlet (pamb). */
+#define MOSE_MAX_SPECIES 16
+typedef struct {
+  double mach, t0, p0, pamb, alpha, beta, rel_fac, reserved;
+  double ci[MOSE_MAX_SPECIES];
+  double rans[2];
+} mose_farfield;
 
 typedef struct {
   int32_t nsc;          /* species (nsc >= 1) */
@@ -129,12 +142,17 @@ int mose_ctx_info(mose_ctx* ctx, int64_t* hbm_total, int64_t* hbm_free, int* cu_
 int mose_metrics_build(int nb, const int* dims, const double* nodes, int nrec, const mose_bc_record* rec, int ndir,
                        double* dl, double* m, double* vol, double* yn, double* fm1, double* fm2, double* fm3);
 
+/* The unit tangent of every far-field record (type 4) from the nodes (Setup_BC_Device, Mod_BC.f90:228-240).
+ * Host only; writes rec[e].tvec. */
+int mose_records_tangent(int nb, const int* dims, const double* nodes, int nrec, mose_bc_record* rec);
+
 /* A case: blocks, geometry and boundary records made device resident (Setup_Device_Residency,
  * Mod_Explicit.f90:404-482; Setup_BC_Device).  Records must be grouped by block, then by face 1..6
  * (the launch order of BC_Fluxes, Mod_BC.f90:1042-1062).  Returns the case id in *case_id. */
 int mose_case_create(mose_ctx* ctx, const mose_scheme* scheme, int nb, const int* dims, const double* dl,
                      const double* m, const double* vol, const double* yn, const double* fm1, const double* fm2,
-                     const double* fm3, int nrec, const mose_bc_record* rec, int* case_id);
+                     const double* fm3, int nrec, const mose_bc_record* rec, int nff, const mose_farfield* ff,
+                     int* case_id);
 int mose_case_destroy(mose_ctx* ctx, int case_id);
 /* bytes of device memory a case of these blocks takes (the serving loop admits by this figure) */
 int mose_case_bytes(const mose_scheme* scheme, int nb, const int* dims, int nrec, int64_t* bytes);

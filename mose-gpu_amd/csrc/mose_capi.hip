@@ -29,6 +29,8 @@ struct Case {
   int nb = 0, nprim = 0;
   std::vector<mose_blk> blk;
   std::vector<mose_bc_record> rec;
+  std::vector<mose_farfield> ff;
+  std::vector<signed char> btyp;   // record type per boundary-plane position
   int64_t NC = 0, NF1 = 0, NF2 = 0, NF3 = 0, NFI[3] = {0, 0, 0}, NBP = 0, ninterior = 0;
   // device memory (one slab)
   void* slab = nullptr;
@@ -39,6 +41,8 @@ struct Case {
   double* binc = nullptr;
   mose_blk* dblk = nullptr;
   mose_bc_record* drec = nullptr;
+  mose_farfield* dff = nullptr;
+  signed char* dbtyp = nullptr;
   double* dt = nullptr;          // [1] current dt
   double* dthist = nullptr;      // [hist_cap]
   int hist_cap = 0;
@@ -185,11 +189,12 @@ extern "C" int mose_ctx_destroy(mose_ctx* ctx) {
 }
 
 // block table, arena sizes and validation of the scheme / records
-static int plan_case(const mose_scheme* sc, int nb, const int* dims, int nrec, const mose_bc_record* rec, Case& c,
-                     std::string& why) {
+static int plan_case(const mose_scheme* sc, int nb, const int* dims, int nrec, const mose_bc_record* rec, int nff,
+                     const mose_farfield* ff, Case& c, std::string& why) {
   if (!sc || nb <= 0 || !dims) { why = "bad arguments"; return MOSE_ERR_ARG; }
   if (sc->nsc < 1 || sc->nsc > MOSE_MAX_SPECIES) { why = "nsc out of range"; return MOSE_ERR_ARG; }
   if (sc->nrans < 0 || sc->nrans > 1) { why = "nrans must be 0 or 1 (Spalart-Allmaras)"; return MOSE_ERR_ARG; }
+  if (sc->nrans > 0 && !sc->viscous) { why = "Spalart-Allmaras needs the viscous fluxes"; return MOSE_ERR_ARG; }
   if (sc->limiter < 1 || sc->limiter > 6) { why = "limiter id must be 1..6"; return MOSE_ERR_ARG; }
   if (!(sc->riemann == MOSE_RS_HLLC || sc->riemann == MOSE_RS_AUSMP || sc->riemann == MOSE_RS_LMROE)) {
     why = "riemann solver must be HLLC (1), AUSM+ (4) or LMRoe (12)";
@@ -261,6 +266,21 @@ static int plan_case(const mose_scheme* sc, int nb, const int* dims, int nrec, c
       case MOSE_BC_EXTRAPOLATION:
         if (n[dir] == 2) { why = "extrapolation faces need a block extent of 1 or >= 3 along the face normal"; return MOSE_ERR_ARG; }
         break;
+      case MOSE_BC_WALL_ADIABATIC:
+      case MOSE_BC_WALL_ISOTHERMAL:
+        // the cubic wall ghost reads three interior layers; the record stencil reads the interior neighbour
+        if (n[dir] < 3) { why = "record " + std::to_string(e) + ": a wall needs a block extent >= 3 along its normal"; return MOSE_ERR_ARG; }
+        if (r.typ == MOSE_BC_WALL_ISOTHERMAL && !(r.tw > 1.0 && r.tw < 9990.0)) { why = "record " + std::to_string(e) + ": wall temperature outside (1, 9990) K"; return MOSE_ERR_ARG; }
+        break;
+      case MOSE_BC_FARFIELD: {
+        if (n[dir] < 2) { why = "record " + std::to_string(e) + ": a far-field face needs a block extent >= 2 along its normal"; return MOSE_ERR_ARG; }
+        if (r.ff < 0 || r.ff >= nff || !ff) { why = "record " + std::to_string(e) + ": far-field row out of range"; return MOSE_ERR_ARG; }
+        const mose_farfield& q = ff[r.ff];
+        if (q.mach == 1.0) { why = "far-field row " + std::to_string(r.ff) + ": Mach exactly 1 has no closure"; return MOSE_ERR_ARG; }
+        double tn = r.tvec[0] * r.tvec[0] + r.tvec[1] * r.tvec[1] + r.tvec[2] * r.tvec[2];
+        if (q.mach >= 0.0 && q.mach < 1.0 && !(tn > 0.5)) { why = "record " + std::to_string(e) + ": far-field face tangent not set (mose_records_tangent)"; return MOSE_ERR_ARG; }
+        break;
+      }
       default:
         why = "record " + std::to_string(e) + ": boundary type " + std::to_string(r.typ) + " is not supported yet";
         return MOSE_ERR_ARG;
@@ -270,10 +290,14 @@ static int plan_case(const mose_scheme* sc, int nb, const int* dims, int nrec, c
                                                 : (r.i - 1) + (int64_t)B.n1 * (r.j - 1));
     if (seen[pos]) { why = "record " + std::to_string(e) + ": boundary face listed twice"; return MOSE_ERR_ARG; }
     seen[pos] = 1;
+    if ((int64_t)c.btyp.size() < NBP) c.btyp.assign((size_t)NBP, 0);
+    c.btyp[pos] = (signed char)r.typ;
   }
   for (int64_t q = 0; q < NBP; ++q)
     if (!seen[q]) { why = "a boundary cell-face has no record (every block face needs one record per cell)"; return MOSE_ERR_ARG; }
   c.rec.assign(rec, rec + nrec);
+  if (nff > 0) c.ff.assign(ff, ff + nff);
+  if (c.btyp.empty()) c.btyp.assign((size_t)std::max<int64_t>(NBP, 1), 0);
   return MOSE_OK;
 }
 
@@ -286,6 +310,8 @@ static size_t case_bytes(const Case& c) {
   for (int q = 0; q < 3; ++q) b += 2 * (size_t)c.NFI[q] * c.nprim * d;   // face buffers
   b += 2 * (size_t)std::max<int64_t>(c.NBP, 1) * c.nprim * d;           // boundary increments
   b += c.blk.size() * sizeof(mose_blk) + c.rec.size() * sizeof(mose_bc_record) + 8192 + 64 * 1024;
+  b += (std::max<size_t>(c.ff.size(), 1) * sizeof(mose_farfield) + 255) / 256 * 256;
+  b += ((size_t)std::max<int64_t>(c.NBP, 1) + 255) / 256 * 256;
   return b;
 }
 
@@ -342,6 +368,8 @@ static int alloc_case(mose_ctx* ctx, Case& c) {
   c.binc = (double*)take(2 * (size_t)std::max<int64_t>(c.NBP, 1) * c.nprim * d);
   c.dblk = (mose_blk*)take(c.blk.size() * sizeof(mose_blk));
   c.drec = (mose_bc_record*)take(std::max<size_t>(c.rec.size(), 1) * sizeof(mose_bc_record));
+  c.dff = (mose_farfield*)take(std::max<size_t>(c.ff.size(), 1) * sizeof(mose_farfield));
+  c.dbtyp = (signed char*)take((size_t)std::max<int64_t>(c.NBP, 1));
   c.dt = (double*)take(64);
   c.dthist = (double*)take((size_t)c.hist_cap * d);
   c.dtmin = (unsigned long long*)take(64);
@@ -376,11 +404,12 @@ static void choose_default_tile(mose_ctx* ctx, Case& c) {
 
 extern "C" int mose_case_create(mose_ctx* ctx, const mose_scheme* sc, int nb, const int* dims, const double* dl,
                                 const double* m, const double* vol, const double* yn, const double* fm1,
-                                const double* fm2, const double* fm3, int nrec, const mose_bc_record* rec, int* case_id) {
-  if (!ctx || !case_id || !dl || !m || !vol || !fm1 || !fm2 || !fm3 || (nrec > 0 && !rec)) return MOSE_ERR_ARG;
+                                const double* fm2, const double* fm3, int nrec, const mose_bc_record* rec, int nff,
+                                const mose_farfield* ff, int* case_id) {
+  if (!ctx || !case_id || !dl || !m || !vol || !fm1 || !fm2 || !fm3 || (nrec > 0 && !rec) || nff < 0) return MOSE_ERR_ARG;
   auto c = std::make_unique<Case>();
   std::string why;
-  int rc = plan_case(sc, nb, dims, nrec, rec, *c, why);
+  int rc = plan_case(sc, nb, dims, nrec, rec, nff, ff, *c, why);
   if (rc != MOSE_OK) return fail(ctx, rc, why);
   if (sc->nsc != ctx->nsc) return fail(ctx, MOSE_ERR_ARG, "the scheme's nsc differs from the context tables");
   if (sc->viscous && !ctx->viscous_tables) return fail(ctx, MOSE_ERR_ARG, "viscous case needs mu and k tables");
@@ -402,6 +431,9 @@ extern "C" int mose_case_create(mose_ctx* ctx, const mose_scheme* sc, int nb, co
   HIPCHK(ctx, hipMemcpyAsync(c->dblk, c->blk.data(), c->blk.size() * sizeof(mose_blk), hipMemcpyHostToDevice, s));
   if (!c->rec.empty())
     HIPCHK(ctx, hipMemcpyAsync(c->drec, c->rec.data(), c->rec.size() * sizeof(mose_bc_record), hipMemcpyHostToDevice, s));
+  if (!c->ff.empty())
+    HIPCHK(ctx, hipMemcpyAsync(c->dff, c->ff.data(), c->ff.size() * sizeof(mose_farfield), hipMemcpyHostToDevice, s));
+  HIPCHK(ctx, hipMemcpyAsync(c->dbtyp, c->btyp.data(), c->btyp.size(), hipMemcpyHostToDevice, s));
   HIPCHK(ctx, hipMemsetAsync(c->P0, 0, 3 * (size_t)c->NC * c->nprim * d, s));
   HIPCHK(ctx, hipStreamSynchronize(s));
   c->valid = true;
@@ -451,8 +483,9 @@ static CaseDev make_dev(mose_ctx* ctx, Case& c, const double* Pin, double* Pout)
   D.dl = c.dl; D.m = c.m; D.vol = c.vol; D.yn = c.yn; D.fm1 = c.fm1; D.fm2 = c.fm2; D.fm3 = c.fm3;
   for (int q = 0; q < 3; ++q) { D.Fc[q] = c.Fc[q]; D.Fd[q] = c.Fd[q]; D.NFI[q] = c.NFI[q]; }
   D.binc = c.binc; D.NBP = std::max<int64_t>(c.NBP, 1);
-  D.blk = c.dblk; D.rec = c.drec; D.nrec = (int)c.rec.size();
+  D.blk = c.dblk; D.rec = c.drec; D.nrec = (int)c.rec.size(); D.ff = c.dff; D.btyp = c.dbtyp;
   D.lim = c.sc.limiter; D.rs = c.sc.riemann; D.visc = c.sc.viscous; D.guard = c.sc.thermo_guard; D.mco = c.sc.mco;
+  D.pimp = c.sc.point_implicit;
   D.vc.sc = c.sc.sc; D.vc.sct = c.sc.sct; D.vc.prt = c.sc.prt; D.vc.prl = c.sc.prandtl;
   D.tab = ctx->dtab;
   D.dt = c.dt; D.dtmin = c.dtmin; D.bad = c.bad; D.first_bad = c.first_bad;

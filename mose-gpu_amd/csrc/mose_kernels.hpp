@@ -10,6 +10,7 @@
 //   k_dt         one thread per interior cell: the CFL / VNN limit and a wavefront + atomic min
 #pragma once
 
+#include "mose_bcdev.hpp"
 #include "mose_physics.hpp"
 
 namespace mose {
@@ -32,8 +33,10 @@ struct CaseDev {
   int64_t NBP;
   const mose_blk* blk;
   const mose_bc_record* rec;
+  const mose_farfield* ff;       // far-field table of the type-4 records
+  const signed char* btyp;       // record type of each boundary-plane position (walls mark the SA wall ghost)
   int nrec;
-  int lim, rs, visc, guard;
+  int lim, rs, visc, guard, pimp;
   double mco;
   ViscConst vc;
   const mose_dtab* tab;
@@ -250,6 +253,15 @@ __global__ void __launch_bounds__(128) k_bcrec(CaseDev C) {
       for (int s = 0; s < V::nprim; ++s) e2[s] = -(0.5 * (double)modfm2 * (Fl[s] + Fg[s]));
     }
   }
+  if (r.typ == MOSE_BC_WALL_ADIABATIC || r.typ == MOSE_BC_WALL_ISOTHERMAL || r.typ == MOSE_BC_FARFIELD) {
+    RecCells rc{c, c + gstep * st, dir, modfm};
+    bool closed = false;
+    if (r.typ == MOSE_BC_FARFIELD)
+      closed = !bc_farfield<NS, NR>(T, P, NC, C.lim, C.rs, C.mco, C.ff[r.ff], r.tvec, rc, nx, ny, nz, area, e1);
+    if (r.typ != MOSE_BC_FARFIELD || closed) bc_symmetry<NS, NR>(T, P, NC, C.dl, C.lim, C.rs, C.mco, rc, nx, ny, nz, area, e1);
+    if (C.visc && r.typ != MOSE_BC_FARFIELD)
+      bc_wall_visc<NS, NR>(T, P, NC, C.m, C.vc.prl, r.typ, r.tw, rc, nx, ny, nz, area, e2);
+  }
   int64_t pos = B.bpoff[Fm - 1] + plane_pos(B, Fm, Im, Jm, Km);
 #pragma unroll
   for (int s = 0; s < V::nprim; ++s) {
@@ -411,6 +423,42 @@ __device__ __forceinline__ void flag_bad(const CaseDev& C, int b, int i, int j, 
   atomicMin(C.first_bad, key);
 }
 
+// The Spalart-Allmaras source of one interior cell into tmp(nt) (Mod_Explicit.f90:215-220: after the boundary
+// records).  rho and mil are the cell's thermo-cache values (Mod_Thermo_Cache.f90:291-350).
+template <int NS, int NR>
+__device__ __forceinline__ void cell_sa(const CaseDev& C, const mose_blk& B, int i, int j, int k, int64_t c0, double rho,
+                                        double mil, double* tmp) {
+  if constexpr (NR > 0) {
+    using V = Vars<NS, NR>;
+    const int64_t st[3] = {1, (int64_t)B.pi, (int64_t)B.pi * B.pj};
+    const bool on[6] = {i == 1, i == B.n1, j == 1, j == B.n2, k == 1, k == B.n3};
+    int64_t nb[6];
+    bool wall[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      nb[q] = c0 + ((q & 1) ? st[q >> 1] : -st[q >> 1]);
+      wall[q] = false;
+      if (on[q]) {
+        signed char t = C.btyp[B.bpoff[q] + plane_pos(B, q + 1, i, j, k)];
+        wall[q] = (t == MOSE_BC_WALL_ADIABATIC || t == MOSE_BC_WALL_ISOTHERMAL);
+      }
+    }
+    double src = sa_source<NS, NR>(C.Pin, C.NC, C.m, c0, nb, wall, rho, mil, C.yn[c0], C.vol[c0], C.pimp != 0, *C.dt);
+    tmp[V::nt] = tmp[V::nt] - src;
+  } else {
+    (void)C; (void)B; (void)i; (void)j; (void)k; (void)c0; (void)rho; (void)mil; (void)tmp;
+  }
+}
+
+// the laminar viscosity of the thermo cache from the (guarded) cell temperature
+template <int NS>
+__device__ __forceinline__ double cell_mil(const mose_dtab& T, const double* p, double rho, double Tk) {
+  int Ti = ftrunc(Tk);
+  double mil, kl;
+  wilke<NS>(T, p, rho, Ti, Tk - Ti, mil, kl);
+  return mil;
+}
+
 template <int NS, int NR>
 __global__ void __launch_bounds__(256) k_update(CaseDev C, int b, double rkc) {
   using V = Vars<NS, NR>;
@@ -434,6 +482,7 @@ __global__ void __launch_bounds__(256) k_update(CaseDev C, int b, double rkc) {
     if (Tseed != Tseed) Tseed = 300.0;
     Tseed = dmin2(dmax2(Tseed, 1.0), 9990.0);
   }
+  if (NR > 0) cell_sa<NS, NR>(C, B, i, j, k, c0, rho, cell_mil<NS>(T, p, rho, Tseed), res);
   update_cell<NS, NR>(T, p, po, res, C.vol[c0], *C.dt, rkc, Tseed, out);
   bool ok = out[V::np] >= 0.0;
 #pragma unroll
@@ -456,6 +505,18 @@ __global__ void __launch_bounds__(256) k_residual_out(CaseDev C, int b, double* 
   double res[V::nprim];
   cell_residual<NS, NR>(C, B, i, j, k, res);
   int64_t c0 = row(B, i, j, k);
+  if (NR > 0) {
+    const mose_dtab& T = *C.tab;
+    double p[V::nprim];
+#pragma unroll
+    for (int s = 0; s < V::nprim; ++s) p[s] = C.Pin[s * C.NC + c0];
+    double rho = rho_sum<NS>(p), Tk = p[V::np] / (rho * f_Rtot<NS>(T, p));
+    if (C.guard && !(Tk > 1.0 && Tk < 9990.0)) {
+      if (Tk != Tk) Tk = 300.0;
+      Tk = dmin2(dmax2(Tk, 1.0), 9990.0);
+    }
+    cell_sa<NS, NR>(C, B, i, j, k, c0, rho, cell_mil<NS>(T, p, rho, Tk), res);
+  }
 #pragma unroll
   for (int s = 0; s < V::nprim; ++s) R[s * C.NC + c0] = res[s];
 }

@@ -369,3 +369,37 @@ extern "C" int mose_metrics_build(int nb, const int* dims, const double* nodes, 
   }
   return ok ? MOSE_OK : MOSE_ERR_MESH;
 }
+
+// The unit tangent of each far-field face: the diagonal node(Fi,Fj,Fk) - node(one back along both transverse
+// directions), over its norm (Setup_BC_Device, Mod_BC.f90:228-240).  The reference takes norm2 of the diagonal;
+// here it is sqrt of the sum of squares in index order.
+extern "C" int mose_records_tangent(int nb, const int* dims, const double* nodes, int nrec, mose_bc_record* rec) {
+  if (nb <= 0 || !dims || !nodes || (nrec > 0 && !rec)) return MOSE_ERR_ARG;
+  std::vector<BlockGeo> B(nb);
+  int64_t NN = 0;
+  for (int b = 0; b < nb; ++b) {
+    int n1 = dims[3 * b], n2 = dims[3 * b + 1], n3 = dims[3 * b + 2];
+    if (n1 < 1 || n2 < 1 || n3 < 1) return MOSE_ERR_ARG;
+    B[b] = BlockGeo{n1, n2, n3, 0, 0, 0, 0, n1 + 2 * GC, n2 + 2 * GC, nodes + NN};
+    NN += 3 * (int64_t)(n1 + 1) * (n2 + 1) * (n3 + 1);
+  }
+  for (int e = 0; e < nrec; ++e) {
+    mose_bc_record& r = rec[e];
+    if (r.typ != MOSE_BC_FARFIELD) continue;
+    if (r.b < 0 || r.b >= nb || r.f < 1 || r.f > 6) return MOSE_ERR_ARG;
+    const BlockGeo& G = B[r.b];
+    int Fi = r.i, Fj = r.j, Fk = r.k;
+    if (r.f <= 2) Fi = r.i - r.f % 2;
+    else if (r.f <= 4) Fj = r.j - r.f % 2;
+    else Fk = r.k - r.f % 2;
+    V3 a = G.nd(Fi, Fj, Fk), c;
+    if (r.f <= 2) c = G.nd(Fi, Fj - 1, Fk - 1);
+    else if (r.f <= 4) c = G.nd(Fi - 1, Fj, Fk - 1);
+    else c = G.nd(Fi - 1, Fj - 1, Fk);
+    V3 t = sub(a, c);
+    double tm = sqrt(t.c[0] * t.c[0] + t.c[1] * t.c[1] + t.c[2] * t.c[2]);
+    if (!(tm > 0.0)) return MOSE_ERR_MESH;
+    for (int q = 0; q < 3; ++q) r.tvec[q] = t.c[q] / tm;
+  }
+  return MOSE_OK;
+}

@@ -301,6 +301,10 @@ __global__ void __launch_bounds__(256) k_stage(CaseDev C, StageArgs A) {
       double rho = rho_sum<NS>(p), Rg = f_Rtot<NS>(T, p);
       Tseed = p[V::np] / (rho * Rg);
     }
+    if (NR > 0) {
+      double rho = cache[(ca + 1) + (TI + 2) * ((cb + 1) + (TJ + 2) * (cc + 1))];
+      cell_sa<NS, NR>(C, B, ci, cj, ck, c0, rho, cell_mil<NS>(T, p, rho, Tseed), tmp);
+    }
     update_cell<NS, NR>(T, p, po, tmp, C.vol[c0], *C.dt, A.rkc, Tseed, out);
     bool good = out[V::np] >= 0.0;
 #pragma unroll

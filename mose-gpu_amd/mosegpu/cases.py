@@ -5,6 +5,9 @@ h = cp T, FLINT's T = 0..10000 K rows; MOSE boundary types 400 -> 11, 201 -> 1 p
   sod(n)            1D shock tube, n x 1 x 1 on x in [0, 1], minmod + HLLC, SSP-RK3, CFL 0.5, transmissive ends
   taylor_green(n)   3D periodic n^3 on [0, 2 pi]^3, M = 0.1, Re = 1600, Roe (LMRoe with Mco = 1) + van Albada,
                     Sutherland viscosity tabulated per Kelvin, Pr = 0.72, SSP-RK3, seeded velocity perturbation
+  channel_lattice() a lattice of blocks of seeded uneven widths with one-to-one interfaces (MOSE 201 -> 1 without a
+                    period), a no-slip wall at y = 0 (301 -> 5 adiabatic or 302 -> 6 isothermal), far field (407 -> 4)
+                    on the other faces, Spalart-Allmaras, M = 0.3, Re = 1e6
 """
 from __future__ import annotations
 
@@ -12,8 +15,9 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .core import (BC_CONNECTION, BC_EXTRAPOLATION, GC, LIM, NTAB, RECORD_DTYPE, RS, Geometry, Layout, Scheme,
-                   ThermoTables, metrics_build)
+from .core import (BC_CONNECTION, BC_EXTRAPOLATION, BC_FARFIELD, BC_WALL_ADIABATIC, BC_WALL_ISOTHERMAL,
+                   FARFIELD_DTYPE, GC, LIM, NTAB, RECORD_DTYPE, RS, Geometry, Layout, Scheme, ThermoTables,
+                   metrics_build, records_tangent)
 
 RUNIV = 8314.46261815324
 WM_AIR = 28.970418          # test/1D/Sod79/INPUT/phase.txt
@@ -122,6 +126,7 @@ class Case:
     P: np.ndarray                    # initial state [nprim, NC]
     nsteps: int
     meta: dict = field(default_factory=dict)
+    farfield: np.ndarray = field(default_factory=lambda: np.zeros(0, dtype=FARFIELD_DTYPE))
 
     @property
     def nprim(self):
@@ -198,3 +203,145 @@ def taylor_green(n: int = 64, nsteps: int = 100, mach: float = 0.1, re: float = 
     fill_ghosts_nearest(layout, P)
     return Case("taylor_green", tables, sch, layout, nodes, rec, geom, P, nsteps,
                 meta={"V0": V0, "rho0": rho0, "mu0": mu0, "a0": a0})
+
+
+def channel_lattice(nb=(4, 4, 4), wmin: int = 24, wmax: int = 96, seed: int = 7, nsteps: int = 200, mach: float = 0.3,
+                    re: float = 1e6, T_inf: float = 300.0, p_inf: float = 101325.0, wall: int = BC_WALL_ADIABATIC,
+                    T_wall: float = 300.0, extent=(1.0, 0.3, 0.3), stretch: float = 3.0, riemann="HLLC",
+                    limiter="vanalbada", cfl: float = 0.5, vnn: float = 0.5, point_implicit: int = 1,
+                    widths=None) -> Case:
+    """Channel-like flow over a no-slip wall on a lattice of nb[0] x nb[1] x nb[2] blocks.  The per-axis block widths
+    are seeded integers in [wmin, wmax] (widths overrides them); block (a, b, c) has w_x[a] x w_y[b] x w_z[c] cells.
+    Interfaces are one-to-one connections, the y = 0 face is a wall, every other outer face is far field with the
+    free stream's stagnation state (inflow) and its static pressure (outflow).  The grid is uniform in x and z and
+    stretched towards the wall in y.  State: free stream at M = mach with a tanh boundary layer of thickness
+    0.02 extent[1], a 1e-3 seeded velocity perturbation, and rho nu~ = rho nu U(1, 5) (seeded).  Spalart-Allmaras,
+    SSP-RK3, Pr = 0.72, Re = rho U L / mu with L = 1."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    if widths is None:
+        widths = [rng.integers(wmin, wmax + 1, size=n).tolist() for n in nb]
+    wx, wy, wz = (list(map(int, w)) for w in widths)
+    gam = CP_AIR / (CP_AIR - R_AIR)
+    a_inf = np.sqrt(gam * R_AIR * T_inf)
+    U = mach * a_inf
+    rho_inf = p_inf / (R_AIR * T_inf)
+    mu_inf = rho_inf * U * 1.0 / re
+    tables = air_tables(sutherland(mu_inf, T_inf), prandtl=0.72)
+    # global node lines
+    Nx, Ny, Nz = sum(wx), sum(wy), sum(wz)
+    gx = extent[0] * np.arange(Nx + 1) / Nx
+    eta = np.arange(Ny + 1) / Ny
+    gy = extent[1] * (np.exp(stretch * eta) - 1.0) / (np.exp(stretch) - 1.0)
+    gz = extent[2] * np.arange(Nz + 1) / Nz
+    ox = np.concatenate([[0], np.cumsum(wx)]); oy = np.concatenate([[0], np.cumsum(wy)])
+    oz = np.concatenate([[0], np.cumsum(wz)])
+    nbx, nby, nbz = len(wx), len(wy), len(wz)
+
+    def bid(a, b, c):
+        return a + nbx * (b + nby * c)
+
+    dims, nodes, recs = [], [], []
+    for c in range(nbz):
+        for b in range(nby):
+            for a in range(nbx):
+                n1, n2, n3 = wx[a], wy[b], wz[c]
+                dims.append((n1, n2, n3))
+                xs, ys, zs = gx[ox[a]:ox[a] + n1 + 1], gy[oy[b]:oy[b] + n2 + 1], gz[oz[c]:oz[c] + n3 + 1]
+                Z, Y, X = np.meshgrid(zs, ys, xs, indexing="ij")
+                nodes.append(np.stack([X, Y, Z], axis=-1))
+                ft, don = {}, {}
+                nbr = {1: (a - 1, b, c), 2: (a + 1, b, c), 3: (a, b - 1, c), 4: (a, b + 1, c), 5: (a, b, c - 1),
+                       6: (a, b, c + 1)}
+                for f, (aa, bb, cc) in nbr.items():
+                    if 0 <= aa < nbx and 0 <= bb < nby and 0 <= cc < nbz:
+                        ft[f] = BC_CONNECTION
+                        m1, m2, m3 = wx[aa], wy[bb], wz[cc]
+                        fs = f + 1 if f % 2 == 1 else f - 1
+                        if f == 1:
+                            fmap = (lambda m1: lambda i, j, k: (np.full_like(i, m1), j, k))(m1)
+                        elif f == 2:
+                            fmap = lambda i, j, k: (np.full_like(i, 1), j, k)
+                        elif f == 3:
+                            fmap = (lambda m2: lambda i, j, k: (i, np.full_like(j, m2), k))(m2)
+                        elif f == 4:
+                            fmap = lambda i, j, k: (i, np.full_like(j, 1), k)
+                        elif f == 5:
+                            fmap = (lambda m3: lambda i, j, k: (i, j, np.full_like(k, m3)))(m3)
+                        else:
+                            fmap = lambda i, j, k: (i, j, np.full_like(k, 1))
+                        don[f] = (bid(aa, bb, cc), fs, fmap, False)
+                    elif f == 3:
+                        ft[f] = wall
+                    else:
+                        ft[f] = BC_FARFIELD
+                r = block_records(bid(a, b, c), (n1, n2, n3), ft, don)
+                r["ff"] = np.where(r["typ"] == BC_FARFIELD, 0, -1)
+                r["tw"] = np.where(r["typ"] == BC_WALL_ISOTHERMAL, T_wall, 0.0)
+                recs.append(r)
+    rec = np.ascontiguousarray(np.concatenate(recs))
+    layout = Layout(dims)
+    records_tangent(layout, nodes, rec)
+    geom = metrics_build(layout, nodes, rec, ndir=3)
+    # far field: stagnation state of the free stream (MOSE_open 407: T0, p0, pamb, alpha, beta, rel_fac, ci, rans)
+    T0 = T_inf * (1.0 + 0.5 * (gam - 1.0) * mach ** 2)
+    p0 = p_inf * (T0 / T_inf) ** (gam / (gam - 1.0))
+    ff = np.zeros(1, dtype=FARFIELD_DTYPE)
+    ff["mach"] = 0.0; ff["t0"] = T0; ff["p0"] = p0; ff["pamb"] = p_inf; ff["rel_fac"] = 1.0
+    ff["ci"][0, 0] = 1.0
+    ff["rans"][0, 0] = 3.0 * mu_inf / rho_inf
+    sch = make_scheme(nrans=1, limiter=limiter, riemann=riemann, viscous=True, ndir=3, cfl=cfl, vnn=vnn,
+                      prandtl=0.72, point_implicit=point_implicit)
+    P = np.zeros((6, layout.NC))
+    delta = 0.02 * extent[1]
+    for bix, nd in enumerate(nodes):
+        ctr = 0.125 * (nd[:-1, :-1, :-1] + nd[1:, :-1, :-1] + nd[:-1, 1:, :-1] + nd[:-1, :-1, 1:] + nd[1:, 1:, :-1]
+                       + nd[1:, :-1, 1:] + nd[:-1, 1:, 1:] + nd[1:, 1:, 1:])          # (n3, n2, n1, 3)
+        yc = ctr[..., 1]
+        V = layout.interior(P, bix)
+        pert = rng.uniform(-1.0, 1.0, size=(3,) + yc.shape)
+        V[0] = rho_inf
+        V[1] = U * np.tanh(yc / delta) + 1e-3 * U * pert[0]
+        V[2] = 1e-3 * U * pert[1]
+        V[3] = 1e-3 * U * pert[2]
+        V[4] = p_inf
+        V[5] = mu_inf * rng.uniform(1.0, 5.0, size=yc.shape)
+        layout.interior(geom.yn, bix)[...] = yc
+    fill_ghosts_nearest(layout, P)
+    return Case("channel_lattice", tables, sch, layout, nodes, rec, geom, P, nsteps,
+                meta={"U": U, "rho_inf": rho_inf, "mu_inf": mu_inf, "widths": (wx, wy, wz), "T0": T0, "p0": p0},
+                farfield=ff)
+
+
+def box_case(dims, faces, extent=(1.0, 1.0, 1.0), nrans: int = 0, viscous: bool = True, mu_of_T=None,
+             prandtl: float = 0.72, farfield=None, T_wall: float = 300.0, ndir: int = 3, nsteps: int = 1,
+             **scheme) -> Case:
+    """One Cartesian block [0, extent] with a boundary type per face: faces[f] (f = 1..6) is a type id, or "periodic"
+    for a connection to the opposite face (both faces of the pair must say so).  farfield: one mose_farfield row
+    (FARFIELD_DTYPE) used by every type-4 face.  The state is left zero (set it with layout.interior)."""
+    n1, n2, n3 = dims
+    nprim = 5 + nrans
+    tables = air_tables(mu_of_T if mu_of_T is not None else (sutherland(1.8e-5, 300.0) if viscous else None),
+                        prandtl=prandtl)
+    layout = Layout([dims])
+    nodes = [box_nodes(n1, n2, n3, (0.0, 0.0, 0.0), extent)]
+    opp = {1: (2, lambda i, j, k: (np.full_like(i, n1), j, k)), 2: (1, lambda i, j, k: (np.full_like(i, 1), j, k)),
+           3: (4, lambda i, j, k: (i, np.full_like(j, n2), k)), 4: (3, lambda i, j, k: (i, np.full_like(j, 1), k)),
+           5: (6, lambda i, j, k: (i, j, np.full_like(k, n3))), 6: (5, lambda i, j, k: (i, j, np.full_like(k, 1)))}
+    ft, don = {}, {}
+    for f in range(1, 7):
+        if faces[f] == "periodic":
+            ft[f] = BC_CONNECTION
+            fs, fmap = opp[f]
+            don[f] = (0, fs, fmap, True)
+        else:
+            ft[f] = int(faces[f])
+    rec = np.ascontiguousarray(block_records(0, dims, ft, don))
+    rec["ff"] = np.where(rec["typ"] == BC_FARFIELD, 0, -1)
+    rec["tw"] = np.where(rec["typ"] == BC_WALL_ISOTHERMAL, T_wall, 0.0)
+    records_tangent(layout, nodes, rec)
+    geom = metrics_build(layout, nodes, rec, ndir=ndir)
+    kw = dict(limiter="minmod", riemann="HLLC", viscous=viscous, ndir=ndir, prandtl=prandtl if viscous else 0.0)
+    kw.update(scheme)
+    sch = make_scheme(nrans=nrans, **kw)
+    ff = farfield if farfield is not None else np.zeros(0, dtype=FARFIELD_DTYPE)
+    return Case("box", tables, sch, layout, nodes, rec, geom, np.zeros((nprim, layout.NC)), nsteps, farfield=ff)

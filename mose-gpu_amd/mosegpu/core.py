@@ -33,10 +33,19 @@ RECORD_DTYPE = np.dtype(
     [("b", "<i4"), ("i", "<i4"), ("j", "<i4"), ("k", "<i4"), ("f", "<i4"), ("typ", "<i4"),
      ("bs", "<i4"), ("is_", "<i4"), ("js", "<i4"), ("ks", "<i4"), ("fs", "<i4"),
      ("d11", "<i4"), ("d12", "<i4"), ("d21", "<i4"), ("d22", "<i4"), ("periodic", "<i4"),
-     ("tw", "<f8"), ("farfield", "<f8", (8,))],
+     ("ff", "<i4"), ("pad", "<i4"), ("tw", "<f8"), ("tvec", "<f8", (3,))],
     align=True,
 )
-assert RECORD_DTYPE.itemsize == 136
+assert RECORD_DTYPE.itemsize == 104
+
+MAX_SPECIES = 16
+# the data of a far-field / inflow-outflow face (mose_farfield)
+FARFIELD_DTYPE = np.dtype(
+    [("mach", "<f8"), ("t0", "<f8"), ("p0", "<f8"), ("pamb", "<f8"), ("alpha", "<f8"), ("beta", "<f8"),
+     ("rel_fac", "<f8"), ("reserved", "<f8"), ("ci", "<f8", (MAX_SPECIES,)), ("rans", "<f8", (2,))],
+    align=True,
+)
+assert FARFIELD_DTYPE.itemsize == 208
 
 
 class MoseError(RuntimeError):
@@ -105,8 +114,9 @@ def lib() -> C.CDLL:
     L.mose_ctx_create.argtypes = [ip, C.POINTER(Tables), C.POINTER(vp)]
     L.mose_ctx_destroy.argtypes = [vp]
     L.mose_metrics_build.argtypes = [ip, C.POINTER(C.c_int), dp, ip, C.c_void_p, ip, dp, dp, dp, dp, dp, dp, dp]
+    L.mose_records_tangent.argtypes = [ip, C.POINTER(C.c_int), dp, ip, C.c_void_p]
     L.mose_case_create.argtypes = [vp, C.POINTER(Scheme), ip, C.POINTER(C.c_int), dp, dp, dp, dp, dp, dp, dp, ip,
-                                   C.c_void_p, C.POINTER(C.c_int)]
+                                   C.c_void_p, ip, C.c_void_p, C.POINTER(C.c_int)]
     L.mose_case_destroy.argtypes = [vp, ip]
     L.mose_case_bytes.argtypes = [C.POINTER(Scheme), ip, C.POINTER(C.c_int), ip, C.POINTER(C.c_int64)]
     L.mose_state_upload.argtypes = [vp, ip, dp]
@@ -121,7 +131,7 @@ def lib() -> C.CDLL:
     L.mose_case_tune.argtypes = [vp, ip, C.POINTER(Tuning), C.POINTER(Tuning)]
     L.mose_bandwidth_probe.argtypes = [vp, C.c_int64, ip, dp]
     L.mose_ctx_info.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int), C.POINTER(C.c_int)]
-    for name in ("mose_ctx_create", "mose_ctx_destroy", "mose_metrics_build", "mose_case_create",
+    for name in ("mose_ctx_create", "mose_ctx_destroy", "mose_metrics_build", "mose_records_tangent", "mose_case_create",
                  "mose_case_destroy", "mose_case_bytes", "mose_state_upload", "mose_state_download",
                  "mose_case_step", "mose_cases_step", "mose_case_residual", "mose_case_dt",
                  "mose_case_residual_norm", "mose_case_state_hash", "mose_case_timing", "mose_device_count",
@@ -190,6 +200,16 @@ def metrics_build(layout: Layout, nodes: list[np.ndarray], records: np.ndarray, 
     if rc != MOSE_OK:
         raise MoseError(rc, "mose_metrics_build failed (degenerate cell or bad record)")
     return Geometry(layout, dl, m, vol, yn, fm1, fm2, fm3)
+
+
+def records_tangent(layout: Layout, nodes: list[np.ndarray], records: np.ndarray) -> None:
+    """The unit face tangent of the far-field records, in place (Setup_BC_Device, Mod_BC.f90:228-240)."""
+    flat = np.ascontiguousarray(np.concatenate([np.ascontiguousarray(n, dtype=np.float64).reshape(-1) for n in nodes]))
+    assert records.dtype == RECORD_DTYPE and records.flags.c_contiguous
+    rc = lib().mose_records_tangent(layout.nb, iptr(layout.dims_arr), dptr(flat), len(records),
+                                    records.ctypes.data_as(C.c_void_p))
+    if rc != MOSE_OK:
+        raise MoseError(rc, "mose_records_tangent failed")
 
 
 # ------------------------------------------------------------------------------------------------------------
@@ -263,8 +283,9 @@ class DeviceCase:
         cid = C.c_int(-1)
         ctx.check(lib().mose_case_create(ctx.h, C.byref(case.scheme), case.layout.nb, iptr(case.layout.dims_arr),
                                          dptr(g.dl), dptr(g.m), dptr(g.vol), dptr(g.yn), dptr(g.fm1), dptr(g.fm2),
-                                         dptr(g.fm3), len(case.records),
-                                         case.records.ctypes.data_as(C.c_void_p), C.byref(cid)), "mose_case_create")
+                                         dptr(g.fm3), len(case.records), case.records.ctypes.data_as(C.c_void_p),
+                                         len(case.farfield), case.farfield.ctypes.data_as(C.c_void_p), C.byref(cid)),
+                  "mose_case_create")
         self.id = cid.value
 
     def upload(self, P: np.ndarray):

@@ -919,11 +919,476 @@ static void bc_connection_visc(const o_case *C, const double *P, double *R, cons
   for (int s = 0; s < S->nprim; ++s) PV(R, s, rrec) = PV(R, s, rrec) - 0.5 * (double)modfm2 * (Fl[s] + Fg[s]);
 }
 
+/* the two cells of a record along its face direction: the boundary cell and the first interior one */
+static void record_cells(const o_case *C, const mose_bc_record *rc, int64_t *rb, int64_t *ri) {
+  const o_block *B = &C->blk[rc->b];
+  int dir = (rc->f + 1) / 2, gstep = 2 * (rc->f % 2) - 1;
+  int ii = rc->i, jj = rc->j, kk = rc->k;
+  if (dir == 1) ii += gstep; else if (dir == 2) jj += gstep; else kk += gstep;
+  *rb = ROW(B, rc->i, rc->j, rc->k);
+  *ri = ROW(B, ii, jj, kk);
+}
+
+/* Bound_Dl = dl_f(dir, record cell) / 2, Int_Dl = dl_f(dir, interior neighbour) / 2 (bc_record_body.inc:25-26) */
+static void record_half_lengths(const o_case *C, const mose_bc_record *rc, double *bdl, double *idl) {
+  int64_t rb, ri;
+  record_cells(C, rc, &rb, &ri);
+  int d = (rc->f + 1) / 2 - 1;
+  *bdl = 0.5 * C->dl[(int64_t)d * C->NC + rb];
+  *idl = 0.5 * C->dl[(int64_t)d * C->NC + ri];
+}
+
+/* BC_Symmetry_Eul: Lib_BC.f90:640-733 (the wall pressure flux of types 3, 5, 6 and the closed inlet of type 4;
+ * momentum only).  Bound_Dl / Int_Dl = half the lengths of the boundary and interior cells (bc_record_body.inc:25-26). */
+static void bc_symmetry_eul(const o_case *C, const double *P, double *R, const mose_bc_record *rc, const double nrm[3],
+                            double area, double Bound_Dl, double Int_Dl) {
+  const o_scheme *S = &C->sc;
+  const int64_t NC = C->NC;
+  int nu = S->nu, nv = S->nv, nw = S->nw, np = S->np;
+  int modfm = rc->f % 2, modfm1 = 1 - modfm, modfm2 = 1 - 2 * modfm;
+  int64_t rb, ri;
+  record_cells(C, rc, &rb, &ri);
+  double Bd[ONPRIM], In[ONPRIM], Gh[ONPRIM], P1[ONPRIM], P3[ONPRIM], Fa[ONPRIM], Sl[ONPRIM];
+  for (int s = 0; s < S->nprim; ++s) { Bd[s] = PV(P, s, rb); In[s] = PV(P, s, ri); }
+  double Un = Bd[nu] * nrm[0] + Bd[nv] * nrm[1] + Bd[nw] * nrm[2];
+  for (int s = 0; s < S->nprim; ++s) Gh[s] = Bd[s];
+  Gh[nu] = Bd[nu] - 2.0 * Un * nrm[0];
+  Gh[nv] = Bd[nv] - 2.0 * Un * nrm[1];
+  Gh[nw] = Bd[nw] - 2.0 * Un * nrm[2];
+  for (int s = 0; s < S->nprim; ++s) {
+    P1[s] = (double)modfm * Gh[s] + (double)modfm1 * In[s];
+    P3[s] = (double)modfm * In[s] + (double)modfm1 * Gh[s];
+  }
+  double D12 = 2.0 * Bound_Dl * (double)modfm + (Bound_Dl + Int_Dl) * (double)modfm1;
+  double D23 = (Bound_Dl + Int_Dl) * (double)modfm + 2.0 * Bound_Dl * (double)modfm1;
+  for (int s = 0; s <= np; ++s) {
+    double s32 = (P3[s] - Bd[s]) / D23, s21 = (Bd[s] - P1[s]) / D12;
+    Sl[s] = o_rlimiter(S->lim, s32, s21);
+  }
+  double Dist = Bound_Dl * (double)modfm2;
+  for (int s = 0; s <= np; ++s) Fa[s] = Bd[s] + Sl[s] * Dist;
+  double rotot, Rtot;
+  co_rotot_Rtot(&C->tab, Fa, &rotot, &Rtot);
+  double Sound = f_ss_dev(&C->tab, Fa, Fa[np], rotot, Rtot);
+  Un = Fa[nu] * nrm[0] + Fa[nv] * nrm[1] + Fa[nw] * nrm[2];
+  double Vt[3] = {Fa[nu] - Un * nrm[0], Fa[nv] - Un * nrm[1], Fa[nw] - Un * nrm[2]};
+  double Un1 = Un * (double)modfm2;
+  double U1[3], U4[3];
+  for (int q = 0; q < 3; ++q) { U1[q] = Un1 * nrm[q] + Vt[q]; U4[q] = -Un1 * nrm[q] + Vt[q]; }
+  double F[5], Rg = f_Rtot(&C->tab, Fa);
+  riemann_any(C, S->rs, 0, Fa, U1[0], U1[1], U1[2], Fa[np], Sound, rotot, Fa, U4[0], U4[1], U4[2], Fa[np], Sound, rotot,
+              nrm[0], nrm[1], nrm[2], Rg, Rg, F);
+  PV(R, nu, rb) = PV(R, nu, rb) + (double)modfm2 * F[1] * area;
+  PV(R, nv, rb) = PV(R, nv, rb) + (double)modfm2 * F[2] * area;
+  PV(R, nw, rb) = PV(R, nw, rb) + (double)modfm2 * F[3] * area;
+}
+
+/* Stress_Vector_Std: Mod_Fluid.f90:20-44 */
+static void stress_std(const double G[3][3], const double n[3], double mi_l, double mi_t, double St[3]) {
+  double mi = mi_l + mi_t, lam = NTT * mi;
+  double Div = G[0][0] + G[1][1] + G[2][2], lamDiv = lam * Div;
+  double Txx = lamDiv + 2.0 * mi * G[0][0], Tyy = lamDiv + 2.0 * mi * G[1][1], Tzz = lamDiv + 2.0 * mi * G[2][2];
+  double Txy = mi * (G[0][1] + G[1][0]), Txz = mi * (G[0][2] + G[2][0]), Tyz = mi * (G[1][2] + G[2][1]);
+  St[0] = Txx * n[0] + Txy * n[1] + Txz * n[2];
+  St[1] = Txy * n[0] + Tyy * n[1] + Tyz * n[2];
+  St[2] = Txz * n[0] + Tyz * n[1] + Tzz * n[2];
+}
+
+/* BC_Wall_Heat_dev (type 5, adiabatic no-slip: Lib_BC.f90:879-982) and BC_Wall_Temperature_dev (type 6, isothermal:
+ * Lib_BC.f90:997-1115), with the Spalart-Allmaras wall flux of nu~ (nu~_wall = 0). */
+static void bc_wall_visc(const o_case *C, const double *P, double *R, const mose_bc_record *rc, const double nrm[3],
+                         double area) {
+  const o_scheme *S = &C->sc;
+  const o_tables *t = &C->tab;
+  const int64_t NC = C->NC;
+  int nu = S->nu, np = S->np, nsc = S->nsc;
+  int dir = (rc->f + 1) / 2, modfm = rc->f % 2, modfm2 = 1 - 2 * modfm, modfm3 = 2 * (2 * modfm - 1);
+  int64_t rb = ROW(&C->blk[rc->b], rc->i, rc->j, rc->k);
+  double Pb[ONPRIM], M[3][3];
+  for (int s = 0; s < S->nprim; ++s) Pb[s] = PV(P, s, rb);
+  for (int a = 0; a < 3; ++a)
+    for (int c = 0; c < 3; ++c) M[a][c] = C->m[(int64_t)(a + 3 * c) * NC + rb];
+  double rho, Rgas;
+  co_rotot_Rtot(t, Pb, &rho, &Rgas);
+  double T = Pb[np] / (rho * Rgas), mil, kl, Tw = rc->tw;
+  if (rc->typ == 5) {
+    int Ti = fint(T);
+    wilke(t, Pb, rho, Ti, T - Ti, &mil, &kl);
+  } else {
+    double rho_w = Pb[np] / (Rgas * Tw), Pw[ONSMAX];
+    for (int s = 0; s < nsc; ++s) Pw[s] = Pb[s] * rho_w / rho;
+    int Ti = fint(Tw);
+    double Td = Tw - Ti;
+    wilke(t, Pw, rho_w, Ti, Td, &mil, &kl);
+    if (S->prl > 0.0) kl = mil * f_cp_dev(t, Pw, Ti, Td, rho_w) / S->prl;
+  }
+  double Gr[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}}, G2[3][3];
+  for (int s = 0; s < 3; ++s) Gr[s][dir - 1] = Pb[nu + s] * (double)modfm3;
+  for (int kk = 0; kk < 3; ++kk)
+    for (int ii = 0; ii < 3; ++ii) G2[ii][kk] = Gr[ii][0] * M[0][kk] + Gr[ii][1] * M[1][kk] + Gr[ii][2] * M[2][kk];
+  double St[3];
+  stress_std(G2, nrm, mil, 0.0, St);
+  for (int q = 0; q < 3; ++q) PV(R, nu + q, rb) = PV(R, nu + q, rb) - (double)modfm2 * area * St[q];
+  if (rc->typ == 6) {
+    double Gd = (T - Tw) * (double)modfm3, Gn[3];
+    for (int kk = 0; kk < 3; ++kk) Gn[kk] = Gd * M[dir - 1][kk];
+    double Fnp = area * kl * (Gn[0] * nrm[0] + Gn[1] * nrm[1] + Gn[2] * nrm[2]);
+    PV(R, np, rb) = PV(R, np, rb) - (double)modfm2 * Fnp;
+  }
+  if (S->rans == 2) {
+    double gd = Pb[S->nt] / rho * (double)modfm3, g2[3];
+    for (int kk = 0; kk < 3; ++kk) g2[kk] = gd * M[dir - 1][kk];
+    double Fr = area * mil * (g2[0] * nrm[0] + g2[1] * nrm[1] + g2[2] * nrm[2]) / SA_SIGMA;
+    PV(R, S->nt, rb) = PV(R, S->nt, rb) - (double)modfm2 * Fr;
+  }
+}
+
+/* the face flux of the far-field closures from a face state (the common tails of BC_Outflow, Lib_BC.f90:1218-1242,
+ * and BC_Subsonic_Inflow, :1489-1511; `sub` selects the operand order of the latter) */
+static void farfield_tail(const o_case *C, const double *Fa, double Face_rho, double Face_Rgas, double Face_Un,
+                          const double nrm[3], double Area, int sub, double *Flux) {
+  const o_scheme *S = &C->sc;
+  const o_tables *t = &C->tab;
+  int nu = S->nu, nv = S->nv, nw = S->nw, np = S->np;
+  double He = 0.0, Face_T = Fa[np] / (Face_Rgas * Face_rho);
+  int Ti = (int)Face_T;
+  double Td = Face_T - Ti;
+  for (int s = 0; s < S->nprim; ++s) Flux[s] = 0.0;
+  for (int s = 0; s < S->nsc; ++s) {
+    He = He + Fa[s] / Face_rho * tabT(t->h, t, s, Ti, Ti + 1, Td);
+    Flux[s] = sub ? Fa[s] * Face_Un * Area : Fa[s] * Area * Face_Un;
+  }
+  double Fmass = sub ? Face_rho * Face_Un * Area : Face_rho * Area * Face_Un;
+  if (sub) {
+    Flux[nu] = Fmass * Fa[nu] + Fa[np] * Area * nrm[0];
+    Flux[nv] = Fmass * Fa[nv] + Fa[np] * Area * nrm[1];
+    Flux[nw] = Fmass * Fa[nw] + Fa[np] * Area * nrm[2];
+  } else {
+    Flux[nu] = Fa[nu] * Fmass + Fa[np] * Area * nrm[0];
+    Flux[nv] = Fa[nv] * Fmass + Fa[np] * Area * nrm[1];
+    Flux[nw] = Fa[nw] * Fmass + Fa[np] * Area * nrm[2];
+  }
+  double vt2 = Fa[nu] * Fa[nu] + Fa[nv] * Fa[nv] + Fa[nw] * Fa[nw];
+  Flux[np] = Fmass * (0.5 * vt2 + He);
+  for (int s = np + 1; s < S->nprim; ++s) Flux[s] = Fa[s] / Face_rho * Fmass;
+}
+
+/* BC_Outflow: Lib_BC.f90:1168-1243 */
+static void bc_outflow(const o_case *C, const double *Pr, double rho, double Rgas, double Sound, double Un, double Gamma,
+                       double Riem, const double nrm[3], double Area, double p_Amb, double BC_Sign, double *Flux) {
+  const o_scheme *S = &C->sc;
+  int nu = S->nu, nv = S->nv, nw = S->nw, np = S->np;
+  double Vt[3] = {Pr[nu] - Un * nrm[0], Pr[nv] - Un * nrm[1], Pr[nw] - Un * nrm[2]};
+  double Mach_N = fabs(Un / Sound), p_exit = p_Amb, Fa[ONPRIM], Face_Un, Face_rho, Face_Rgas;
+  if (Mach_N < 1.0 && p_Amb > 0.0) {
+    double delta = 0.5 * (Gamma - 1.0);
+    double denom = 1.0 / pow(1.0 + delta, Gamma / (Gamma - 1.0));
+    double pstar = Pr[np] * denom * pow(1.0 + delta * (Mach_N * Mach_N), Gamma / (Gamma - 1.0));
+    if (pstar >= p_Amb) p_exit = pstar;
+    double Isen = (Gamma - 1.0) / (2.0 * Gamma);
+    double Face_Sound = Sound * pow(p_exit / Pr[np], Isen);
+    Face_Un = (Riem - Face_Sound) / (BC_Sign * 0.5 * (Gamma - 1.0));
+    Fa[np] = p_exit;
+    Face_rho = Gamma * Fa[np] / (Face_Sound * Face_Sound);
+    for (int s = 0; s < S->nsc; ++s) Fa[s] = Pr[s] / rho * Face_rho;
+    Fa[nu] = Vt[0] + Face_Un * nrm[0];
+    Fa[nv] = Vt[1] + Face_Un * nrm[1];
+    Fa[nw] = Vt[2] + Face_Un * nrm[2];
+    for (int s = np + 1; s < S->nprim; ++s) Fa[s] = Pr[s] / rho * Face_rho;
+    Face_Rgas = Rgas;
+  } else {
+    for (int s = 0; s < S->nprim; ++s) Fa[s] = Pr[s];
+    Face_Un = Un; Face_rho = rho; Face_Rgas = Rgas;
+  }
+  farfield_tail(C, Fa, Face_rho, Face_Rgas, Face_Un, nrm, Area, 0, Flux);
+}
+
+/* BC_Supersonic_Inflow: Lib_BC.f90:1247-1330 */
+static void bc_supersonic_inflow(const o_case *C, const double *P2, const double *In, int modfm, const mose_farfield *ff,
+                                 const double nrm[3], double Area, double *Flux) {
+  const o_scheme *S = &C->sc;
+  const o_tables *t = &C->tab;
+  int nu = S->nu, nv = S->nv, nw = S->nw, np = S->np, nsc = S->nsc, modfm1 = 1 - modfm, modfm2 = 1 - 2 * modfm;
+  double flow_Rgas = f_Rtot(t, P2);
+  double vmag2 = P2[nu] * P2[nu] + P2[nv] * P2[nv] + P2[nw] * P2[nw];
+  double rsum = 0.0;
+  for (int s = 0; s < nsc; ++s) rsum = rsum + P2[s];
+  double flow_Sound = f_ss_dev(t, P2, P2[np], rsum, flow_Rgas);
+  double flow_Mach = sqrt(vmag2) / flow_Sound;
+  double rel = ff->rel_fac;
+  double Mloc = rel * ff->mach + (1.0 - rel) * flow_Mach;
+  double p0loc = rel * ff->p0 + (1.0 - rel) * P2[np];
+  double Sup[ONPRIM], Sup_T = ff->t0, Sup_Rgas = 0.0;
+  Sup[np] = p0loc;
+  for (int s = 0; s < nsc; ++s) Sup_Rgas = Sup_Rgas + ff->ci[s] * t->Ri[s];
+  double Sup_rho = Sup[np] / (Sup_Rgas * Sup_T);
+  for (int s = 0; s < nsc; ++s) Sup[s] = Sup_rho * ff->ci[s];
+  double Sup_Sound = f_ss_dev(t, Sup, Sup[np], Sup_rho, Sup_Rgas);
+  Sup[nu] = Mloc * Sup_Sound * cos(ff->alpha) * cos(ff->beta);
+  Sup[nv] = Mloc * Sup_Sound * sin(ff->alpha) * cos(ff->beta);
+  Sup[nw] = Mloc * Sup_Sound * sin(ff->beta);
+  for (int s = 0; s < S->nrans; ++s) Sup[S->nt + s] = Sup_rho * ff->rans[s];
+  double L[ONPRIM], Rr[ONPRIM];
+  for (int s = 0; s < S->nprim; ++s) {
+    double p1 = (double)modfm * Sup[s] + (double)modfm1 * In[s];
+    double p3 = (double)modfm * In[s] + (double)modfm1 * Sup[s];
+    double Slope = o_rlimiter(S->lim, p3 - P2[s], P2[s] - p1);
+    double Fa = P2[s] + 0.5 * Slope * (double)modfm2;
+    L[s] = (double)modfm * Sup[s] + (double)modfm1 * Fa;
+    Rr[s] = (double)modfm * Fa + (double)modfm1 * Sup[s];
+  }
+  double rho_L, Rg_L, rho_R, Rg_R;
+  co_rotot_Rtot(t, L, &rho_L, &Rg_L);
+  double aL = f_ss_dev(t, L, L[np], rho_L, Rg_L);
+  co_rotot_Rtot(t, Rr, &rho_R, &Rg_R);
+  double aR = f_ss_dev(t, Rr, Rr[np], rho_R, Rg_R);
+  double F[5];
+  riemann_any(C, S->rs, 0, L, L[nu], L[nv], L[nw], L[np], aL, rho_L, Rr, Rr[nu], Rr[nv], Rr[nw], Rr[np], aR, rho_R,
+              nrm[0], nrm[1], nrm[2], f_Rtot(t, L), f_Rtot(t, Rr), F);
+  double su = copysign(1.0, F[0]), SelL = (1.0 + su) / 2.0, SelR = (su - 1.0) / 2.0, Fmass = F[0] * Area;
+  for (int s = 0; s < S->nprim; ++s) Flux[s] = 0.0;
+  for (int s = 0; s < nsc; ++s) Flux[s] = Fmass * (SelL * L[s] / rho_L - SelR * Rr[s] / rho_R);
+  Flux[nu] = F[1] * Area; Flux[nv] = F[2] * Area; Flux[nw] = F[3] * Area; Flux[np] = F[4] * Area;
+  for (int s = np + 1; s < S->nprim; ++s) Flux[s] = Fmass * (SelL * L[s] / rho_L - SelR * Rr[s] / rho_R);
+}
+
+/* BC_Subsonic_Inflow: Lib_BC.f90:1335-1512.  Returns 1 when the face must close (the caller applies the symmetry). */
+static int bc_subsonic_inflow(const o_case *C, const double *Pr, double Un, double Sound, double Gamma, double Riem,
+                              const mose_farfield *ff, double BC_Sign, const double tv[3], const double nrm[3],
+                              double Area, double *Flux) {
+  const o_scheme *S = &C->sc;
+  const o_tables *t = &C->tab;
+  int nu = S->nu, nv = S->nv, nw = S->nw, np = S->np, nsc = S->nsc;
+  double bv[3] = {nrm[1] * tv[2] - nrm[2] * tv[1], nrm[2] * tv[0] - nrm[0] * tv[2], nrm[0] * tv[1] - nrm[1] * tv[0]};
+  double bm = sqrt(bv[0] * bv[0] + bv[1] * bv[1] + bv[2] * bv[2]);
+  bv[0] = bv[0] / bm; bv[1] = bv[1] / bm; bv[2] = bv[2] / bm;
+  double alpha, beta;
+  if (ff->alpha == 0.0 && ff->beta == 0.0) {
+    alpha = atan(nrm[1] / (nrm[0] + 1e-20));
+    beta = atan(nrm[2] / (nrm[0] + 1e-20));
+  } else {
+    alpha = ff->alpha; beta = ff->beta;
+  }
+  double ta = tan(alpha), tb = tan(beta);
+  double XA = nrm[1] - nrm[0] * ta, XB = tv[1] - tv[0] * ta, XC = bv[1] - bv[0] * ta;
+  double XD = nrm[2] - nrm[0] * tb, XE = tv[2] - tv[0] * tb, XF = bv[2] - bv[0] * tb;
+  double check = (XC * XE - XF * XB);
+  double E1 = (XF * XA - XC * XD) / check, E2 = (XD * XB - XE * XA) / check;
+  double Fa[ONPRIM], Face_rho, Face_Rgas, Face_Un;
+  for (int s = 0; s < S->nprim; ++s) Fa[s] = 0.0;
+  if (ff->mach >= 0.0) {
+    double Discr = -1.0, p0t = ff->p0, Sound0 = 0.0, Coef_D = 0.0, YA = 0.0, YB = 0.0;
+    int it = 0;
+    Face_Rgas = 0.0;
+    while (Discr < 0.0) {
+      it = it + 1;
+      Face_Rgas = 0.0;
+      for (int s = 0; s < nsc; ++s) Face_Rgas = Face_Rgas + ff->ci[s] * t->Ri[s];
+      Sound0 = sqrt(Gamma * Face_Rgas * ff->t0);
+      Coef_D = (1.0 + E1 * E1 + E2 * E2) * (Gamma - 1.0) / 2.0;
+      double Coef_C = pow(p0t / Pr[np], (Gamma - 1.0) / Gamma);
+      double sr = Sound / Sound0;
+      Coef_C = Coef_C * (sr * sr);
+      double gh = (Gamma - 1.0) / 2.0;
+      YA = gh * gh + Coef_C * Coef_D;
+      YB = BC_Sign * Riem * ((Gamma - 1.0) / 2.0);
+      double YC = Riem * Riem - Coef_C * (Sound0 * Sound0);
+      Discr = YB * YB - YA * YC;
+      if (Discr < 0.0 && fabs(Discr) < 1e-2) Discr = 0.0;
+      if (Discr < 0.0 && fabs(Discr) > 1e-2) {
+        if (ff->rel_fac == 1.0) { Discr = 0.0; break; }
+        p0t = ff->rel_fac * p0t + (1.0 - ff->rel_fac) * Pr[np];
+      }
+      if (it > 1000) { Discr = 0.0; break; }
+    }
+    double Un_Sign = copysign(1.0, Un);
+    double Un3 = (YB + Un_Sign * sqrt(Discr)) / YA;
+    double Sound3 = sqrt(Sound0 * Sound0 - Coef_D * (Un3 * Un3));
+    double p3 = p0t * pow(Sound3 / Sound0, 2.0 * Gamma / (Gamma - 1.0));
+    double Ut3 = Un3 * E1, Ub3 = Un3 * E2;
+    Fa[np] = p3;
+    Face_rho = Gamma * Fa[np] / (Sound3 * Sound3);
+    for (int s = 0; s < nsc; ++s) Fa[s] = ff->ci[s] * Face_rho;
+    Fa[nu] = Un3 * nrm[0] + Ut3 * tv[0] + Ub3 * bv[0];
+    Fa[nv] = Un3 * nrm[1] + Ut3 * tv[1] + Ub3 * bv[1];
+    Fa[nw] = Un3 * nrm[2] + Ut3 * tv[2] + Ub3 * bv[2];
+    Face_Un = Un3;
+  } else {
+    double mdot = ff->p0, p3 = Pr[np], Rgas3 = 0.0, cp3 = 0.0;
+    int Ti = (int)ff->t0;
+    double Td = ff->t0 - Ti;
+    for (int s = 0; s < nsc; ++s) {
+      cp3 = cp3 + ff->ci[s] * tabT(t->cp, t, s, Ti, Ti + 1, Td);
+      Rgas3 = Rgas3 + ff->ci[s] * t->Ri[s];
+    }
+    if (ff->mach != -15.0) {
+      double T3;
+      if (ff->mach == -10.0) {
+        double q = mdot * Rgas3 / p3, q2 = p3 / (mdot * Rgas3);
+        T3 = (-cp3 + sqrt(cp3 * cp3 + 2.0 * cp3 * ff->t0 * (q * q))) * (q2 * q2);
+      } else {
+        T3 = ff->t0;
+      }
+      double rho3 = p3 / (Rgas3 * T3), Un3 = mdot / rho3, Ut3 = Un3 * E1, Ub3 = Un3 * E2;
+      Face_Rgas = Rgas3; Face_rho = rho3;
+      for (int s = 0; s < nsc; ++s) Fa[s] = Face_rho * ff->ci[s];
+      Fa[nu] = Un3 * nrm[0] + Ut3 * tv[0] + Ub3 * bv[0];
+      Fa[nv] = Un3 * nrm[1] + Ut3 * tv[1] + Ub3 * bv[1];
+      Fa[nw] = Un3 * nrm[2] + Ut3 * tv[2] + Ub3 * bv[2];
+      Fa[np] = p3;
+      Face_Un = Un3;
+    } else {
+      Face_Rgas = Rgas3;
+      Face_rho = p3 / (Face_Rgas * ff->t0);
+      for (int s = 0; s < nsc; ++s) Fa[s] = Face_rho * ff->ci[s];
+      Fa[nu] = mdot / Face_rho; Fa[nv] = 0.0; Fa[nw] = Fa[nu] * 0.25; Fa[np] = p3;
+      Face_Un = Fa[nu] * nrm[0] + Fa[nv] * nrm[1] + Fa[nw] * nrm[2];
+    }
+  }
+  for (int s = 0; s < S->nrans; ++s) Fa[S->nt + s] = ff->rans[s] * Face_rho;
+  farfield_tail(C, Fa, Face_rho, Face_Rgas, Face_Un, nrm, Area, 1, Flux);
+  return 0;
+}
+
+/* BC_Inflow_Outflow: Lib_BC.f90:1519-1597.  Returns 1 when the face closes to a reflective wall. */
+static int bc_inflow_outflow(const o_case *C, const double *P, double *R, const mose_bc_record *rc, const double nrm[3],
+                             double Area) {
+  const o_scheme *S = &C->sc;
+  const o_tables *t = &C->tab;
+  const int64_t NC = C->NC;
+  int nu = S->nu, nv = S->nv, nw = S->nw, np = S->np;
+  int modfm = rc->f % 2, modfm2 = 1 - 2 * modfm;
+  const mose_farfield *ff = &C->ff[rc->ff];
+  int64_t rb, ri;
+  record_cells(C, rc, &rb, &ri);
+  double Bd[ONPRIM], In[ONPRIM], Flux[ONPRIM];
+  for (int s = 0; s < S->nprim; ++s) { Bd[s] = PV(P, s, rb); In[s] = PV(P, s, ri); }
+  double BC_Sign = (double)modfm2;
+  double Un = Bd[nu] * nrm[0] + Bd[nv] * nrm[1] + Bd[nw] * nrm[2];
+  double rho, Rg;
+  co_rotot_Rtot(t, Bd, &rho, &Rg);
+  double Sound = f_ss_dev(t, Bd, Bd[np], rho, Rg);
+  double T = Bd[np] / (rho * Rg);
+  int Ti = (int)T;
+  double cp = f_cp_dev(t, Bd, Ti, T - Ti, rho);
+  double Gamma = cp / (cp - Rg);
+  double Riem = Sound + BC_Sign * 0.5 * (Gamma - 1.0) * Un;
+  double InOut = BC_Sign * Un;
+  if (ff->mach < 0.0) InOut = -1.0;
+  if (InOut >= 0.0) {
+    if (ff->pamb == -3.14 || ff->pamb == -2.71) return 1;
+    bc_outflow(C, Bd, rho, Rg, Sound, Un, Gamma, Riem, nrm, Area, ff->pamb, BC_Sign, Flux);
+  } else {
+    if (ff->mach > 1.0 || ff->pamb == -2.71) {
+      bc_supersonic_inflow(C, Bd, In, modfm, ff, nrm, Area, Flux);
+    } else if (ff->mach < 1.0) {
+      if (bc_subsonic_inflow(C, Bd, Un, Sound, Gamma, Riem, ff, BC_Sign, rc->tvec, nrm, Area, Flux)) return 1;
+    } else {
+      return 0;   /* Mach exactly 1: no closure applies and the reference adds nothing defined; rejected at set-up */
+    }
+  }
+  for (int s = 0; s < S->nprim; ++s) PV(R, s, rb) = PV(R, s, rb) + (double)modfm2 * Flux[s];
+  return 0;
+}
+
+/* ------------------------------------------------------------------------------------------------ */
+/* Spalart-Allmaras: the wall ghost of Extrapolate_Wall_blk (Lib_Extrapolation.f90:128-190, eddy_dev 2) and the   */
+/* source of SA_Source_Blk + sa_body.inc (Mod_Spalart.f90:105-151; constants :14-34)                             */
+/* ------------------------------------------------------------------------------------------------ */
+#define SA_CB1 0.1355
+#define SA_CB2 0.622
+#define SA_CW2 0.3
+#define SA_CW3 2.0
+#define SA_CK 0.41
+#define SA_C2 0.7
+#define SA_C3 0.9
+
+static void extrapolate_wall(const o_case *C, double *P) {
+  const o_scheme *S = &C->sc;
+  const int64_t NC = C->NC;
+  for (int e = 0; e < C->nrec; ++e) {
+    const mose_bc_record *rc = &C->rec[e];
+    if (rc->typ != 5 && rc->typ != 6) continue;
+    const o_block *B = &C->blk[rc->b];
+    int Ig = rc->i - guide[rc->f - 1][0], Jg = rc->j - guide[rc->f - 1][1], Kg = rc->k - guide[rc->f - 1][2];
+    int64_t ci = ROW(B, rc->i, rc->j, rc->k), cg = ROW(B, Ig, Jg, Kg);
+    for (int s = S->nu; s <= S->nw; ++s) PV(P, s, cg) = -PV(P, s, ci);
+    for (int s = 0; s < S->nsc; ++s) PV(P, s, cg) = PV(P, s, ci);
+    PV(P, S->np, cg) = PV(P, S->np, ci);
+    PV(P, S->nt, cg) = -PV(P, S->nt, ci);
+  }
+}
+
+static void sa_source(const o_case *C, const double *P, double *R, const double *rc, const double *mc, double dt) {
+  const o_scheme *S = &C->sc;
+  const int64_t NC = C->NC;
+  const double cw1 = SA_CB1 / (SA_CK * SA_CK) + (1.0 + SA_CB2) / SA_SIGMA, cK_sq = SA_CK * SA_CK;
+  const double cv1_3 = SA_CV1 * SA_CV1 * SA_CV1, cw3_6 = 64.0;
+  int nt = S->nt, nu = S->nu, nsc = S->nsc;
+  for (int b = 0; b < C->nb; ++b) {
+    const o_block *B = &C->blk[b];
+    int64_t st[3] = {1, B->pi, (int64_t)B->pi * B->pj};
+#pragma omp parallel for collapse(2) schedule(static) num_threads(C->nthreads > 0 ? C->nthreads : 1)
+    for (int k = 1; k <= B->n3; ++k)
+      for (int j = 1; j <= B->n2; ++j)
+        for (int i = 1; i <= B->n1; ++i) {
+          int64_t c0 = ROW(B, i, j, k);
+          double rho = rc[c0], mil = mc[c0];
+          double nit = PV(P, nt, c0) / rho, chi = PV(P, nt, c0) / mil, dist = C->yn[c0];
+          double M[3][3], Gt[3][3], Gv[3][3];
+          for (int a = 0; a < 3; ++a)
+            for (int c = 0; c < 3; ++c) M[a][c] = C->m[(int64_t)(a + 3 * c) * NC + c0];
+          for (int d = 0; d < 3; ++d)
+            for (int a = 0; a < 3; ++a) Gt[a][d] = (PV(P, nu + a, c0 + st[d]) - PV(P, nu + a, c0 - st[d])) * 0.5;
+          for (int c = 0; c < 3; ++c)
+            for (int a = 0; a < 3; ++a) Gv[a][c] = Gt[a][0] * M[0][c] + Gt[a][1] * M[1][c] + Gt[a][2] * M[2][c];
+          double v1 = Gv[2][1] - Gv[1][2], v2 = Gv[0][2] - Gv[2][0], v3 = Gv[1][0] - Gv[0][1];
+          double Om = sqrt(v1 * v1 + v2 * v2 + v3 * v3);
+          double chi3 = chi * chi * chi;
+          double fv1 = chi3 / (chi3 + cv1_3);
+          double fv2 = 1.0 - chi / (1.0 + chi * fv1);
+          double Sbar = nit / (cK_sq * (dist * dist)) * fv2, Stilde;
+          if (Sbar >= (-SA_C2 * Om)) Stilde = Om + Sbar;
+          else Stilde = Om + Om * (SA_C2 * SA_C2 * Om + SA_C3 * Sbar) / ((SA_C3 - 2.0 * SA_C2) * Om - Sbar);
+          if (Stilde == 0.0) Stilde = 1e-40;
+          double Prod = SA_CB1 * Stilde * nit;
+          double sum[6];
+          for (int q = 0; q < 6; ++q) {
+            int64_t r = c0 + (q % 2 ? st[q / 2] : -st[q / 2]);
+            double ssum = 0.0;
+            for (int s = 0; s < nsc; ++s) ssum = ssum + PV(P, s, r);
+            sum[q] = ssum;
+          }
+          double Gn[3];
+          for (int d = 0; d < 3; ++d)
+            Gn[d] = (PV(P, nt, c0 + st[d]) / sum[2 * d + 1] - PV(P, nt, c0 - st[d]) / sum[2 * d]) * 0.5;
+          double Gr[3];
+          for (int c = 0; c < 3; ++c) Gr[c] = Gn[0] * M[0][c] + Gn[1] * M[1][c] + Gn[2] * M[2][c];
+          double Gr2 = Gr[0] * Gr[0] + Gr[1] * Gr[1] + Gr[2] * Gr[2];
+          double Diff = SA_CB2 / SA_SIGMA * Gr2;
+          double r_ = nit / (Stilde * cK_sq * (dist * dist));
+          r_ = min2(r_, 10.0);
+          double r2 = r_ * r_, r6 = r2 * r2 * r2;
+          double g = r_ + SA_CW2 * (r6 - r_);
+          double g2 = g * g, g6 = g2 * g2 * g2;
+          double fw = g * pow((1.0 + cw3_6) / (g6 + cw3_6), 1.0 / 6.0);
+          double nd = nit / dist;
+          double Dest = cw1 * fw * (nd * nd);
+          double Src = rho * (Prod + Diff - Dest) * C->vol[c0];
+          if (S->point_implicit) {
+            double fnu = 1.0 / (1.0 + dt * 2.0 * cw1 * fw * nit / (dist * dist));
+            Src = Src * fnu;
+          }
+          PV(R, nt, c0) = PV(R, nt, c0) - Src;
+        }
+  }
+}
+
 /* ------------------------------------------------------------------------------------------------ */
 /* residual: Fluxes (conv + diff pass 1, pass 2 in the order of pass2_body.inc:1-61) then BC_Fluxes  */
 /* in face order 1..6 (Mod_BC.f90:1042-1062 bc_launch_ranges)                                         */
 /* ------------------------------------------------------------------------------------------------ */
-void o_residual(const o_case *C, const double *P, double *R) {
+void o_residual(const o_case *C, double *P, double *R, double dt) {
   const o_scheme *S = &C->sc;
   const int64_t NC = C->NC;
   int nprim = S->nprim;
@@ -1018,9 +1483,29 @@ void o_residual(const o_case *C, const double *P, double *R) {
       case 11:
         bc_extrapolation(C, P, R, rc2, nrm, area);
         break;
+      case 5:
+      case 6: {
+        double bdl, idl;
+        record_half_lengths(C, rc2, &bdl, &idl);
+        bc_symmetry_eul(C, P, R, rc2, nrm, area, bdl, idl);
+        if (S->visc) bc_wall_visc(C, P, R, rc2, nrm, area);
+        break;
+      }
+      case 4:
+        if (bc_inflow_outflow(C, P, R, rc2, nrm, area)) {
+          double bdl, idl;
+          record_half_lengths(C, rc2, &bdl, &idl);
+          bc_symmetry_eul(C, P, R, rc2, nrm, area, bdl, idl);
+        }
+        break;
       default:
         break;
     }
+  }
+  /* RANS (Mod_Explicit.f90:215-220): the wall ghost of nu~ on the state, then the source */
+  if (S->rans == 2) {
+    extrapolate_wall(C, P);
+    sa_source(C, P, R, rc, mc, dt);
   }
   free(rc);
 }
@@ -1178,7 +1663,7 @@ int o_step(const o_case *C, double *P, double *PO, double *R, int nsteps, double
     memcpy(PO, P, sizeof(double) * NC * nprim);
     for (int irk = 1; irk <= C->sc.n_rk; ++irk) {
       o_fill_ghosts(C, P);
-      o_residual(C, P, R);
+      o_residual(C, P, R, dt);
       o_newstate(C, P, PO, R, dt, rk_coeff(C->sc.n_rk, irk));
     }
   }

@@ -75,6 +75,8 @@ typedef struct {
   const double *dl, *m, *vol, *yn, *fm1, *fm2, *fm3;
   int nrec;
   const mose_bc_record *rec;    /* grouped by block, then face 1..6 */
+  int nff;
+  const mose_farfield *ff;      /* far-field table of the type-4 records (rec->ff) */
   int nthreads;
 } o_case;
 

@@ -9,7 +9,7 @@
 #include "mose_oracle.h"
 
 void o_fill_ghosts(const o_case *C, double *P);
-void o_residual(const o_case *C, const double *P, double *R);
+void o_residual(const o_case *C, double *P, double *R, double dt);
 double o_compute_dt(const o_case *C, const double *P);
 void o_newstate(const o_case *C, double *P, const double *PO, const double *R, double dt, double rkc);
 int o_step(const o_case *C, double *P, double *PO, double *R, int nsteps, double *dt_out);
@@ -25,6 +25,7 @@ typedef struct {
   o_case c;
   o_block *blk;
   mose_bc_record *rec;
+  mose_farfield *ff;
   double *tabs;
 } oa_t;
 
@@ -50,7 +51,8 @@ static void tables_init(o_tables *t, const mose_tables *tb, double *store) {
 
 void *oa_create(const mose_scheme *sc, const mose_tables *tb, int nb, const int *dims, const double *dl,
                 const double *m, const double *vol, const double *yn, const double *fm1, const double *fm2,
-                const double *fm3, int nrec, const mose_bc_record *rec, int nthreads) {
+                const double *fm3, int nrec, const mose_bc_record *rec, int nff, const mose_farfield *ff,
+                int nthreads) {
   if (tb->nsc < 1 || tb->nsc > ONSMAX) return NULL;
   oa_t *a = (oa_t *)calloc(1, sizeof *a);
   o_case *C = &a->c;
@@ -80,6 +82,9 @@ void *oa_create(const mose_scheme *sc, const mose_tables *tb, int nb, const int 
   a->rec = (mose_bc_record *)malloc(sizeof(mose_bc_record) * (size_t)(nrec > 0 ? nrec : 1));
   if (nrec > 0) memcpy(a->rec, rec, sizeof(mose_bc_record) * (size_t)nrec);
   C->nrec = nrec; C->rec = a->rec;
+  a->ff = (mose_farfield *)malloc(sizeof(mose_farfield) * (size_t)(nff > 0 ? nff : 1));
+  if (nff > 0) memcpy(a->ff, ff, sizeof(mose_farfield) * (size_t)nff);
+  C->nff = nff; C->ff = a->ff;
   C->nthreads = nthreads;
   return a;
 }
@@ -87,16 +92,17 @@ void *oa_create(const mose_scheme *sc, const mose_tables *tb, int nb, const int 
 void oa_destroy(void *h) {
   oa_t *a = (oa_t *)h;
   if (!a) return;
-  free(a->blk); free(a->rec); free(a->tabs); free(a);
+  free(a->blk); free(a->rec); free(a->ff); free(a->tabs); free(a);
 }
 
 int64_t oa_ncell(void *h) { return ((oa_t *)h)->c.NC; }
 
 void oa_fill_ghosts(void *h, double *P) { o_fill_ghosts(&((oa_t *)h)->c, P); }
-void oa_residual(void *h, double *P, double *R) {
+/* dt: the time step the Spalart-Allmaras point-implicit factor reads (sa_body.inc:113-116); 0 otherwise */
+void oa_residual(void *h, double *P, double *R, double dt) {
   oa_t *a = (oa_t *)h;
   o_fill_ghosts(&a->c, P);
-  o_residual(&a->c, P, R);
+  o_residual(&a->c, P, R, dt);
 }
 double oa_dt(void *h, const double *P) { return o_compute_dt(&((oa_t *)h)->c, P); }
 void oa_newstate(void *h, double *P, const double *PO, const double *R, double dt, double rkc) {

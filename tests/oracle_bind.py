@@ -23,12 +23,12 @@ def oracle_lib():
     L = C.CDLL(str(ORACLE_LIB))
     vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.c_int
     L.oa_create.restype = vp
-    L.oa_create.argtypes = [vp, vp, ip, C.POINTER(C.c_int), dp, dp, dp, dp, dp, dp, dp, ip, vp, ip]
+    L.oa_create.argtypes = [vp, vp, ip, C.POINTER(C.c_int), dp, dp, dp, dp, dp, dp, dp, ip, vp, ip, vp, ip]
     L.oa_destroy.argtypes = [vp]
     L.oa_ncell.restype = C.c_int64
     L.oa_ncell.argtypes = [vp]
     L.oa_fill_ghosts.argtypes = [vp, dp]
-    L.oa_residual.argtypes = [vp, dp, dp]
+    L.oa_residual.argtypes = [vp, dp, dp, C.c_double]
     L.oa_dt.restype = C.c_double
     L.oa_dt.argtypes = [vp, dp]
     L.oa_newstate.argtypes = [vp, dp, dp, dp, C.c_double, C.c_double]
@@ -60,7 +60,8 @@ class OracleCase:
         self.h = L.oa_create(C.cast(C.pointer(case.scheme), C.c_void_p), C.cast(C.pointer(self._t), C.c_void_p),
                              case.layout.nb, case.layout.dims_arr.ctypes.data_as(C.POINTER(C.c_int)),
                              _d(g.dl), _d(g.m), _d(g.vol), _d(g.yn), _d(g.fm1), _d(g.fm2), _d(g.fm3),
-                             len(case.records), case.records.ctypes.data_as(C.c_void_p), nthreads)
+                             len(case.records), case.records.ctypes.data_as(C.c_void_p),
+                             len(case.farfield), case.farfield.ctypes.data_as(C.c_void_p), nthreads)
         assert self.h, "oa_create failed"
 
     def step(self, P: np.ndarray, nsteps: int):
@@ -68,9 +69,9 @@ class OracleCase:
         oracle_lib().oa_step(self.h, _d(P), nsteps, _d(dts))
         return dts[:nsteps]
 
-    def residual(self, P: np.ndarray) -> np.ndarray:
+    def residual(self, P: np.ndarray, dt: float = 0.0) -> np.ndarray:
         R = np.zeros_like(P)
-        oracle_lib().oa_residual(self.h, _d(P), _d(R))
+        oracle_lib().oa_residual(self.h, _d(P), _d(R), dt)
         return R
 
     def dt(self, P: np.ndarray) -> float:
