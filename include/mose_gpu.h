@@ -179,11 +179,15 @@ int mose_case_residual(mose_ctx* ctx, int case_id, double* R);
 /* Global dt of the current state (Compute_dt + the min reduction). */
 int mose_case_dt(mose_ctx* ctx, int case_id, double* dt);
 
-/* Sum over the interior cells of (sum of species, u, v, w, p, RANS)^2 differences between the current
- * state and the step base (Compute_Residual_blk, Mod_Diagnostic.f90:63-106): out[nprim]. */
+/* The L2 norm of the last step's change, per variable (Compute_Residual_blk, Mod_Diagnostic.f90:63-106, then the
+ * sqrt of Mod_Explicit.f90:358): out[q] = sqrt(sum over the interior cells of d_q^2), d_0 = the sum of the species
+ * changes, d_1.. = the changes of u, v, w, p and the RANS variables; nprim - nsc + 1 values.  The last stage of
+ * every step leaves per-workgroup partial sums; this call adds them in a fixed order, so it is deterministic. */
 int mose_case_residual_norm(mose_ctx* ctx, int case_id, double* out);
 
-/* Order-independent hash of the interior state (field_state_hash, MOSE_Field_m.f90:254-284). */
+/* field_state_hash (MOSE_Field_m.f90:254-284): h_lo = the sum of the low 32 bits and h_hi = the sum of the high
+ * 32 bits of the IEEE patterns of every interior value (all variables), as wrapping 64-bit integers.
+ * Order-independent: equal states give equal hashes on every launch order. */
 int mose_case_state_hash(mose_ctx* ctx, int case_id, uint64_t* h_lo, uint64_t* h_hi);
 
 /* Execution path and tile shape of a case.  path: MOSE_PATH_FUSED (default) runs one fused stage kernel per RK

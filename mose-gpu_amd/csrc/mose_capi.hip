@@ -60,6 +60,11 @@ struct Case {
   int ntiles = 0, tiles_cap = 0;
   unsigned long long* dtnext = nullptr;
   bool dt_valid = false;
+  // residual norm of the last step (per-workgroup partials of the last stage) and the state hash slots
+  double* rpart = nullptr;
+  int64_t rpart_cap = 0, nparts_last = 0;
+  double* rnorm = nullptr;
+  unsigned long long* hslot = nullptr;
   // timing
   bool timing = false;
   std::vector<hipEvent_t> tev;
@@ -165,6 +170,9 @@ extern "C" int mose_ctx_create(int device, const mose_tables* t, mose_ctx** out)
 static void case_free(Case* c) {
   if (!c) return;
   if (c->slab) (void)hipFree(c->slab);
+  if (c->rpart) (void)hipFree(c->rpart);
+  c->rpart = nullptr;
+  c->rpart_cap = 0;
   if (c->dtiles) (void)hipFree(c->dtiles);
   c->dtiles = nullptr;
   for (auto& e : c->tev)
@@ -312,6 +320,7 @@ static size_t case_bytes(const Case& c) {
   b += c.blk.size() * sizeof(mose_blk) + c.rec.size() * sizeof(mose_bc_record) + 8192 + 64 * 1024;
   b += (std::max<size_t>(c.ff.size(), 1) * sizeof(mose_farfield) + 255) / 256 * 256;
   b += ((size_t)std::max<int64_t>(c.NBP, 1) + 255) / 256 * 256;
+  b += 64 * sizeof(double) + 256 + 512;
   return b;
 }
 
@@ -376,8 +385,27 @@ static int alloc_case(mose_ctx* ctx, Case& c) {
   c.bad = (unsigned int*)take(64);
   c.first_bad = (unsigned long long*)take(64);
   c.dtnext = (unsigned long long*)take(64);
+  c.rnorm = (double*)take(64 * sizeof(double));
+  c.hslot = (unsigned long long*)take(64);
   if ((size_t)(p - (char*)c.slab) > total) return fail(ctx, MOSE_ERR_NOMEM, "internal: slab overflow");
   for (auto& e : c.ev) HIPCHK(ctx, hipEventCreate(&e));
+  return MOSE_OK;
+}
+
+// workgroups of the per-block update kernels (the plain path's partials of the residual norm)
+static int64_t plain_parts(const Case& c) {
+  int64_t n = 0;
+  for (auto& B : c.blk) n += ((int64_t)B.n1 * B.n2 * B.n3 + 255) / 256;
+  return n;
+}
+
+static int ensure_rpart(mose_ctx* ctx, Case& c, int64_t nparts) {
+  if (nparts <= c.rpart_cap) return MOSE_OK;
+  if (c.rpart) HIPCHK(ctx, hipFree(c.rpart));
+  c.rpart = nullptr;
+  c.rpart_cap = 0;
+  HIPCHK(ctx, hipMalloc(&c.rpart, (size_t)nparts * (c.nprim - c.sc.nsc + 1) * sizeof(double)));
+  c.rpart_cap = nparts;
   return MOSE_OK;
 }
 
@@ -388,6 +416,8 @@ static Case* get_case(mose_ctx* ctx, int id) {
 }
 
 static size_t stage_lds_(int nprim, bool visc, int TI, int TJ, int TK);
+// static LDS of the stage kernel beside its dynamic part (the workgroup reductions of dt and of the residual norm)
+static constexpr size_t kStaticLds = 2048;
 // the first tile of the preference list that fits the device's LDS and does not straddle a one-cell direction
 static void choose_default_tile(mose_ctx* ctx, Case& c) {
   static const int pref[][3] = {{8, 8, 4}, {16, 4, 4}, {32, 4, 2}, {16, 8, 2}, {32, 8, 1}, {64, 4, 1}, {16, 16, 1},
@@ -396,7 +426,7 @@ static void choose_default_tile(mose_ctx* ctx, Case& c) {
   for (auto& B : c.blk) { mx[0] = std::max(mx[0], B.n1); mx[1] = std::max(mx[1], B.n2); mx[2] = std::max(mx[2], B.n3); }
   for (auto& t : pref) {
     if ((t[1] > 1 && mx[1] == 1) || (t[2] > 1 && mx[2] == 1)) continue;
-    if (stage_lds_(c.nprim, c.sc.viscous != 0, t[0], t[1], t[2]) > (size_t)ctx->max_lds) continue;
+    if (stage_lds_(c.nprim, c.sc.viscous != 0, t[0], t[1], t[2]) + kStaticLds > (size_t)ctx->max_lds) continue;
     c.TI = t[0]; c.TJ = t[1]; c.TK = t[2];
     return;
   }
@@ -436,6 +466,8 @@ extern "C" int mose_case_create(mose_ctx* ctx, const mose_scheme* sc, int nb, co
   HIPCHK(ctx, hipMemcpyAsync(c->dbtyp, c->btyp.data(), c->btyp.size(), hipMemcpyHostToDevice, s));
   HIPCHK(ctx, hipMemsetAsync(c->P0, 0, 3 * (size_t)c->NC * c->nprim * d, s));
   HIPCHK(ctx, hipStreamSynchronize(s));
+  rc = ensure_rpart(ctx, *c, plain_parts(*c));
+  if (rc != MOSE_OK) { case_free(c.get()); return rc; }
   c->valid = true;
   choose_default_tile(ctx, *c);
   ctx->cases.push_back(std::move(c));
@@ -547,7 +579,7 @@ static size_t stage_lds_(int nprim, bool visc, int TI, int TJ, int TK) {
 
 static int set_tiles(mose_ctx* ctx, Case& c, int TI, int TJ, int TK) {
   if (TI < 1 || TJ < 1 || TK < 1 || TI * TJ * TK != 256) return fail(ctx, MOSE_ERR_ARG, "tile must hold 256 cells");
-  if (stage_lds(c, TI, TJ, TK) > (size_t)ctx->max_lds)
+  if (stage_lds(c, TI, TJ, TK) + kStaticLds > (size_t)ctx->max_lds)
     return fail(ctx, MOSE_ERR_ARG, "tile " + std::to_string(TI) + "x" + std::to_string(TJ) + "x" + std::to_string(TK) +
                                        " needs " + std::to_string(stage_lds(c, TI, TJ, TK)) + " B of LDS, the device allows " +
                                        std::to_string(ctx->max_lds));
@@ -568,7 +600,7 @@ static int set_tiles(mose_ctx* ctx, Case& c, int TI, int TJ, int TK) {
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   c.ntiles = (int)t.size();
   c.TI = TI; c.TJ = TJ; c.TK = TK;
-  return MOSE_OK;
+  return ensure_rpart(ctx, c, c.ntiles);
 }
 
 template <int NS, int NR>
@@ -579,7 +611,7 @@ static void launch_stage(Case& c, const CaseDev& D, hipStream_t s, double rkc, b
   A.nhs = hs_slots(c.TI, c.TJ, c.TK);
   A.ncache = (c.TI + 2) * (c.TJ + 2) * (c.TK + 2);
   A.r0 = stage_r0(c, c.TI, c.TJ, c.TK);
-  A.last = last ? 1 : 0; A.rkc = rkc; A.dtnext = c.dtnext;
+  A.last = last ? 1 : 0; A.rkc = rkc; A.dtnext = c.dtnext; A.rpart = c.rpart;
   size_t lds = stage_lds(c, c.TI, c.TJ, c.TK);
   unsigned grid = (unsigned)(((c.ntiles + 7) / 8) * 8);
   const int rs = c.sc.riemann;
@@ -608,7 +640,7 @@ static int run_steps(mose_ctx* ctx, Case& c, int nsteps) {
     int rc = set_tiles(ctx, c, c.TI, c.TJ, c.TK);
     if (rc != MOSE_OK) return rc;
   }
-  if (fused && stage_lds(c, c.TI, c.TJ, c.TK) > (size_t)ctx->max_lds) return fail(ctx, MOSE_ERR_ARG, "tile does not fit");
+  if (fused && stage_lds(c, c.TI, c.TJ, c.TK) + kStaticLds > (size_t)ctx->max_lds) return fail(ctx, MOSE_ERR_ARG, "tile does not fit");
   HIPCHK(ctx, hipMemsetAsync(c.bad, 0, 2 * sizeof(unsigned), s));
   HIPCHK(ctx, hipMemsetAsync(c.first_bad, 0xff, sizeof(unsigned long long), s));
   size_t nev = c.timing ? (size_t)2 * nsteps * nrk : 0;
@@ -637,15 +669,19 @@ static int run_steps(mose_ctx* ctx, Case& c, int nsteps) {
         if (last) k_dt_init<<<1, 64, 0, s>>>(c.dtnext);
         if (c.timing) HIPCHK(ctx, hipEventRecord(c.tev[iev++], s));
         launch_stage<NS, NR>(c, D, s, rkc, last);
+        if (last) c.nparts_last = c.ntiles;
         if (c.timing) HIPCHK(ctx, hipEventRecord(c.tev[iev++], s));
         if (last) k_dt_final<<<1, 64, 0, s>>>(c.dtnext, c.dt, nullptr, 0);
       } else {
         launch_flux<NS, NR>(c, D, s);
         if (c.timing) HIPCHK(ctx, hipEventRecord(c.tev[iev++], s));
+        int64_t part0 = 0;
         for (int b = 0; b < c.nb; ++b) {
           int64_t ncell = (int64_t)c.blk[b].n1 * c.blk[b].n2 * c.blk[b].n3;
-          k_update<NS, NR><<<nblocks(ncell, 256), 256, 0, s>>>(D, b, rkc);
+          k_update<NS, NR><<<nblocks(ncell, 256), 256, 0, s>>>(D, b, rkc, irk == nrk ? c.rpart : nullptr, part0);
+          part0 += nblocks(ncell, 256);
         }
+        if (irk == nrk) c.nparts_last = part0;
         if (c.timing) HIPCHK(ctx, hipEventRecord(c.tev[iev++], s));
       }
     }
@@ -684,7 +720,7 @@ static int autotune(mose_ctx* ctx, Case& c, mose_tuning* chosen) {
   int bi = 0;
   for (int q = 0; q < (int)(sizeof cand / sizeof cand[0]); ++q) {
     int TI = cand[q][0], TJ = cand[q][1], TK = cand[q][2];
-    if (stage_lds(c, TI, TJ, TK) > (size_t)ctx->max_lds) continue;
+    if (stage_lds(c, TI, TJ, TK) + kStaticLds > (size_t)ctx->max_lds) continue;
     if ((TJ > 1 && dims_max[1] == 1) || (TK > 1 && dims_max[2] == 1)) continue;   // no tile across a 1-cell direction
     if (set_tiles(ctx, c, TI, TJ, TK) != MOSE_OK) continue;
     CaseDev D = make_dev(ctx, c, c.P0, c.PA);
@@ -820,13 +856,43 @@ extern "C" int mose_case_dt(mose_ctx* ctx, int id, double* dt) {
 }
 
 extern "C" int mose_case_residual_norm(mose_ctx* ctx, int id, double* out) {
-  (void)ctx; (void)id; (void)out;
-  return fail(ctx, MOSE_ERR_ARG, "mose_case_residual_norm: not implemented in this build");
+  Case* c = get_case(ctx, id);
+  if (!c || !out) return MOSE_ERR_ARG;
+  if (c->steps == 0 || c->nparts_last == 0) return fail(ctx, MOSE_ERR_ARG, "mose_case_residual_norm: no step taken yet");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  int nres = c->nprim - c->sc.nsc + 1;
+  k_rnorm_final<<<1, 64, 0, ctx->stream>>>(c->rpart, c->nparts_last, nres, c->rnorm);
+  HIPCHK(ctx, hipGetLastError());
+  double h[64];
+  HIPCHK(ctx, hipMemcpyAsync(h, c->rnorm, nres * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  for (int q = 0; q < nres; ++q) out[q] = std::sqrt(h[q]);   // Mod_Explicit.f90:358
+  return MOSE_OK;
+}
+
+template <int NS, int NR>
+static void launch_hash(Case& c, hipStream_t s) {
+  for (int b = 0; b < c.nb; ++b) {
+    const mose_blk& B = c.blk[b];
+    int64_t n = (int64_t)B.n1 * B.n2 * B.n3;
+    k_hash<NS, NR><<<nblocks(n, 256), 256, 0, s>>>(c.P0, c.NC, B, c.hslot);
+  }
 }
 
 extern "C" int mose_case_state_hash(mose_ctx* ctx, int id, uint64_t* lo, uint64_t* hi) {
-  (void)id; (void)lo; (void)hi;
-  return fail(ctx, MOSE_ERR_ARG, "mose_case_state_hash: not implemented in this build");
+  Case* c = get_case(ctx, id);
+  if (!c || !lo || !hi) return MOSE_ERR_ARG;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  HIPCHK(ctx, hipMemsetAsync(c->hslot, 0, 2 * sizeof(unsigned long long), ctx->stream));
+  if (c->sc.nrans == 0) launch_hash<1, 0>(*c, ctx->stream);
+  else launch_hash<1, 1>(*c, ctx->stream);
+  HIPCHK(ctx, hipGetLastError());
+  unsigned long long h[2];
+  HIPCHK(ctx, hipMemcpyAsync(h, c->hslot, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  *lo = h[0];
+  *hi = h[1];
+  return MOSE_OK;
 }
 
 extern "C" int mose_case_timing(mose_ctx* ctx, int id, double* avg, double* total, int64_t* launches) {

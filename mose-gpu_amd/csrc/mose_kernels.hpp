@@ -423,6 +423,88 @@ __device__ __forceinline__ void flag_bad(const CaseDev& C, int b, int i, int j, 
   atomicMin(C.first_bad, key);
 }
 
+// The squared changes of one cell over the step (Compute_Residual_blk, Mod_Diagnostic.f90:63-106): the species
+// sum, then u, v, w, p and the RANS variables.
+template <int NS, int NR>
+__device__ __forceinline__ void step_change(const double* pnew, const double* pold, double* r) {
+  using V = Vars<NS, NR>;
+  double d = 0.0;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) d = d + (pnew[s] - pold[s]);
+  r[0] = d * d;
+#pragma unroll
+  for (int s = V::nu; s < V::nprim; ++s) {
+    double e = pnew[s] - pold[s];
+    r[s - NS + 1] = e * e;
+  }
+}
+
+// Workgroup sums of N values in a fixed order (wavefront butterfly, then the waves in order); thread 0 stores them.
+// Every thread of the workgroup must call it.
+template <int N>
+__device__ __forceinline__ void block_sum_store(const double* v, double* out) {
+  __shared__ double ws[16][N > 0 ? N : 1];
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < N; ++q) {
+    double x = v[q];
+    for (int o = 32; o > 0; o >>= 1) x = x + __shfl_xor(x, o, 64);
+    if (lane == 0) ws[w][q] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+      double t = ws[0][q];
+      for (int k = 1; k < nw; ++k) t = t + ws[k][q];
+      out[q] = t;
+    }
+  }
+  __syncthreads();
+}
+
+// sums of the per-workgroup partials in index order (one thread per component)
+__global__ void k_rnorm_final(const double* part, int64_t nparts, int nres, double* out) {
+  int q = threadIdx.x;
+  if (blockIdx.x != 0 || q >= nres) return;
+  double t = 0.0;
+  for (int64_t i = 0; i < nparts; ++i) t = t + part[i * nres + q];
+  out[q] = t;
+}
+
+// field_state_hash (MOSE_Field_m.f90:254-284): sums of the low and of the high 32 bits of every interior value
+template <int NS, int NR>
+__global__ void __launch_bounds__(256) k_hash(const double* P, int64_t NC, mose_blk B, unsigned long long* h) {
+  using V = Vars<NS, NR>;
+  __shared__ unsigned long long s1[256], s2[256];
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long a = 0, b = 0;
+  if (t < (int64_t)B.n1 * B.n2 * B.n3) {
+    int i = (int)(t % B.n1) + 1;
+    int64_t rem = t / B.n1;
+    int j = (int)(rem % B.n2) + 1, k = (int)(rem / B.n2) + 1;
+    int64_t c = row(B, i, j, k);
+#pragma unroll
+    for (int s = 0; s < V::nprim; ++s) {
+      unsigned long long x = (unsigned long long)__double_as_longlong(P[s * NC + c]);
+      a += x & 0xffffffffull;
+      b += x >> 32;
+    }
+  }
+  s1[threadIdx.x] = a;
+  s2[threadIdx.x] = b;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) { s1[threadIdx.x] += s1[threadIdx.x + o]; s2[threadIdx.x] += s2[threadIdx.x + o]; }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    atomicAdd(&h[0], s1[0]);
+    atomicAdd(&h[1], s2[0]);
+  }
+}
+
 // The Spalart-Allmaras source of one interior cell into tmp(nt) (Mod_Explicit.f90:215-220: after the boundary
 // records).  rho and mil are the cell's thermo-cache values (Mod_Thermo_Cache.f90:291-350).
 template <int NS, int NR>
@@ -460,11 +542,27 @@ __device__ __forceinline__ double cell_mil(const mose_dtab& T, const double* p, 
 }
 
 template <int NS, int NR>
-__global__ void __launch_bounds__(256) k_update(CaseDev C, int b, double rkc) {
+__device__ __forceinline__ void update_one(const CaseDev& C, const mose_blk& B, int b, int64_t t, double rkc,
+                                           double* dres);
+
+// one thread per interior cell of block b: residual, SA source, RK stage update, state check; on the last stage
+// also the per-workgroup sums of the squared step changes (rpart != nullptr)
+template <int NS, int NR>
+__global__ void __launch_bounds__(256) k_update(CaseDev C, int b, double rkc, double* rpart, int64_t part0) {
   using V = Vars<NS, NR>;
   const mose_blk B = C.blk[b];
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)B.n1 * B.n2 * B.n3) return;
+  double dres[V::nprim - NS + 1];
+#pragma unroll
+  for (int q = 0; q < V::nprim - NS + 1; ++q) dres[q] = 0.0;
+  if (t < (int64_t)B.n1 * B.n2 * B.n3) update_one<NS, NR>(C, B, b, t, rkc, rpart ? dres : nullptr);
+  if (rpart) block_sum_store<V::nprim - NS + 1>(dres, rpart + (part0 + blockIdx.x) * (V::nprim - NS + 1));
+}
+
+template <int NS, int NR>
+__device__ __forceinline__ void update_one(const CaseDev& C, const mose_blk& B, int b, int64_t t, double rkc,
+                                           double* dres) {
+  using V = Vars<NS, NR>;
   int i = (int)(t % B.n1) + 1;
   int64_t rem = t / B.n1;
   int j = (int)(rem % B.n2) + 1, k = (int)(rem / B.n2) + 1;
@@ -490,6 +588,7 @@ __global__ void __launch_bounds__(256) k_update(CaseDev C, int b, double rkc) {
   if (!ok) flag_bad(C, b, i, j, k, 0);
 #pragma unroll
   for (int s = 0; s < V::nprim; ++s) C.Pout[s * NC + c0] = out[s];
+  if (dres) step_change<NS, NR>(out, po, dres);
 }
 
 // the residual alone (for checks): R = pass 2 + boundary increments

@@ -34,6 +34,7 @@ struct StageArgs {
   int last;           // 1 on the last RK stage: take the dt limit of the new state
   double rkc;
   unsigned long long* dtnext;
+  double* rpart;      // last stage: per-tile sums of the squared step changes [ntiles][nprim - NS + 1]
 };
 
 __device__ __forceinline__ int xcd_tile(int wg, int n) {
@@ -272,6 +273,9 @@ __global__ void __launch_bounds__(256) k_stage(CaseDev C, StageArgs A) {
 
   // ---------------- phase D: boundary increments, update, next dt ----------------
   double dtl = 1e300;
+  double dres[NP - NS + 1];
+#pragma unroll
+  for (int q = 0; q < NP - NS + 1; ++q) dres[q] = 0.0;
   if (mine) {
     bool on[6] = {ci == 1, ci == B.n1, cj == 1, cj == B.n2, ck == 1, ck == B.n3};
 #pragma unroll
@@ -312,9 +316,15 @@ __global__ void __launch_bounds__(256) k_stage(CaseDev C, StageArgs A) {
     if (!good) flag_bad(C, tl.b, ci, cj, ck, 0);
 #pragma unroll
     for (int s = 0; s < NP; ++s) C.Pout[s * NC + c0] = out[s];
-    if (A.last) dtl = cell_dt<NS, NR>(C, c0, out);
+    if (A.last) {
+      dtl = cell_dt<NS, NR>(C, c0, out);
+      step_change<NS, NR>(out, po, dres);
+    }
   }
-  if (A.last) block_min_atomic(dtl, A.dtnext);
+  if (A.last) {
+    block_sum_store<NP - NS + 1>(dres, A.rpart + (int64_t)tidx * (NP - NS + 1));
+    block_min_atomic(dtl, A.dtnext);
+  }
 }
 
 }  // namespace mose

@@ -150,6 +150,17 @@ def device_count() -> int:
 # ------------------------------------------------------------------------------------------------------------
 # geometry
 # ------------------------------------------------------------------------------------------------------------
+def state_hash(layout: "Layout", P: np.ndarray) -> tuple[int, int]:
+    """field_state_hash (MOSE_Field_m.f90:254-284) of a host state: the wrapping 64-bit sums of the low and of the
+    high 32 bits of the IEEE patterns of every interior value."""
+    lo = hi = 0
+    for b in range(layout.nb):
+        x = np.ascontiguousarray(layout.interior(P, b)).view(np.uint64)
+        lo += int(np.sum(x & np.uint64(0xFFFFFFFF), dtype=np.uint64))
+        hi += int(np.sum(x >> np.uint64(32), dtype=np.uint64))
+    return lo % (1 << 64), hi % (1 << 64)
+
+
 class Layout:
     """Arena sizes of a list of blocks (rows of the padded cells, face counts)."""
 
@@ -311,6 +322,18 @@ class DeviceCase:
         v = C.c_double(0.0)
         self.ctx.check(lib().mose_case_dt(self.ctx.h, self.id, C.byref(v)), "mose_case_dt")
         return v.value
+
+    def residual_norm(self) -> np.ndarray:
+        """sqrt of the summed squared changes of the last step: species sum, u, v, w, p, RANS (nprim - nsc + 1)."""
+        out = np.zeros(self.case.nprim - self.case.scheme.nsc + 1)
+        self.ctx.check(lib().mose_case_residual_norm(self.ctx.h, self.id, dptr(out)), "mose_case_residual_norm")
+        return out
+
+    def state_hash(self) -> tuple[int, int]:
+        """field_state_hash: (sum of the low, sum of the high 32-bit halves) of every interior value."""
+        lo, hi = C.c_uint64(0), C.c_uint64(0)
+        self.ctx.check(lib().mose_case_state_hash(self.ctx.h, self.id, C.byref(lo), C.byref(hi)), "mose_case_state_hash")
+        return lo.value, hi.value
 
     def tune(self, path: int = PATH_FUSED, tile=(0, 0, 0), timing: bool = False):
         """Execution path and tile (0,0,0 = autotune); returns the tile chosen."""

@@ -165,3 +165,42 @@ def test_fused_equals_plain_at_bench_size(gpu_ready):
     m0 = np.sum(case.layout.interior(case.P, 0)[0] * vol)
     m1 = np.sum(b[0] * vol)
     assert abs(m1 - m0) <= 1e-12 * m0
+
+
+# --------------------------------------------------------------------------------------------------------------
+# residual norm (Compute_Residual_blk) and state hash (field_state_hash)
+# --------------------------------------------------------------------------------------------------------------
+def oracle_residual_norm(case, P_new, P_old):
+    """sqrt of sum over the interior cells of (species-sum change)^2, (u, v, w, p, RANS change)^2."""
+    ns = case.scheme.nsc
+    tot = np.zeros(case.nprim - ns + 1)
+    for b in range(case.layout.nb):
+        a = case.layout.interior(P_new, b).reshape(case.nprim, -1)
+        o = case.layout.interior(P_old, b).reshape(case.nprim, -1)
+        d = (a - o)
+        tot[0] += np.sum(d[:ns].sum(axis=0) ** 2)
+        tot[1:] += np.sum(d[ns:] ** 2, axis=1)
+    return np.sqrt(tot)
+
+
+@pytest.mark.parametrize("path", ["fused", "plain"])
+def test_residual_norm_and_hash(gpu_ready, path):
+    import mosegpu
+    case = cases.taylor_green(16, 2)
+    ctx = Context(case.tables)
+    dc = DeviceCase(ctx, case)
+    dc.upload(case.P)
+    assert dc.state_hash() == mosegpu.state_hash(case.layout, case.P)
+    dc.tune(mosegpu.PATH_FUSED if path == "fused" else mosegpu.PATH_PLAIN, (0, 0, 0))
+    dc.step(1)
+    P1 = dc.download()
+    dc.step(1)
+    P2 = dc.download()
+    rn = dc.residual_norm()
+    ref = oracle_residual_norm(case, P2, P1)
+    assert np.allclose(rn, ref, rtol=1e-12, atol=0), (rn, ref)
+    assert dc.state_hash() == mosegpu.state_hash(case.layout, P2)
+    # the oracle's state after two steps is bitwise the device's: same hash
+    Po = case.P.copy()
+    OracleCase(case).step(Po, 2)
+    assert mosegpu.state_hash(case.layout, Po) == dc.state_hash()
