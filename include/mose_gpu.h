@@ -212,6 +212,62 @@ int mose_case_timing(mose_ctx* ctx, int case_id, double* ms_stage_avg, double* m
  * against a kernel of known traffic and measures the box's achievable bandwidth.  No reference counterpart. */
 int mose_bandwidth_probe(mose_ctx* ctx, int64_t bytes, int reps, double* gbs);
 
+/* ---------------------------------------------------------------------------------------------------------
+ * Stiff chemistry (FLINT): per-cell constant-volume ODE integration over dt of independent cells
+ * (Chemistry_Newstate_blk, src/lib/physics/Lib_Chemistry.f90:33-76) with RODAS (hairerRodas.f ROSCOR, METH = 1,
+ * the host integrator wrap_rodas, OSlo.f90:485-527).
+ * --------------------------------------------------------------------------------------------------------- */
+
+/* A reaction mechanism in FLINT's table form (read_chemistry, Load_Chemistry.f90:7-164; the general loop of
+ * Lib_Chemistry_wdot.f90:163-288).  Every rate table has one row per Kelvin on [tmin, tmax] (row T at
+ * index T - tmin), like the thermo tables.  Per reaction r of each class, ni1/ni2/eps hold ns + 1 values:
+ * the reactant and product stoichiometric coefficients and third-body efficiencies of every species, and in
+ * slot ns the third-body flag (ni1[ns] > 0: the rate is multiplied by sum_s eps_s c_s).  Classes are stored
+ * one after the other (Arrhenius, Troe, Lindemann). */
+typedef struct {
+  int32_t ns;             /* species, <= MOSE_MAX_SPECIES */
+  int32_t n_arrh, n_troe, n_lind;
+  int32_t tmin, tmax;     /* table rows tmin..tmax (FLINT: 0..10000) */
+  const double* wm;       /* [ns] kg/kmol */
+  const double* cp;       /* [ns][rows] J/(kg K) */
+  const double* h;        /* [ns][rows] J/kg, formation enthalpy included */
+  const double* ni1;      /* [n_arrh + n_troe + n_lind][ns + 1] */
+  const double* ni2;      /* same shape */
+  const double* eps;      /* same shape */
+  const double* kf;       /* [n_arrh][rows] forward rate constants (SI: kmol, m^3, s) */
+  const double* kb;       /* [n_arrh][rows] reverse rate constants */
+  const double* troe;     /* [n_troe][4][rows]: k_inf, k_0, K_c, F_cent */
+  const double* lind;     /* [n_lind][3][rows]: k_inf, k_0, K_c */
+} mose_mechanism;
+
+enum { MOSE_ODE_RODAS = 1 };
+typedef struct {
+  int32_t method;         /* MOSE_ODE_RODAS */
+  int32_t reserved[3];
+  double rtol[MOSE_MAX_SPECIES + 1];   /* per ODE component (species..., T): raw tolerances, as wrap_rodas */
+  double atol[MOSE_MAX_SPECIES + 1];
+} mose_ode_opts;
+
+typedef struct {
+  int64_t cells;
+  int64_t failed;         /* cells with idid < 0 */
+  int64_t steps_total, rejected_total, rhs_total;
+  int32_t steps_max, first_failed_idid;
+  int64_t first_failed_cell;
+  double ms;              /* device time of the integration kernel */
+} mose_chem_report;
+
+/* A batch of independent chemistry cells resident on the device: state [ns + 1][ncell] (species densities,
+ * then pressure), the layout of the flow arena's chemistry columns. */
+int mose_chem_create(mose_ctx* ctx, const mose_mechanism* mech, int64_t ncell_max, int* chem_id);
+int mose_chem_destroy(mose_ctx* ctx, int chem_id);
+int mose_chem_upload(mose_ctx* ctx, int chem_id, int64_t ncell, const double* state);
+int mose_chem_download(mose_ctx* ctx, int chem_id, double* state, double* hr, int32_t* idid, int32_t* nstep);
+/* Every cell: T = p / (rho R), RODAS over [0, dt] on (rho_1..rho_ns, T), species floored at 1e-20, p = T rho R
+ * (Chemistry_Newstate_blk); hr = |T_new - T_old| / T_old.  One persistent launch: each lane takes the next cell
+ * from an atomic work queue when its cell is done, so cells of uneven stiffness share the device evenly. */
+int mose_chem_advance(mose_ctx* ctx, int chem_id, double dt, const mose_ode_opts* opts, mose_chem_report* report);
+
 #ifdef __cplusplus
 }
 #endif

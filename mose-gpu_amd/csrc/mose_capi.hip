@@ -940,3 +940,171 @@ extern "C" int mose_bandwidth_probe(mose_ctx* ctx, int64_t bytes, int reps, doub
   (void)hipFree(b);
   return MOSE_OK;
 }
+
+// ----------------------------------------------------------------------------------------------------------------
+// stiff chemistry: batches of independent cells (Chemistry_Newstate_blk with the host RODAS of the general path)
+// ----------------------------------------------------------------------------------------------------------------
+#include "mose_chem.hpp"
+
+namespace {
+struct ChemObj {
+  ChemDev M{};
+  void* tabs = nullptr;          // device copies of the mechanism tables (one allocation)
+  double* state = nullptr;       // [NZ][cap]
+  double* hr = nullptr;
+  int* idid = nullptr;
+  int* nstep = nullptr;
+  double* fjac = nullptr;        // [NZ*NZ][nlanes]
+  unsigned long long* ctr = nullptr;   // [0] queue, [1..5] stats
+  int64_t cap = 0, n = 0, nlanes = 0;
+  int grid = 0;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  bool valid = false;
+};
+std::vector<std::unique_ptr<ChemObj>> g_chem;   // per process; ids index this list (contexts share it)
+}  // namespace
+
+static ChemObj* get_chem(int id) {
+  if (id < 0 || id >= (int)g_chem.size() || !g_chem[id] || !g_chem[id]->valid) return nullptr;
+  return g_chem[id].get();
+}
+
+static void chem_free(ChemObj* o) {
+  if (!o) return;
+  if (o->tabs) (void)hipFree(o->tabs);
+  if (o->state) (void)hipFree(o->state);
+  if (o->fjac) (void)hipFree(o->fjac);
+  for (auto& e : o->ev)
+    if (e) (void)hipEventDestroy(e);
+  o->tabs = nullptr; o->state = nullptr; o->fjac = nullptr;
+  o->valid = false;
+}
+
+extern "C" int mose_chem_create(mose_ctx* ctx, const mose_mechanism* m, int64_t ncell_max, int* chem_id) {
+  if (!ctx || !m || !chem_id || ncell_max < 1) return MOSE_ERR_ARG;
+  if (m->ns != 9) return fail(ctx, MOSE_ERR_ARG, "this build instantiates the chemistry kernel for 9 species");
+  if (m->tmax <= m->tmin || m->n_arrh < 0 || m->n_troe < 0 || m->n_lind < 0 || !m->wm || !m->cp || !m->h ||
+      !m->ni1 || !m->ni2 || !m->eps || (m->n_arrh > 0 && (!m->kf || !m->kb)) || (m->n_troe > 0 && !m->troe) ||
+      (m->n_lind > 0 && !m->lind))
+    return fail(ctx, MOSE_ERR_ARG, "mose_chem_create: incomplete mechanism");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  auto o = std::make_unique<ChemObj>();
+  const int ns = m->ns, rows = m->tmax - m->tmin + 1, nr = m->n_arrh + m->n_troe + m->n_lind;
+  size_t n_thermo = (size_t)ns * rows, n_st = (size_t)nr * (ns + 1), n_kf = (size_t)m->n_arrh * rows;
+  size_t n_troe = (size_t)m->n_troe * 4 * rows, n_lind = (size_t)m->n_lind * 3 * rows;
+  size_t total = 2 * n_thermo + 3 * n_st + 2 * n_kf + n_troe + n_lind + 8;
+  HIPCHK(ctx, hipMalloc(&o->tabs, total * sizeof(double)));
+  double* p = (double*)o->tabs;
+  auto put = [&](const double* src, size_t cnt) -> double* {
+    double* dst = p;
+    if (cnt) (void)hipMemcpy(dst, src, cnt * sizeof(double), hipMemcpyHostToDevice);
+    p += cnt;
+    return dst;
+  };
+  ChemDev& M = o->M;
+  M.ns = ns; M.na = m->n_arrh; M.nt = m->n_troe; M.nl = m->n_lind; M.tmin = m->tmin; M.rows = rows;
+  for (int s = 0; s < ns; ++s) { M.wm[s] = m->wm[s]; M.Ri[s] = MOSE_RUNIV / m->wm[s]; }
+  M.cp = put(m->cp, n_thermo); M.h = put(m->h, n_thermo);
+  M.ni1 = put(m->ni1, n_st); M.ni2 = put(m->ni2, n_st); M.eps = put(m->eps, n_st);
+  M.kf = put(m->kf, n_kf); M.kb = put(m->kb, n_kf);
+  M.troe = put(m->troe, n_troe); M.lind = put(m->lind, n_lind);
+  HIPCHK(ctx, hipGetLastError());
+  int cus = 0;
+  HIPCHK(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+  // persistent grid: enough 64-lane workgroups to fill every SIMD a few times over
+  int64_t want = (ncell_max + 63) / 64;
+  o->grid = (int)std::min<int64_t>(want, (int64_t)cus * 16);
+  o->nlanes = (int64_t)o->grid * 64;
+  o->cap = ncell_max;
+  const int nz = ns + 1;
+  size_t sbytes = (size_t)nz * ncell_max * sizeof(double) + (size_t)ncell_max * (sizeof(double) + 2 * sizeof(int)) + 1024;
+  HIPCHK(ctx, hipMalloc(&o->state, sbytes));
+  o->hr = o->state + (size_t)nz * ncell_max;
+  o->idid = (int*)(o->hr + ncell_max);
+  o->nstep = o->idid + ncell_max;
+  o->ctr = (unsigned long long*)(((uintptr_t)(o->nstep + ncell_max) + 255) & ~(uintptr_t)255);
+  HIPCHK(ctx, hipMalloc(&o->fjac, (size_t)nz * nz * o->nlanes * sizeof(double)));
+  for (auto& e : o->ev) HIPCHK(ctx, hipEventCreate(&e));
+  o->valid = true;
+  g_chem.push_back(std::move(o));
+  *chem_id = (int)g_chem.size() - 1;
+  return MOSE_OK;
+}
+
+extern "C" int mose_chem_destroy(mose_ctx* ctx, int id) {
+  ChemObj* o = get_chem(id);
+  if (!ctx || !o) return MOSE_ERR_ARG;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  chem_free(o);
+  return MOSE_OK;
+}
+
+extern "C" int mose_chem_upload(mose_ctx* ctx, int id, int64_t ncell, const double* state) {
+  ChemObj* o = get_chem(id);
+  if (!ctx || !o || !state || ncell < 1 || ncell > o->cap) return fail(ctx, MOSE_ERR_ARG, "mose_chem_upload: bad arguments");
+  const int nz = o->M.ns + 1;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  // [nz][ncell] host rows into [nz][cap] device rows
+  for (int q = 0; q < nz; ++q)
+    HIPCHK(ctx, hipMemcpyAsync(o->state + (size_t)q * o->cap, state + (size_t)q * ncell, ncell * sizeof(double),
+                               hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  o->n = ncell;
+  return MOSE_OK;
+}
+
+extern "C" int mose_chem_download(mose_ctx* ctx, int id, double* state, double* hr, int32_t* idid, int32_t* nstep) {
+  ChemObj* o = get_chem(id);
+  if (!ctx || !o || !state) return MOSE_ERR_ARG;
+  const int nz = o->M.ns + 1;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  for (int q = 0; q < nz; ++q)
+    HIPCHK(ctx, hipMemcpyAsync(state + (size_t)q * o->n, o->state + (size_t)q * o->cap, o->n * sizeof(double),
+                               hipMemcpyDeviceToHost, ctx->stream));
+  if (hr) HIPCHK(ctx, hipMemcpyAsync(hr, o->hr, o->n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  if (idid) HIPCHK(ctx, hipMemcpyAsync(idid, o->idid, o->n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  if (nstep) HIPCHK(ctx, hipMemcpyAsync(nstep, o->nstep, o->n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return MOSE_OK;
+}
+
+extern "C" int mose_chem_advance(mose_ctx* ctx, int id, double dt, const mose_ode_opts* opts, mose_chem_report* rep) {
+  ChemObj* o = get_chem(id);
+  if (!ctx || !o || !opts || !(dt > 0.0)) return fail(ctx, MOSE_ERR_ARG, "mose_chem_advance: bad arguments");
+  if (opts->method != MOSE_ODE_RODAS) return fail(ctx, MOSE_ERR_ARG, "mose_chem_advance: only RODAS is built");
+  if (o->n < 1) return fail(ctx, MOSE_ERR_ARG, "mose_chem_advance: no cells uploaded");
+  const int nz = o->M.ns + 1;
+  for (int q = 0; q < nz; ++q)
+    if (!(opts->rtol[q] > 0.0) || !(opts->atol[q] > 0.0)) return fail(ctx, MOSE_ERR_ARG, "mose_chem_advance: tolerances must be > 0");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  ChemArgs A{};
+  A.state = o->state; A.hr = o->hr; A.idid = o->idid; A.nstep = o->nstep; A.fjac = o->fjac;
+  A.queue = o->ctr; A.stats = o->ctr + 1; A.ncell = o->n; A.nlanes = o->nlanes; A.dt = dt;
+  for (int q = 0; q < nz; ++q) { A.rtol[q] = opts->rtol[q]; A.atol[q] = opts->atol[q]; }
+  A.stride = o->cap;
+  HIPCHK(ctx, hipMemsetAsync(o->ctr, 0, 8 * sizeof(unsigned long long), s));
+  int grid = (int)std::min<int64_t>(o->grid, (o->n + 63) / 64);
+  HIPCHK(ctx, hipEventRecord(o->ev[0], s));
+  k_chem<9><<<grid, 64, 0, s>>>(o->M, A);
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipEventRecord(o->ev[1], s));
+  unsigned long long st[6];
+  HIPCHK(ctx, hipMemcpyAsync(st, o->ctr, sizeof st, hipMemcpyDeviceToHost, s));
+  HIPCHK(ctx, hipStreamSynchronize(s));
+  float ms = 0.f;
+  HIPCHK(ctx, hipEventElapsedTime(&ms, o->ev[0], o->ev[1]));
+  if (rep) {
+    rep->cells = o->n;
+    rep->steps_total = (int64_t)st[1];
+    rep->rejected_total = (int64_t)st[2];
+    rep->rhs_total = (int64_t)st[3];
+    rep->failed = (int64_t)st[4];
+    rep->steps_max = (int32_t)st[5];
+    rep->first_failed_idid = 0;
+    rep->first_failed_cell = -1;
+    rep->ms = ms;
+  }
+  if (st[4] > 0) return fail(ctx, MOSE_ERR_ODE, "mose_chem_advance: " + std::to_string(st[4]) + " cells failed (idid < 0)");
+  return MOSE_OK;
+}

@@ -2,10 +2,11 @@
 from .core import (BC_CONNECTION, BC_EXTRAPOLATION, BC_FARFIELD, BC_WALL_ADIABATIC, BC_WALL_ISOTHERMAL, GC,
                    FARFIELD_DTYPE, LIB_PATH, NTAB, PATH_FUSED, PATH_PLAIN, RECORD_DTYPE, Context, DeviceCase, Geometry, Layout, MoseError, Scheme,
                    ThermoTables, device_count, lib, metrics_build, records_tangent, state_hash)
-from . import cases
+from .core import DeviceChem
+from . import cases, chem
 
 __all__ = [
     "BC_CONNECTION", "BC_EXTRAPOLATION", "BC_FARFIELD", "FARFIELD_DTYPE", "BC_WALL_ADIABATIC", "BC_WALL_ISOTHERMAL", "GC", "LIB_PATH",
     "NTAB", "PATH_FUSED", "PATH_PLAIN", "RECORD_DTYPE", "Context", "DeviceCase", "Geometry", "Layout", "MoseError", "Scheme", "ThermoTables",
-    "device_count", "lib", "metrics_build", "records_tangent", "state_hash", "cases",
+    "DeviceChem", "chem", "device_count", "lib", "metrics_build", "records_tangent", "state_hash", "cases",
 ]

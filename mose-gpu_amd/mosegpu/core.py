@@ -84,6 +84,26 @@ class Report(C.Structure):
                 ("first_bad_k", C.c_int32), ("status", C.c_int32)]
 
 
+class MechanismStruct(C.Structure):
+    _fields_ = [("ns", C.c_int32), ("n_arrh", C.c_int32), ("n_troe", C.c_int32), ("n_lind", C.c_int32),
+                ("tmin", C.c_int32), ("tmax", C.c_int32),
+                ("wm", C.POINTER(C.c_double)), ("cp", C.POINTER(C.c_double)), ("h", C.POINTER(C.c_double)),
+                ("ni1", C.POINTER(C.c_double)), ("ni2", C.POINTER(C.c_double)), ("eps", C.POINTER(C.c_double)),
+                ("kf", C.POINTER(C.c_double)), ("kb", C.POINTER(C.c_double)), ("troe", C.POINTER(C.c_double)),
+                ("lind", C.POINTER(C.c_double))]
+
+
+class OdeOpts(C.Structure):
+    _fields_ = [("method", C.c_int32), ("reserved", C.c_int32 * 3), ("rtol", C.c_double * 17),
+                ("atol", C.c_double * 17)]
+
+
+class ChemReport(C.Structure):
+    _fields_ = [("cells", C.c_int64), ("failed", C.c_int64), ("steps_total", C.c_int64),
+                ("rejected_total", C.c_int64), ("rhs_total", C.c_int64), ("steps_max", C.c_int32),
+                ("first_failed_idid", C.c_int32), ("first_failed_cell", C.c_int64), ("ms", C.c_double)]
+
+
 def dptr(a: np.ndarray | None):
     if a is None:
         return None
@@ -131,11 +151,17 @@ def lib() -> C.CDLL:
     L.mose_case_tune.argtypes = [vp, ip, C.POINTER(Tuning), C.POINTER(Tuning)]
     L.mose_bandwidth_probe.argtypes = [vp, C.c_int64, ip, dp]
     L.mose_ctx_info.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.mose_chem_create.argtypes = [vp, C.POINTER(MechanismStruct), C.c_int64, C.POINTER(C.c_int)]
+    L.mose_chem_destroy.argtypes = [vp, ip]
+    L.mose_chem_upload.argtypes = [vp, ip, C.c_int64, dp]
+    L.mose_chem_download.argtypes = [vp, ip, dp, dp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+    L.mose_chem_advance.argtypes = [vp, ip, C.c_double, C.POINTER(OdeOpts), C.POINTER(ChemReport)]
     for name in ("mose_ctx_create", "mose_ctx_destroy", "mose_metrics_build", "mose_records_tangent", "mose_case_create",
                  "mose_case_destroy", "mose_case_bytes", "mose_state_upload", "mose_state_download",
                  "mose_case_step", "mose_cases_step", "mose_case_residual", "mose_case_dt",
                  "mose_case_residual_norm", "mose_case_state_hash", "mose_case_timing", "mose_device_count",
-                 "mose_case_tune", "mose_bandwidth_probe", "mose_ctx_info"):
+                 "mose_case_tune", "mose_bandwidth_probe", "mose_ctx_info", "mose_chem_create", "mose_chem_destroy",
+                 "mose_chem_upload", "mose_chem_download", "mose_chem_advance"):
         getattr(L, name).restype = C.c_int
     _lib_handle = L
     return L
@@ -351,4 +377,43 @@ class DeviceCase:
     def destroy(self):
         if self.id >= 0:
             lib().mose_case_destroy(self.ctx.h, self.id)
+            self.id = -1
+
+
+class DeviceChem:
+    """A batch of independent chemistry cells resident on the device (mose_chem_*)."""
+
+    def __init__(self, ctx: "Context", mech, ncell_max: int):
+        self.ctx = ctx
+        self.mech = mech
+        self._m = mech.as_struct()
+        cid = C.c_int(-1)
+        ctx.check(lib().mose_chem_create(ctx.h, C.byref(self._m), int(ncell_max), C.byref(cid)), "mose_chem_create")
+        self.id = cid.value
+        self.n = 0
+
+    def upload(self, state: np.ndarray):
+        st = np.ascontiguousarray(state, dtype=np.float64)
+        self.n = st.shape[1]
+        self.ctx.check(lib().mose_chem_upload(self.ctx.h, self.id, self.n, dptr(st)), "mose_chem_upload")
+
+    def advance(self, dt: float, opts) -> ChemReport:
+        rep = ChemReport()
+        self.ctx.check(lib().mose_chem_advance(self.ctx.h, self.id, dt, C.byref(opts), C.byref(rep)),
+                       "mose_chem_advance")
+        return rep
+
+    def download(self):
+        st = np.zeros((self.mech.ns + 1, self.n))
+        hr = np.zeros(self.n)
+        idid = np.zeros(self.n, dtype=np.int32)
+        nstep = np.zeros(self.n, dtype=np.int32)
+        self.ctx.check(lib().mose_chem_download(self.ctx.h, self.id, dptr(st), dptr(hr),
+                                                idid.ctypes.data_as(C.POINTER(C.c_int32)),
+                                                nstep.ctypes.data_as(C.POINTER(C.c_int32))), "mose_chem_download")
+        return st, hr, idid, nstep
+
+    def destroy(self):
+        if self.id >= 0:
+            lib().mose_chem_destroy(self.ctx.h, self.id)
             self.id = -1

@@ -40,6 +40,19 @@ def oracle_lib():
     L.oa_riemann.argtypes = [vp, ip, dp, dp, dp, dp]
     L.oa_sound_speed.restype = C.c_double
     L.oa_sound_speed.argtypes = [vp, dp, C.c_double]
+    L.oc_create.restype = vp
+    L.oc_create.argtypes = [vp]
+    L.oc_destroy.argtypes = [vp]
+    L.oc_rhs.argtypes = [vp, dp, dp]
+    L.oc_wdot_pub.argtypes = [vp, dp, C.c_double, dp]
+    L.oc_dec.restype = C.c_int
+    L.oc_dec.argtypes = [C.c_int, dp, C.POINTER(C.c_int)]
+    L.oc_sol.argtypes = [C.c_int, dp, dp, C.POINTER(C.c_int)]
+    L.oc_rodas_robertson.restype = C.c_int
+    L.oc_rodas_robertson.argtypes = [dp, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_int)]
+    L.oc_advance.restype = C.c_int
+    L.oc_advance.argtypes = [vp, C.c_int64, dp, C.c_double, dp, dp, dp, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                             C.c_int]
     _L = L
     return L
 
@@ -101,3 +114,39 @@ class OracleCase:
             oracle_lib().oa_destroy(self.h)
         except Exception:
             pass
+
+
+class OracleChem:
+    """The chemistry oracle (oracle/mose_chem_oracle.c) on a mechanism of mosegpu.chem."""
+
+    def __init__(self, mech, nthreads: int = 8):
+        self.mech = mech
+        self._m = mech.as_struct()
+        self.h = oracle_lib().oc_create(C.cast(C.pointer(self._m), C.c_void_p))
+        assert self.h, "oc_create failed"
+        self.nthreads = nthreads
+
+    def rhs(self, Z):
+        Z = np.ascontiguousarray(Z, dtype=np.float64)
+        F = np.zeros_like(Z)
+        oracle_lib().oc_rhs(self.h, _d(Z), _d(F))
+        return F
+
+    def wdot(self, roi, T):
+        roi = np.ascontiguousarray(roi, dtype=np.float64)
+        om = np.zeros_like(roi)
+        oracle_lib().oc_wdot_pub(self.h, _d(roi), T, _d(om))
+        return om
+
+    def advance(self, state, dt, rtol=1e-8, atol=1e-12):
+        st = np.ascontiguousarray(state, dtype=np.float64).copy()
+        n = st.shape[1]
+        nz = self.mech.ns + 1
+        rt = np.full(nz, rtol); at = np.full(nz, atol)
+        hr = np.zeros(n)
+        idid = np.zeros(n, dtype=np.int32)
+        nstep = np.zeros(n, dtype=np.int32)
+        oracle_lib().oc_advance(self.h, n, _d(st), dt, _d(rt), _d(at), _d(hr),
+                                idid.ctypes.data_as(C.POINTER(C.c_int32)), nstep.ctypes.data_as(C.POINTER(C.c_int32)),
+                                self.nthreads)
+        return st, hr, idid, nstep

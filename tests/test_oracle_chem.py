@@ -1,0 +1,180 @@
+"""Pins of the chemistry oracle (CPU).
+
+- The integrator (RODAS, METH = 1, numerical Jacobian) on Robertson's problem against the published t = 40 values,
+  and the invariant y1 + y2 + y3 = 1.
+- DEC / SOL against numpy.linalg.solve on random systems, including pivoting.
+- The law-of-mass-action source on one-reaction mechanisms:
+  - the closed-form rate of progress;
+  - detailed balance: the net rate vanishes where the concentration quotient equals Kc, and the third-body and
+    falloff forms are covered;
+  - the Troe rate against the falloff formula written out here.
+- Conservation: the mass sum and the H, O and N element sums of omega_dot are zero on the full mechanism.
+- rhs_native's constant-volume T equation against its closed form.
+- A reacting cell keeps its mass, and its internal energy from the tables, over the step.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from mosegpu import chem
+from oracle_bind import OracleChem, _d, oracle_lib
+
+MECH = chem.h2_air_mechanism()
+RU = chem.RU
+ROBERTSON_T40 = np.array([0.7158270687193605, 9.185534764557338e-06, 0.2841637457458750])
+ELEMENTS = {"H": [2, 0, 2, 1, 0, 1, 1, 2, 0], "O": [0, 2, 1, 0, 1, 1, 2, 2, 0], "N": [0, 0, 0, 0, 0, 0, 0, 0, 2]}
+
+
+def test_rodas_robertson():
+    L = oracle_lib()
+    y = np.array([1.0, 0.0, 0.0])
+    st = (C.c_int * 4)()
+    assert L.oc_rodas_robertson(_d(y), 40.0, 1e-10, 1e-14, st) == 1
+    assert np.allclose(y, ROBERTSON_T40, rtol=1e-7, atol=0)
+    assert abs(y.sum() - 1.0) < 1e-13
+    assert st[2] <= st[1] // 10            # few rejections
+
+
+def test_lu_against_numpy():
+    L = oracle_lib()
+    rng = np.random.default_rng(3)
+    for n in (1, 3, 10):
+        for _ in range(5):
+            A = rng.normal(size=(n, n))
+            A[0, 0] = 1e-8                  # forces a row exchange
+            b = rng.normal(size=n)
+            a = np.asfortranarray(A).ravel(order="F").copy()
+            ip = (C.c_int * n)()
+            assert L.oc_dec(n, _d(a), ip) == 0
+            x = b.copy()
+            L.oc_sol(n, _d(a), _d(x), ip)
+            assert np.allclose(x, np.linalg.solve(A, b), rtol=1e-10, atol=1e-12)
+    a = np.zeros(4)
+    ip = (C.c_int * 2)()
+    assert L.oc_dec(2, _d(a), ip) != 0      # singular
+
+
+def single(rindex):
+    """The mechanism reduced to one reaction (index into Arrhenius + Troe rows)."""
+    m = chem.h2_air_mechanism()
+    na = m.n_arrh
+    keep_arr = [rindex] if rindex < na else []
+    keep_troe = [rindex - na] if rindex >= na else []
+    rows = keep_arr + [na + t for t in keep_troe]
+    R = m.kf.shape[1]
+    return chem.Mechanism(m.ns, m.wm, m.cp, m.h, np.ascontiguousarray(m.ni1[rows]), np.ascontiguousarray(m.ni2[rows]),
+                          np.ascontiguousarray(m.eps[rows]),
+                          np.ascontiguousarray(m.kf[keep_arr]).reshape(len(keep_arr), R),
+                          np.ascontiguousarray(m.kb[keep_arr]).reshape(len(keep_arr), R),
+                          np.ascontiguousarray(m.troe[keep_troe]).reshape(len(keep_troe), 4, R),
+                          m.lind, m.names)
+
+
+def tab(t, T):
+    Ti = int(T)
+    return t[Ti] + (t[Ti + 1] - t[Ti]) * (T - Ti)
+
+
+def conc(rng):
+    return np.exp(rng.uniform(np.log(1e-6), np.log(1e-2), size=9))      # kmol/m^3
+
+
+@pytest.mark.parametrize("r", range(19))
+def test_rate_of_progress_and_detailed_balance(r):
+    m = single(r)
+    o = OracleChem(m)
+    rng = np.random.default_rng(100 + r)
+    T = 1234.5
+    c = conc(rng)
+    n1, n2, eps = m.ni1[0], m.ni2[0], m.eps[0]
+    coM = float(np.dot(eps[:9], c)) if n1[9] > 0 else 1.0
+    pf = np.prod(c ** n1[:9])
+    pr = np.prod(c ** n2[:9])
+    if m.n_arrh:
+        kf, kb = tab(m.kf[0], T), tab(m.kb[0], T)
+        net = (kf * pf - kb * pr) * coM
+        Kc = kf / kb
+    else:
+        kinf, k0, Kc, Fc = (tab(m.troe[0, q], T) for q in range(4))
+        Pr = k0 * coM / kinf
+        lF, lP = np.log10(Fc), np.log10(Pr)
+        cc, nn = -0.4 - 0.67 * lF, 0.75 - 1.27 * lF
+        F = 10 ** (lF / (1 + ((lP + cc) / (nn - 0.14 * (lP + cc))) ** 2))
+        kfw = kinf * Pr / (1 + Pr) * F
+        net = kfw * pf - kfw / Kc * pr
+    om = o.wdot(c * chem.WM, T)
+    exp = chem.WM * (n2[:9] - n1[:9]) * net
+    assert np.allclose(om, exp, rtol=1e-12, atol=1e-14 * np.max(np.abs(exp)))
+    # detailed balance: scale one product so that the quotient equals Kc -> the net rate vanishes
+    prod = int(np.flatnonzero(n2[:9])[0])
+    q = pr / pf
+    c2 = c.copy()
+    c2[prod] *= (Kc / q) ** (1.0 / n2[prod]) if n1[prod] == 0 else 1.0
+    if n1[prod] == 0:
+        om2 = o.wdot(c2 * chem.WM, T)
+        coM2 = float(np.dot(eps[:9], c2)) if n1[9] > 0 else 1.0
+        if m.n_arrh:
+            fwd = tab(m.kf[0], T) * pf * coM2
+        else:
+            Pr2 = k0 * coM2 / kinf
+            lP2 = np.log10(Pr2)
+            F2 = 10 ** (lF / (1 + ((lP2 + cc) / (nn - 0.14 * (lP2 + cc))) ** 2))
+            fwd = kinf * Pr2 / (1 + Pr2) * F2 * pf
+        scale = np.max(np.abs(chem.WM * (n2[:9] - n1[:9]))) * fwd
+        assert np.max(np.abs(om2)) <= 1e-12 * scale, (r, np.max(np.abs(om2)), scale)
+
+
+def test_mass_and_element_conservation():
+    o = OracleChem(MECH)
+    rng = np.random.default_rng(7)
+    for T in (950.3, 1500.0, 2345.6):
+        c = conc(rng)
+        om = o.wdot(c * chem.WM, T)
+        scale = np.max(np.abs(om))
+        assert abs(om.sum()) <= 1e-12 * scale
+        for el, a in ELEMENTS.items():
+            assert abs(np.dot(a, om / chem.WM)) <= 1e-12 * scale / np.min(chem.WM), el
+
+
+def test_rhs_temperature_equation():
+    o = OracleChem(MECH)
+    rng = np.random.default_rng(9)
+    rho = conc(rng) * chem.WM
+    T = 1734.25
+    Z = np.concatenate([rho, [T]])
+    F = o.rhs(Z)
+    om = o.wdot(rho, T)
+    Ri = RU / chem.WM
+    h = np.array([tab(MECH.h[s], T) for s in range(9)])
+    cp = np.array([tab(MECH.cp[s], T) for s in range(9)])
+    assert np.allclose(F[:9], om, rtol=0, atol=0)
+    FT = -np.sum((h - Ri * T) * om) / np.sum(rho * (cp - Ri))
+    assert np.isclose(F[9], FT, rtol=1e-12)
+    Z[9] = 20000.0
+    assert np.all(o.rhs(Z) == -1.0)      # off the table
+
+
+def test_cell_keeps_mass_and_energy():
+    o = OracleChem(MECH)
+    st = chem.h2_air_cells(24, seed=12)
+    out, hr, idid, nstep = o.advance(st, 1e-6)
+    assert np.all(idid == 1)
+    m0, m1 = st[:9].sum(0), out[:9].sum(0)
+    assert np.max(np.abs(m1 - m0) / m0) <= 1e-10          # rounding over thousands of steps
+    Ri = RU / chem.WM
+
+    def energy(s):
+        rho = s[:9].sum(0)
+        R = (Ri[:, None] * s[:9]).sum(0) / rho
+        T = s[9] / (rho * R)
+        e = np.zeros_like(T)
+        for q in range(9):
+            e += s[q] * (np.array([tab(MECH.h[q], t) for t in T]) - Ri[q] * T)
+        return e, T
+    e0, T0 = energy(st)
+    e1, T1 = energy(out)
+    # constant volume, adiabatic: sum rho_s e_s(T) is conserved up to the tables' piecewise-linear h vs cp
+    assert np.max(np.abs(e1 - e0) / np.abs(e0).max()) <= 1e-6
+    assert np.allclose(hr, np.abs(T1 - T0) / T0, rtol=1e-9)
+    assert np.max(hr) > 1e-3                     # some cells react
