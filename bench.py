@@ -57,6 +57,28 @@ def cpu_baseline(n: int, steps: int, threads: int):
                       f"(gcc -O2 -ffp-contract=off, OpenMP {threads} threads), {dt:.1f} s"}
 
 
+def serving_leg(count: int, plain: bool):
+    """A Poisson stream (20 cases/s) of 2D four-quadrant Riemann (n log-uniform in [64, 1024]) and 3D TGV (n
+    log-uniform in [48, 192]) cases of 50-500 steps, served with continuous batching; p50/p99 time-to-solution."""
+    from mosegpu import PATH_PLAIN, serve
+    reqs = serve.request_stream(count, rate=20.0, seed=2024)
+    st = serve.serve(reqs, quantum=10)
+    res = {"requests": count, "rate_per_s": 20.0, "cases_per_s": st.cases_per_s, "p50_s": st.p50, "p99_s": st.p99,
+           "wall_s": st.wall, "max_resident": st.max_resident, "cell_updates_per_s": st.cell_updates / st.wall,
+           "workload": "poisson 20/s: 2D 4-quadrant Riemann n^2 (n in [64,1024]) | 3D TGV n^3 (n in [48,192]), "
+                       "50-500 steps"}
+    print(f"bench: served {count} cases in {st.wall:.2f} s, p50 {st.p50:.3f} s, p99 {st.p99:.3f} s", file=sys.stderr,
+          flush=True)
+    if plain:
+        t = 0.0
+        for r in serve.request_stream(count, rate=20.0, seed=2024):
+            _, dt = serve.run_alone(r, PATH_PLAIN)
+            t += dt
+        res["plain_one_by_one_cases_per_s"] = count / t
+        res["speedup_vs_plain"] = (count / t) and st.cases_per_s / (count / t)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,6 +91,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=96)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--probe-bw", action="store_true", help="only run the stream-copy probe (1 GiB buffers)")
+    ap.add_argument("--serve", type=int, default=16, help="requests of the serving leg (0: skip)")
+    ap.add_argument("--serve-plain", action="store_true", help="also run the serving requests one by one on the "
+                    "plain per-block path (the comparison path)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (the multi-GPU sharding "
@@ -127,12 +152,14 @@ def main():
         "roofline": roofline,
         "sim_time": rep.time, "dt_last": rep.dt, "setup_s": t_setup, "device": ctx.info(),
     }
+    dc.destroy()
+    ctx.close()
+    if args.serve > 0:
+        out["serving"] = serving_leg(args.serve, args.serve_plain)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_steps, min(16, os.cpu_count() or 1))
     else:
         out["cpu_baseline"] = None
-    dc.destroy()
-    ctx.close()
     print(json.dumps(out))
 
 

@@ -65,6 +65,13 @@ struct Case {
   int64_t rpart_cap = 0, nparts_last = 0;
   double* rnorm = nullptr;
   unsigned long long* hslot = nullptr;
+  hipStream_t stream = nullptr;   // the case's own stream: several cases' steps overlap on the device
+  // pinned staging of the step report (dt history, bad-cell counters, first bad cell)
+  double* stage_hist = nullptr;
+  unsigned* stage_bad = nullptr;
+  unsigned long long* stage_fb = nullptr;
+  void* stage_host = nullptr;
+  int pending = 0;
   // timing
   bool timing = false;
   std::vector<hipEvent_t> tev;
@@ -169,6 +176,10 @@ extern "C" int mose_ctx_create(int device, const mose_tables* t, mose_ctx** out)
 
 static void case_free(Case* c) {
   if (!c) return;
+  if (c->stream) { (void)hipStreamSynchronize(c->stream); (void)hipStreamDestroy(c->stream); }
+  c->stream = nullptr;
+  if (c->stage_host) (void)hipHostFree(c->stage_host);
+  c->stage_host = nullptr;
   if (c->slab) (void)hipFree(c->slab);
   if (c->rpart) (void)hipFree(c->rpart);
   c->rpart = nullptr;
@@ -389,6 +400,10 @@ static int alloc_case(mose_ctx* ctx, Case& c) {
   c.hslot = (unsigned long long*)take(64);
   if ((size_t)(p - (char*)c.slab) > total) return fail(ctx, MOSE_ERR_NOMEM, "internal: slab overflow");
   for (auto& e : c.ev) HIPCHK(ctx, hipEventCreate(&e));
+  HIPCHK(ctx, hipHostMalloc(&c.stage_host, (size_t)c.hist_cap * d + 64, hipHostMallocDefault));
+  c.stage_hist = (double*)c.stage_host;
+  c.stage_bad = (unsigned*)((char*)c.stage_host + (size_t)c.hist_cap * d);
+  c.stage_fb = (unsigned long long*)((char*)c.stage_host + (size_t)c.hist_cap * d + 16);
   return MOSE_OK;
 }
 
@@ -449,7 +464,8 @@ extern "C" int mose_case_create(mose_ctx* ctx, const mose_scheme* sc, int nb, co
   rc = alloc_case(ctx, *c);
   if (rc != MOSE_OK) { case_free(c.get()); return rc; }
   size_t d = sizeof(double);
-  hipStream_t s = ctx->stream;
+  HIPCHK(ctx, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));   // default priority
+  hipStream_t s = c->stream;
   HIPCHK(ctx, hipMemcpyAsync(c->dl, dl, 3 * (size_t)c->NC * d, hipMemcpyHostToDevice, s));
   HIPCHK(ctx, hipMemcpyAsync(c->m, m, 9 * (size_t)c->NC * d, hipMemcpyHostToDevice, s));
   HIPCHK(ctx, hipMemcpyAsync(c->vol, vol, (size_t)c->NC * d, hipMemcpyHostToDevice, s));
@@ -479,7 +495,7 @@ extern "C" int mose_case_destroy(mose_ctx* ctx, int id) {
   Case* c = get_case(ctx, id);
   if (!c) return MOSE_ERR_ARG;
   (void)hipSetDevice(ctx->device);
-  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(c->stream));
   case_free(c);
   return MOSE_OK;
 }
@@ -489,8 +505,8 @@ extern "C" int mose_state_upload(mose_ctx* ctx, int id, const double* P) {
   if (!c || !P) return MOSE_ERR_ARG;
   HIPCHK(ctx, hipSetDevice(ctx->device));
   size_t n = (size_t)c->NC * c->nprim * sizeof(double);
-  HIPCHK(ctx, hipMemcpyAsync(c->P0, P, n, hipMemcpyHostToDevice, ctx->stream));
-  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(c->P0, P, n, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(ctx, hipStreamSynchronize(c->stream));
   c->dt_valid = false;
   return MOSE_OK;
 }
@@ -500,8 +516,8 @@ extern "C" int mose_state_download(mose_ctx* ctx, int id, double* P) {
   if (!c || !P) return MOSE_ERR_ARG;
   HIPCHK(ctx, hipSetDevice(ctx->device));
   size_t n = (size_t)c->NC * c->nprim * sizeof(double);
-  HIPCHK(ctx, hipMemcpyAsync(P, c->P0, n, hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(P, c->P0, n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(ctx, hipStreamSynchronize(c->stream));
   return MOSE_OK;
 }
 
@@ -596,8 +612,8 @@ static int set_tiles(mose_ctx* ctx, Case& c, int TI, int TJ, int TK) {
     HIPCHK(ctx, hipMalloc(&c.dtiles, t.size() * sizeof(Tile)));
     c.tiles_cap = (int)t.size();
   }
-  HIPCHK(ctx, hipMemcpyAsync(c.dtiles, t.data(), t.size() * sizeof(Tile), hipMemcpyHostToDevice, ctx->stream));
-  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(c.dtiles, t.data(), t.size() * sizeof(Tile), hipMemcpyHostToDevice, c.stream));
+  HIPCHK(ctx, hipStreamSynchronize(c.stream));
   c.ntiles = (int)t.size();
   c.TI = TI; c.TJ = TJ; c.TK = TK;
   return ensure_rpart(ctx, c, c.ntiles);
@@ -632,7 +648,7 @@ __global__ void k_step_begin(const double* dt, double* hist, int n) {
 
 template <int NS, int NR>
 static int run_steps(mose_ctx* ctx, Case& c, int nsteps) {
-  hipStream_t s = ctx->stream;
+  hipStream_t s = c.stream;
   static const double rk[3][3] = {{1.0, 0.0, 0.0}, {1.0, 0.5, 0.0}, {1.0, 0.25, 2.0 / 3.0}};   // Lib_RK.f90:6-9
   int nrk = c.sc.n_rk;
   bool fused = (c.path == MOSE_PATH_FUSED);
@@ -708,7 +724,7 @@ static int autotune(mose_ctx* ctx, Case& c, mose_tuning* chosen) {
   // the state P0 untouched) a few times, the fastest is kept
   static const int cand[][3] = {{32, 4, 2}, {64, 2, 2}, {16, 4, 4}, {16, 8, 2}, {8, 8, 4}, {64, 4, 1}, {32, 8, 1},
                                 {128, 2, 1}, {256, 1, 1}};
-  hipStream_t s = ctx->stream;
+  hipStream_t s = c.stream;
   hipEvent_t e0, e1;
   HIPCHK(ctx, hipEventCreate(&e0));
   HIPCHK(ctx, hipEventCreate(&e1));
@@ -765,25 +781,32 @@ extern "C" int mose_case_tune(mose_ctx* ctx, int id, const mose_tuning* t, mose_
   return MOSE_OK;
 }
 
-extern "C" int mose_case_step(mose_ctx* ctx, int id, int nsteps, mose_report* rep) {
-  Case* c = get_case(ctx, id);
-  if (!c || nsteps < 0) return MOSE_ERR_ARG;
-  HIPCHK(ctx, hipSetDevice(ctx->device));
+// Step n whole steps of a case on its own stream and queue the copy of the step report into the case's pinned
+// staging; nothing waits.  case_collect waits for the case's stream and fills the report.
+static int case_enqueue(mose_ctx* ctx, Case* c, int nsteps) {
   int rc;
   if (c->sc.nrans == 0) rc = run_steps<1, 0>(ctx, *c, nsteps);
   else rc = run_steps<1, 1>(ctx, *c, nsteps);
   if (rc != MOSE_OK) return rc;
-  std::vector<double> hist(std::min(nsteps, c->hist_cap));
-  unsigned bad[2] = {0, 0};
-  unsigned long long fb = ~0ull;
-  if (!hist.empty())
-    HIPCHK(ctx, hipMemcpyAsync(hist.data(), c->dthist, hist.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(ctx, hipMemcpyAsync(bad, c->bad, sizeof bad, hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(ctx, hipMemcpyAsync(&fb, c->first_bad, sizeof fb, hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-  for (double v : hist) c->time += v;
-  if (!hist.empty()) c->last_dt = hist.back();
+  size_t nh = (size_t)std::min(nsteps, c->hist_cap);
+  if (nh > 0)
+    HIPCHK(ctx, hipMemcpyAsync(c->stage_hist, c->dthist, nh * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(ctx, hipMemcpyAsync(c->stage_bad, c->bad, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(ctx, hipMemcpyAsync(c->stage_fb, c->first_bad, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+  c->pending = nsteps;
+  return MOSE_OK;
+}
+
+static int case_collect(mose_ctx* ctx, Case* c, mose_report* rep) {
+  int nsteps = c->pending;
+  c->pending = 0;
+  HIPCHK(ctx, hipStreamSynchronize(c->stream));
+  size_t nh = (size_t)std::min(nsteps, c->hist_cap);
+  for (size_t q = 0; q < nh; ++q) c->time += c->stage_hist[q];
+  if (nh > 0) c->last_dt = c->stage_hist[nh - 1];
   c->steps += nsteps;
+  unsigned bad[2] = {c->stage_bad[0], c->stage_bad[1]};
+  unsigned long long fb = *c->stage_fb;
   int status = MOSE_OK;
   if (bad[1] > 0) status = MOSE_ERR_THERMO;
   if (bad[0] > 0) status = MOSE_ERR_STATE;
@@ -808,11 +831,32 @@ extern "C" int mose_case_step(mose_ctx* ctx, int id, int nsteps, mose_report* re
   return status;
 }
 
+extern "C" int mose_case_step(mose_ctx* ctx, int id, int nsteps, mose_report* rep) {
+  Case* c = get_case(ctx, id);
+  if (!c || nsteps < 0) return MOSE_ERR_ARG;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  int rc = case_enqueue(ctx, c, nsteps);
+  if (rc != MOSE_OK) return rc;
+  return case_collect(ctx, c, rep);
+}
+
+// All cases are queued first, each on its own stream, then collected: the steps of different cases run
+// concurrently on the device (small cases fill the CUs that one case alone leaves idle).
 extern "C" int mose_cases_step(mose_ctx* ctx, const int* ids, int n, int nsteps, mose_report* reps) {
-  if (!ctx || !ids || n < 0) return MOSE_ERR_ARG;
+  if (!ctx || !ids || n < 0 || nsteps < 0) return MOSE_ERR_ARG;
+  for (int q = 0; q < n; ++q)
+    if (!get_case(ctx, ids[q])) return fail(ctx, MOSE_ERR_ARG, "mose_cases_step: bad case id");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
   int worst = MOSE_OK;
+  std::vector<char> queued(n, 0);
   for (int q = 0; q < n; ++q) {
-    int rc = mose_case_step(ctx, ids[q], nsteps, reps ? &reps[q] : nullptr);
+    int rc = case_enqueue(ctx, get_case(ctx, ids[q]), nsteps);
+    if (rc != MOSE_OK) { if (worst == MOSE_OK) worst = rc; continue; }
+    queued[q] = 1;
+  }
+  for (int q = 0; q < n; ++q) {
+    if (!queued[q]) continue;
+    int rc = case_collect(ctx, get_case(ctx, ids[q]), reps ? &reps[q] : nullptr);
     if (rc != MOSE_OK && worst == MOSE_OK) worst = rc;
   }
   return worst;
@@ -820,7 +864,7 @@ extern "C" int mose_cases_step(mose_ctx* ctx, const int* ids, int n, int nsteps,
 
 template <int NS, int NR>
 static int residual_impl(mose_ctx* ctx, Case& c, double* R) {
-  hipStream_t s = ctx->stream;
+  hipStream_t s = c.stream;
   CaseDev D = make_dev(ctx, c, c.P0, c.PA);
   launch_flux<NS, NR>(c, D, s);
   double* dR = c.PB;   // scratch
@@ -847,11 +891,11 @@ extern "C" int mose_case_dt(mose_ctx* ctx, int id, double* dt) {
   if (!c || !dt) return MOSE_ERR_ARG;
   HIPCHK(ctx, hipSetDevice(ctx->device));
   CaseDev D = make_dev(ctx, *c, c->P0, c->PA);
-  if (c->sc.nrans == 0) launch_dt<1, 0>(*c, D, ctx->stream, -1);
-  else launch_dt<1, 1>(*c, D, ctx->stream, -1);
+  if (c->sc.nrans == 0) launch_dt<1, 0>(*c, D, c->stream, -1);
+  else launch_dt<1, 1>(*c, D, c->stream, -1);
   HIPCHK(ctx, hipGetLastError());
-  HIPCHK(ctx, hipMemcpyAsync(dt, c->dt, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(dt, c->dt, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(ctx, hipStreamSynchronize(c->stream));
   return MOSE_OK;
 }
 
@@ -861,11 +905,11 @@ extern "C" int mose_case_residual_norm(mose_ctx* ctx, int id, double* out) {
   if (c->steps == 0 || c->nparts_last == 0) return fail(ctx, MOSE_ERR_ARG, "mose_case_residual_norm: no step taken yet");
   HIPCHK(ctx, hipSetDevice(ctx->device));
   int nres = c->nprim - c->sc.nsc + 1;
-  k_rnorm_final<<<1, 64, 0, ctx->stream>>>(c->rpart, c->nparts_last, nres, c->rnorm);
+  k_rnorm_final<<<1, 64, 0, c->stream>>>(c->rpart, c->nparts_last, nres, c->rnorm);
   HIPCHK(ctx, hipGetLastError());
   double h[64];
-  HIPCHK(ctx, hipMemcpyAsync(h, c->rnorm, nres * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(h, c->rnorm, nres * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(ctx, hipStreamSynchronize(c->stream));
   for (int q = 0; q < nres; ++q) out[q] = std::sqrt(h[q]);   // Mod_Explicit.f90:358
   return MOSE_OK;
 }
@@ -883,13 +927,13 @@ extern "C" int mose_case_state_hash(mose_ctx* ctx, int id, uint64_t* lo, uint64_
   Case* c = get_case(ctx, id);
   if (!c || !lo || !hi) return MOSE_ERR_ARG;
   HIPCHK(ctx, hipSetDevice(ctx->device));
-  HIPCHK(ctx, hipMemsetAsync(c->hslot, 0, 2 * sizeof(unsigned long long), ctx->stream));
-  if (c->sc.nrans == 0) launch_hash<1, 0>(*c, ctx->stream);
-  else launch_hash<1, 1>(*c, ctx->stream);
+  HIPCHK(ctx, hipMemsetAsync(c->hslot, 0, 2 * sizeof(unsigned long long), c->stream));
+  if (c->sc.nrans == 0) launch_hash<1, 0>(*c, c->stream);
+  else launch_hash<1, 1>(*c, c->stream);
   HIPCHK(ctx, hipGetLastError());
   unsigned long long h[2];
-  HIPCHK(ctx, hipMemcpyAsync(h, c->hslot, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(h, c->hslot, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(ctx, hipStreamSynchronize(c->stream));
   *lo = h[0];
   *hi = h[1];
   return MOSE_OK;

@@ -345,3 +345,33 @@ def box_case(dims, faces, extent=(1.0, 1.0, 1.0), nrans: int = 0, viscous: bool 
     sch = make_scheme(nrans=nrans, **kw)
     ff = farfield if farfield is not None else np.zeros(0, dtype=FARFIELD_DTYPE)
     return Case("box", tables, sch, layout, nodes, rec, geom, np.zeros((nprim, layout.NC)), nsteps, farfield=ff)
+
+
+def riemann2d(n: int = 256, nsteps: int = 100, seed: int = 0, cfl: float = 0.5, limiter="minmod",
+              riemann="HLLC") -> Case:
+    """2D Euler four-quadrant Riemann problem on [0, 1]^2 (one cell thick in z, extent 1/n): seeded quadrant states
+    rho, p ~ U(0.1, 2), velocity of magnitude <= 1 at a seeded angle, split at (0.5, 0.5); the constant-cp air
+    table, extrapolation (MOSE 400) on every face, SSP-RK3, two computational directions."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    tables = air_tables()
+    dims = (n, n, 1)
+    layout = Layout([dims])
+    nodes = [box_nodes(n, n, 1, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0 / n))]
+    rec = block_records(0, dims, {f: BC_EXTRAPOLATION for f in range(1, 7)})
+    geom = metrics_build(layout, nodes, rec, ndir=2)
+    sch = make_scheme(limiter=limiter, riemann=riemann, viscous=False, ndir=2, cfl=cfl)
+    P = np.zeros((5, layout.NC))
+    q = []
+    for _ in range(4):
+        rho, p = rng.uniform(0.1, 2.0, size=2)
+        mag, ang = rng.uniform(0.0, 1.0), rng.uniform(0.0, 2 * np.pi)
+        q.append((rho, mag * np.cos(ang), mag * np.sin(ang), p))
+    xc = (np.arange(n) + 0.5) / n
+    Y, X = np.meshgrid(xc, xc, indexing="ij")
+    quad = (X >= 0.5).astype(int) + 2 * (Y >= 0.5).astype(int)
+    V = layout.interior(P, 0)            # (5, 1, n, n)
+    for k in range(4):
+        msk = quad == k
+        V[0, 0][msk], V[1, 0][msk], V[2, 0][msk], V[4, 0][msk] = q[k][0], q[k][1], q[k][2], q[k][3]
+    fill_ghosts_nearest(layout, P)
+    return Case("riemann2d", tables, sch, layout, nodes, rec, geom, P, nsteps, meta={"states": q, "seed": seed})

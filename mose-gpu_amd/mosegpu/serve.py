@@ -1,0 +1,185 @@
+"""Continuous-batching serving loop over the C ABI.
+
+Requests arrive as a seeded Poisson stream. Each request is an independent single-block case:
+- a 2D four-quadrant Riemann problem at n^2, n log-uniform in [64, 1024]; or
+- a 3D viscous Taylor-Green vortex at n^3, n log-uniform in [48, 192].
+Each runs 50..500 steps.
+
+The server works in rounds:
+1. It admits every arrived request whose device footprint (mose_case_bytes) fits the HBM budget left, oldest first.
+2. It steps all resident cases together, a quantum of steps at a time (mose_cases_step: every case on its own
+   stream, so their kernels overlap on the device).
+3. It retires finished cases. A retired case gives its state hash, and its memory goes back to the budget.
+
+Requests that arrive while others run are admitted at the next round: that is continuous batching, not a queue
+in front of a single case.
+
+Time-to-solution = retirement time - arrival time (host preparation, queueing and device time included).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import cases
+from .core import PATH_FUSED, PATH_PLAIN, Context, DeviceCase, Report, lib
+
+
+@dataclass
+class Request:
+    rid: int
+    arrival: float          # seconds from the start of the stream
+    kind: str               # "riemann2d" | "tgv3d"
+    n: int
+    steps: int
+    seed: int
+    done: float = -1.0
+    hash: tuple = ()
+    admitted: float = -1.0
+
+
+def request_stream(count: int, rate: float = 20.0, seed: int = 2024, frac_2d: float = 0.5,
+                   n2d=(64, 1024), n3d=(48, 192), steps=(50, 500)) -> list[Request]:
+    rng = np.random.Generator(np.random.PCG64(seed))
+    t = 0.0
+    out = []
+    for i in range(count):
+        t += rng.exponential(1.0 / rate)
+        if rng.uniform() < frac_2d:
+            kind, lo, hi = "riemann2d", n2d[0], n2d[1]
+        else:
+            kind, lo, hi = "tgv3d", n3d[0], n3d[1]
+        n = int(round(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
+        out.append(Request(i, t, kind, n, int(rng.integers(steps[0], steps[1] + 1)), int(rng.integers(1 << 30))))
+    return out
+
+
+def build_case(r: Request):
+    if r.kind == "riemann2d":
+        return cases.riemann2d(r.n, r.steps, seed=r.seed)
+    return cases.taylor_green(r.n, r.steps, seed=r.seed)
+
+
+def case_bytes(case) -> int:
+    b = C.c_int64(0)
+    lib().mose_case_bytes(C.byref(case.scheme), case.layout.nb,
+                          case.layout.dims_arr.ctypes.data_as(C.POINTER(C.c_int)), len(case.records), C.byref(b))
+    return int(b.value)
+
+
+@dataclass
+class ServeStats:
+    requests: list
+    wall: float
+    cases_per_s: float
+    p50: float
+    p99: float
+    rounds: int
+    max_resident: int
+    cell_updates: float
+    meta: dict = field(default_factory=dict)
+
+
+def serve(reqs: list[Request], budget_bytes: int | None = None, quantum: int = 10, path: int = PATH_FUSED,
+          realtime: bool = True, log=None) -> ServeStats:
+    """Run the stream to completion.  realtime: requests become visible at their arrival time (wall clock);
+    otherwise all are visible at once (offline throughput)."""
+    ctx_cache = {}
+
+    def ctx_for(case):
+        key = case.name in ("riemann2d",) and "euler" or "ns"
+        if key not in ctx_cache:
+            ctx_cache[key] = Context(case.tables)
+        return ctx_cache[key]
+
+    pending = sorted(reqs, key=lambda r: r.arrival)
+    resident = []       # (req, case, dc, remaining, bytes, ctx)
+    t0 = time.perf_counter()
+    used = 0
+    if budget_bytes is None:
+        probe = Context(cases.air_tables())
+        budget_bytes = int(0.8 * probe.info()["hbm_free"])
+        probe.close()
+    rounds = 0
+    max_res = 0
+    cu = 0.0
+    while pending or resident:
+        now = time.perf_counter() - t0
+        # admission: arrived requests that fit, oldest first (a request that does not fit waits; later smaller
+        # ones may pass it only while it cannot fit at all with nothing resident)
+        while pending and (not realtime or pending[0].arrival <= now):
+            r = pending[0]
+            case = build_case(r)
+            nb = case_bytes(case)
+            if used + nb > budget_bytes and resident:
+                break
+            pending.pop(0)
+            ctx = ctx_for(case)
+            dc = DeviceCase(ctx, case)
+            dc.upload(case.P)
+            dc.tune(path, (0, 0, 0) if path == PATH_PLAIN else (0, 0, 0))
+            r.admitted = time.perf_counter() - t0
+            resident.append([r, case, dc, r.steps, nb, ctx])
+            used += nb
+        if not resident:
+            if pending and realtime:
+                time.sleep(max(0.0, pending[0].arrival - (time.perf_counter() - t0)))
+            continue
+        max_res = max(max_res, len(resident))
+        # one round: every resident case advances min(quantum, remaining) steps; cases of the same context are
+        # queued together (each on its own stream) and collected together
+        by_ctx = {}
+        for item in resident:
+            by_ctx.setdefault(id(item[5]), []).append(item)
+        for items in by_ctx.values():
+            ctx = items[0][5]
+            groups = {}
+            for it in items:
+                groups.setdefault(min(quantum, it[3]), []).append(it)
+            for k, its in groups.items():
+                ids = np.array([it[2].id for it in its], dtype=np.int32)
+                reps = (Report * len(its))()
+                ctx.check(lib().mose_cases_step(ctx.h, ids.ctypes.data_as(C.POINTER(C.c_int)), len(its), k, reps),
+                          "mose_cases_step")
+                for it in its:
+                    it[3] -= k
+                    cu += k * it[1].interior_cells()
+        rounds += 1
+        keep = []
+        for it in resident:
+            if it[3] <= 0:
+                r = it[0]
+                r.hash = it[2].state_hash()
+                r.done = time.perf_counter() - t0
+                it[2].destroy()
+                used -= it[4]
+                if log:
+                    log(f"served {r.kind} n={r.n} steps={r.steps}: {r.done - r.arrival:.3f} s")
+            else:
+                keep.append(it)
+        resident = keep
+    wall = time.perf_counter() - t0
+    for c in ctx_cache.values():
+        c.close()
+    lat = np.array([r.done - r.arrival for r in reqs])
+    return ServeStats(reqs, wall, len(reqs) / wall, float(np.percentile(lat, 50)), float(np.percentile(lat, 99)),
+                      rounds, max_res, cu)
+
+
+def run_alone(r: Request, path: int = PATH_FUSED):
+    """The same request alone on the device, start to finish: (state hash, seconds)."""
+    case = build_case(r)
+    ctx = Context(case.tables)
+    dc = DeviceCase(ctx, case)
+    dc.upload(case.P)
+    dc.tune(path, (0, 0, 0))
+    t = time.perf_counter()
+    dc.step(r.steps)
+    h = dc.state_hash()
+    dt = time.perf_counter() - t
+    dc.destroy()
+    ctx.close()
+    return h, dt

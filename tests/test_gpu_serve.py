@@ -1,0 +1,35 @@
+"""The serving loop: continuous batching of independent cases on one device (mosegpu.serve).
+
+Each served case's final state equals that case run alone, bitwise: the same field_state_hash. Admission respects
+the HBM budget, and requests that arrive during the run join the resident set.
+"""
+import numpy as np
+import pytest
+
+from mosegpu import PATH_FUSED, serve
+
+pytestmark = pytest.mark.gpu
+
+
+def small_stream(count=8, seed=5):
+    return serve.request_stream(count, rate=200.0, seed=seed, n2d=(48, 160), n3d=(16, 40), steps=(12, 40))
+
+
+def test_served_equals_alone(gpu_ready):
+    reqs = small_stream()
+    st = serve.serve(reqs, quantum=5, path=PATH_FUSED, realtime=True)
+    assert all(r.done > 0 for r in reqs)
+    assert st.max_resident >= 2          # requests overlapped on the device
+    for r in reqs:
+        h, _ = serve.run_alone(r)
+        assert r.hash == h, (r.kind, r.n, r.steps)
+    print(f"served {len(reqs)} cases: {st.cases_per_s:.2f} cases/s, p50 {st.p50:.3f} s, p99 {st.p99:.3f} s, "
+          f"max resident {st.max_resident}")
+
+
+def test_budget_limits_residency(gpu_ready):
+    reqs = small_stream(6, seed=9)
+    one = max(serve.case_bytes(serve.build_case(r)) for r in reqs)
+    st = serve.serve(reqs, budget_bytes=int(1.5 * one), quantum=5, realtime=False)
+    assert st.max_resident <= 2
+    assert all(r.done > 0 for r in reqs)
