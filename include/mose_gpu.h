@@ -200,7 +200,8 @@ typedef struct {
   int32_t path;
   int32_t ti, tj, tk;
   int32_t timing;
-  int32_t reserved[3];
+  int32_t no_graph;     /* 1: launch every fused step eagerly instead of replaying the captured step graph */
+  int32_t reserved[2];
 } mose_tuning;
 int mose_case_tune(mose_ctx* ctx, int case_id, const mose_tuning* tuning, mose_tuning* chosen);
 

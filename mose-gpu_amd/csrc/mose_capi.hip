@@ -72,6 +72,11 @@ struct Case {
   unsigned long long* stage_fb = nullptr;
   void* stage_host = nullptr;
   int pending = 0;
+  // one fused RK step captured as a hipGraph (kernels only), replayed per step
+  bool use_graph = true;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int* histpos = nullptr;     // device: next slot of the dt history
   // timing
   bool timing = false;
   std::vector<hipEvent_t> tev;
@@ -178,6 +183,10 @@ static void case_free(Case* c) {
   if (!c) return;
   if (c->stream) { (void)hipStreamSynchronize(c->stream); (void)hipStreamDestroy(c->stream); }
   c->stream = nullptr;
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->gexec = nullptr;
+  c->graph = nullptr;
   if (c->stage_host) (void)hipHostFree(c->stage_host);
   c->stage_host = nullptr;
   if (c->slab) (void)hipFree(c->slab);
@@ -397,6 +406,7 @@ static int alloc_case(mose_ctx* ctx, Case& c) {
   c.first_bad = (unsigned long long*)take(64);
   c.dtnext = (unsigned long long*)take(64);
   c.rnorm = (double*)take(64 * sizeof(double));
+  c.histpos = (int*)take(64);
   c.hslot = (unsigned long long*)take(64);
   if ((size_t)(p - (char*)c.slab) > total) return fail(ctx, MOSE_ERR_NOMEM, "internal: slab overflow");
   for (auto& e : c.ev) HIPCHK(ctx, hipEventCreate(&e));
@@ -422,6 +432,13 @@ static int ensure_rpart(mose_ctx* ctx, Case& c, int64_t nparts) {
   HIPCHK(ctx, hipMalloc(&c.rpart, (size_t)nparts * (c.nprim - c.sc.nsc + 1) * sizeof(double)));
   c.rpart_cap = nparts;
   return MOSE_OK;
+}
+
+static void drop_graph(Case& c) {
+  if (c.gexec) (void)hipGraphExecDestroy(c.gexec);
+  if (c.graph) (void)hipGraphDestroy(c.graph);
+  c.gexec = nullptr;
+  c.graph = nullptr;
 }
 
 static Case* get_case(mose_ctx* ctx, int id) {
@@ -616,6 +633,7 @@ static int set_tiles(mose_ctx* ctx, Case& c, int TI, int TJ, int TK) {
   HIPCHK(ctx, hipStreamSynchronize(c.stream));
   c.ntiles = (int)t.size();
   c.TI = TI; c.TJ = TJ; c.TK = TK;
+  drop_graph(c);
   return ensure_rpart(ctx, c, c.ntiles);
 }
 
@@ -642,9 +660,15 @@ static void launch_stage(Case& c, const CaseDev& D, hipStream_t s, double rkc, b
   }
 }
 
-__global__ void k_step_begin(const double* dt, double* hist, int n) {
-  if (threadIdx.x == 0 && blockIdx.x == 0 && hist) hist[n] = *dt;
+// records the step's dt in the history at the device-side position (the same kernel serves eager and graph steps)
+__global__ void k_step_begin(const double* dt, double* hist, int* pos, int cap) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    int t = *pos;
+    if (t < cap) hist[t] = *dt;
+    *pos = t + 1;
+  }
 }
+
 
 template <int NS, int NR>
 static int run_steps(mose_ctx* ctx, Case& c, int nsteps) {
@@ -665,11 +689,21 @@ static int run_steps(mose_ctx* ctx, Case& c, int nsteps) {
     HIPCHK(ctx, hipEventCreate(&e));
     c.tev.push_back(e);
   }
+  HIPCHK(ctx, hipMemsetAsync(c.histpos, 0, sizeof(int), s));
   size_t iev = 0;
   for (int n = 0; n < nsteps; ++n) {
     CaseDev D0 = make_dev(ctx, c, c.P0, c.PA);
     if (!fused || !c.dt_valid) launch_dt<NS, NR>(c, D0, s, -1);
-    k_step_begin<<<1, 64, 0, s>>>(c.dt, n < c.hist_cap ? c.dthist : nullptr, n < c.hist_cap ? n : 0);
+    // the fused step with dt carried over from the last stage is the same launch sequence every step: capture it
+    // once and replay it (no memset node in the graph; every node is a kernel)
+    const bool graph = fused && c.dt_valid && !c.timing && c.use_graph;
+    if (graph && c.gexec) {
+      HIPCHK(ctx, hipGraphLaunch(c.gexec, s));
+      c.nparts_last = c.ntiles;
+      continue;
+    }
+    if (graph) HIPCHK(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    k_step_begin<<<1, 64, 0, s>>>(c.dt, c.dthist, c.histpos, c.hist_cap);
     for (int irk = 1; irk <= nrk; ++irk) {
       double* in = irk == 1 ? c.P0 : (irk == 2 ? c.PA : c.PB);
       double* out = irk == nrk ? c.P0 : (irk == 1 ? c.PA : c.PB);
@@ -700,6 +734,11 @@ static int run_steps(mose_ctx* ctx, Case& c, int nsteps) {
         if (irk == nrk) c.nparts_last = part0;
         if (c.timing) HIPCHK(ctx, hipEventRecord(c.tev[iev++], s));
       }
+    }
+    if (graph) {
+      HIPCHK(ctx, hipStreamEndCapture(s, &c.graph));
+      HIPCHK(ctx, hipGraphInstantiate(&c.gexec, c.graph, nullptr, nullptr, 0));
+      HIPCHK(ctx, hipGraphLaunch(c.gexec, s));
     }
     c.dt_valid = fused;
   }
@@ -771,7 +810,9 @@ extern "C" int mose_case_tune(mose_ctx* ctx, int id, const mose_tuning* t, mose_
   HIPCHK(ctx, hipSetDevice(ctx->device));
   c->path = t->path;
   c->timing = t->timing != 0;
+  c->use_graph = t->no_graph == 0;
   c->dt_valid = false;
+  drop_graph(*c);
   if (chosen) *chosen = *t;
   if (c->path == MOSE_PATH_FUSED) {
     if (t->ti == 0 && t->tj == 0 && t->tk == 0)

@@ -54,8 +54,8 @@ module mose_gpu_c
   end type mose_report
 
   type, bind(C) :: mose_tuning
-    integer(c_int32_t) :: path, ti, tj, tk, timing
-    integer(c_int32_t) :: reserved(3)
+    integer(c_int32_t) :: path, ti, tj, tk, timing, no_graph
+    integer(c_int32_t) :: reserved(2)
   end type mose_tuning
 
   interface

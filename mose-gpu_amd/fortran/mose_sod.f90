@@ -99,7 +99,7 @@ program mose_sod
   end do
   call mose_check(ctx, mose_state_upload(ctx, cid, P), 'mose_state_upload')
   tun%path = path; tun%ti = merge(256, 0, path == MOSE_PATH_FUSED); tun%tj = merge(1, 0, path == MOSE_PATH_FUSED)
-  tun%tk = merge(1, 0, path == MOSE_PATH_FUSED); tun%timing = 0; tun%reserved = 0
+  tun%tk = merge(1, 0, path == MOSE_PATH_FUSED); tun%timing = 0; tun%no_graph = 0; tun%reserved = 0
   call mose_check(ctx, mose_case_tune(ctx, cid, tun, chosen), 'mose_case_tune')
 
   call MOSE_GPU_Explicit_Step(ctx, cid, nsteps, rep)

@@ -75,7 +75,7 @@ PATH_FUSED, PATH_PLAIN = 0, 1
 
 class Tuning(C.Structure):
     _fields_ = [("path", C.c_int32), ("ti", C.c_int32), ("tj", C.c_int32), ("tk", C.c_int32),
-                ("timing", C.c_int32), ("reserved", C.c_int32 * 3)]
+                ("timing", C.c_int32), ("no_graph", C.c_int32), ("reserved", C.c_int32 * 2)]
 
 
 class Report(C.Structure):
@@ -361,10 +361,11 @@ class DeviceCase:
         self.ctx.check(lib().mose_case_state_hash(self.ctx.h, self.id, C.byref(lo), C.byref(hi)), "mose_case_state_hash")
         return lo.value, hi.value
 
-    def tune(self, path: int = PATH_FUSED, tile=(0, 0, 0), timing: bool = False):
-        """Execution path and tile (0,0,0 = autotune); returns the tile chosen."""
+    def tune(self, path: int = PATH_FUSED, tile=(0, 0, 0), timing: bool = False, graph: bool = True):
+        """Execution path and tile (0,0,0 = autotune); graph=False launches every fused step eagerly.  Returns the
+        tile chosen."""
         t, out = Tuning(), Tuning()
-        t.path = path; t.ti, t.tj, t.tk = tile; t.timing = int(timing)
+        t.path = path; t.ti, t.tj, t.tk = tile; t.timing = int(timing); t.no_graph = int(not graph)
         self.ctx.check(lib().mose_case_tune(self.ctx.h, self.id, C.byref(t), C.byref(out)), "mose_case_tune")
         return (out.ti, out.tj, out.tk)
 
