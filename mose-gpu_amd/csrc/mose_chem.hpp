@@ -63,11 +63,43 @@ __device__ __forceinline__ double chem_ipow(double x, int n) {
   return first ? 1.0 : r;
 }
 
+
+/* Deterministic natural log and exp for the Troe broadening factor: the same statements on the CPU and the GPU give
+ * the same bits (the libm log10 / pow of the two platforms differ in the last ulp, and the adaptive integrator turns
+ * an ulp into a different step sequence).  log: x = m 2^e with m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s),
+ * s = (m-1)/(m+1), odd series to s^23; exp: y = k ln2 + r, |r| <= ln2/2, Taylor series to r^16, times 2^k.
+ * Both are within a few ulp of the true values for the arguments met here (Fcent in (0, 1], Pr > 0). */
+__device__ __forceinline__ double dm_log(double x) {
+  int e;
+  double m = frexp(x, &e);            /* x = m 2^e, m in [0.5, 1) */
+  if (m < 0.70710678118654752440) { m = m * 2.0; e = e - 1; }
+  double s = (m - 1.0) / (m + 1.0), s2 = s * s, t = 0.0;
+  for (int k = 23; k >= 3; k -= 2) t = (t + 1.0 / (double)k) * s2;
+  double lm = 2.0 * s * (1.0 + t);
+  return (double)e * 0.69314718055994528623 + ((double)e * 2.3190468138462996154e-17 + lm);
+}
+__device__ __forceinline__ double dm_exp(double y) {
+  double kf = floor(y * 1.44269504088896340736 + 0.5);
+  double r = (y - kf * 0.69314718055994528623) - kf * 2.3190468138462996154e-17;
+  double t = 1.0;
+  for (int k = 16; k >= 1; --k) t = 1.0 + t * r / (double)k;
+  return ldexp(t, (int)kf);
+}
+
+constexpr double DM_LN10 = 2.30258509299404568402;
+
+// f_F (Lib_Chemistry_falloff.f90:63-80); log10 and 10** through dm_log / dm_exp, the statements of the host side
 __device__ __forceinline__ double troe_F(double Pr, double Fcent) {
-  double lF = log10(Fcent), lP = log10(Pr);
+  double lF = dm_log(Fcent) / DM_LN10, lP = dm_log(Pr) / DM_LN10;
   double cc = -0.4 - 0.67 * lF, nn = 0.75 - 1.27 * lF;
   double f1 = (lP + cc) / (nn - 0.14 * (lP + cc));
-  return pow(10.0, lF / (1.0 + f1 * f1));
+  return dm_exp(lF / (1.0 + f1 * f1) * DM_LN10);
+}
+
+// c**nu of the falloff classes: integer power for integral nu, pow otherwise
+__device__ __forceinline__ double rpow(double c, double nu) {
+  double r = rint(nu);
+  return (r == nu && r >= 0.0 && r <= 8.0) ? chem_ipow(c, (int)r) : pow(c, nu);
 }
 
 template <int NS>
@@ -108,8 +140,8 @@ __device__ void wdot(const ChemDev& M, const double* roi, double temp, double* o
 #pragma unroll
       for (int s = 0; s < NS; ++s) {
         double a = n1[s], b = n2[s];
-        if (a != 0) pf = pf * pow(coi[s], a);
-        if (b != 0) pr = pr * pow(coi[s], b);
+        if (a != 0) pf = pf * rpow(coi[s], a);
+        if (b != 0) pr = pr * rpow(coi[s], b);
       }
       const double* tb = cls == 1 ? M.troe + (size_t)(r - M.na) * 4 * M.rows : M.lind + (size_t)(r - M.na - M.nt) * 3 * M.rows;
       double kinf = ctab(M, tb, Ti, Td), k0 = ctab(M, tb + M.rows, Ti, Td), kc = ctab(M, tb + 2 * M.rows, Ti, Td);
@@ -392,12 +424,12 @@ __global__ void __launch_bounds__(64) k_chem(ChemDev M, ChemArgs A) {
           err = err + q * q;
         }
         err = sqrt(err / NZ);
-        fac = fmax(fac2, fmin(fac1, pow(err, 0.25) / safe));
+        fac = fmax(fac2, fmin(fac1, sqrt(sqrt(err)) / safe));   // err**0.25, as the host side
         double hnew = h / fac;
         if (err <= 1.0) {
           nacc++;
           if (nacc > 1) {
-            double facgus = (hacc / h) * pow(err * err / erracc, 0.25) / safe;
+            double facgus = (hacc / h) * sqrt(sqrt(err * err / erracc)) / safe;
             facgus = fmax(fac2, fmin(fac1, facgus));
             fac = fmax(fac, facgus);
             hnew = h / fac;

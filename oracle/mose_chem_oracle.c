@@ -16,6 +16,10 @@
  *                  lib/FLINT/lib/OSlo/src/lib/hairerRodas.f:540-880; coefficients as rodas_dev,
  *                  Lib_Radau5_dev.f90:710-737
  *   oc_advance     Chemistry_Newstate_blk    src/lib/physics/Lib_Chemistry.f90:33-76
+ *
+ * Deliberate ulp-level deviations, shared with the device so that both take the same step sequence:
+ * x**0.25 of the step controller as sqrt(sqrt(x)); log10 / 10** of the Troe factor through dm_log / dm_exp; the
+ * falloff concentration powers as integer powers.  Parity with the reference's own libm is unpinned at the ulp level.
  */
 #include <math.h>
 #include <stdint.h>
@@ -83,12 +87,44 @@ static double ipow(double x, int n) {
   return first ? 1.0 : r;
 }
 
-/* f_F: the Troe broadening factor (Lib_Chemistry_falloff.f90:63-80) */
+
+/* Deterministic natural log and exp for the Troe broadening factor: the same statements on the CPU and the GPU give
+ * the same bits (the libm log10 / pow of the two platforms differ in the last ulp, and the adaptive integrator turns
+ * an ulp into a different step sequence).  log: x = m 2^e with m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s),
+ * s = (m-1)/(m+1), odd series to s^23; exp: y = k ln2 + r, |r| <= ln2/2, Taylor series to r^16, times 2^k.
+ * Both are within a few ulp of the true values for the arguments met here (Fcent in (0, 1], Pr > 0). */
+static inline double dm_log(double x) {
+  int e;
+  double m = frexp(x, &e);            /* x = m 2^e, m in [0.5, 1) */
+  if (m < 0.70710678118654752440) { m = m * 2.0; e = e - 1; }
+  double s = (m - 1.0) / (m + 1.0), s2 = s * s, t = 0.0;
+  for (int k = 23; k >= 3; k -= 2) t = (t + 1.0 / (double)k) * s2;
+  double lm = 2.0 * s * (1.0 + t);
+  return (double)e * 0.69314718055994528623 + ((double)e * 2.3190468138462996154e-17 + lm);
+}
+static inline double dm_exp(double y) {
+  double kf = floor(y * 1.44269504088896340736 + 0.5);
+  double r = (y - kf * 0.69314718055994528623) - kf * 2.3190468138462996154e-17;
+  double t = 1.0;
+  for (int k = 16; k >= 1; --k) t = 1.0 + t * r / (double)k;
+  return ldexp(t, (int)kf);
+}
+
+#define DM_LN10 2.30258509299404568402
+
+/* f_F: the Troe broadening factor (Lib_Chemistry_falloff.f90:63-80); log10 and 10** through dm_log / dm_exp */
 static double troe_F(double Pr, double Fcent) {
-  double lF = log10(Fcent), lP = log10(Pr);
+  double lF = dm_log(Fcent) / DM_LN10, lP = dm_log(Pr) / DM_LN10;
   double cc = -0.4 - 0.67 * lF, nn = 0.75 - 1.27 * lF;
   double f1 = (lP + cc) / (nn - 0.14 * (lP + cc));
-  return pow(10.0, lF / (1.0 + f1 * f1));
+  return dm_exp(lF / (1.0 + f1 * f1) * DM_LN10);
+}
+
+/* c**nu of the falloff classes: the integer power for integral nu (the run-time real power of the reference equals
+ * it up to an ulp), pow otherwise */
+static double rpow(double c, double nu) {
+  double r = nearbyint(nu);
+  return (r == nu && r >= 0.0 && r <= 8.0) ? ipow(c, (int)r) : pow(c, nu);
 }
 
 /* general(): omegadot(s) = sum_r Wm_s (ni2 - ni1) (rate_fwd - rate_rev) */
@@ -123,8 +159,8 @@ static void oc_wdot(const oc_mech *c, const double *roi, double temp, double *om
         rr = tlin(c, c->kb + (size_t)ir * c->rows, Ti, Td) * pr * coM;
       } else {
         for (int s = 0; s < ns; ++s) {
-          if (n1[s] != 0) pf = pf * pow(coi[s], n1[s]);
-          if (n2[s] != 0) pr = pr * pow(coi[s], n2[s]);
+          if (n1[s] != 0) pf = pf * rpow(coi[s], n1[s]);
+          if (n2[s] != 0) pr = pr * rpow(coi[s], n2[s]);
         }
         const double *tb = cls == 1 ? c->troe + (size_t)ir * 4 * c->rows : c->lind + (size_t)ir * 3 * c->rows;
         double kinf = tlin(c, tb, Ti, Td), k0 = tlin(c, tb + c->rows, Ti, Td), kc = tlin(c, tb + 2 * c->rows, Ti, Td);
@@ -337,12 +373,12 @@ L2:;
     err = err + q * q;
   }
   err = sqrt(err / n);
-  fac = fmax(fac2, fmin(fac1, pow(err, 0.25) / safe));
+  fac = fmax(fac2, fmin(fac1, sqrt(sqrt(err)) / safe));   /* err**0.25 */
   double hnew = h / fac;
   if (err <= 1.0) {
     st->naccpt++;
     if (st->naccpt > 1) {
-      double facgus = (hacc / h) * pow(err * err / erracc, 0.25) / safe;
+      double facgus = (hacc / h) * sqrt(sqrt(err * err / erracc)) / safe;
       facgus = fmax(fac2, fmin(fac1, facgus));
       fac = fmax(fac, facgus);
       hnew = h / fac;
