@@ -59,8 +59,6 @@ struct Case {
   Tile* dtiles = nullptr;
   int ntiles = 0, tiles_cap = 0;
   unsigned long long* dtnext = nullptr;
-  unsigned int* tq = nullptr;   // per-XCD tile queue heads of the persistent stage kernel
-  int pgrid = 0;                // persistent grid (workgroups), a multiple of 8
   bool dt_valid = false;
   // residual norm of the last step (per-workgroup partials of the last stage) and the state hash slots
   double* rpart = nullptr;
@@ -407,7 +405,6 @@ static int alloc_case(mose_ctx* ctx, Case& c) {
   c.bad = (unsigned int*)take(64);
   c.first_bad = (unsigned long long*)take(64);
   c.dtnext = (unsigned long long*)take(64);
-  c.tq = (unsigned int*)take(64);
   c.rnorm = (double*)take(64 * sizeof(double));
   c.histpos = (int*)take(64);
   c.hslot = (unsigned long long*)take(64);
@@ -637,28 +634,6 @@ static int set_tiles(mose_ctx* ctx, Case& c, int TI, int TJ, int TK) {
   c.ntiles = (int)t.size();
   c.TI = TI; c.TJ = TJ; c.TK = TK;
   drop_graph(c);
-  {
-    // the persistent grid: as many workgroups as fit on the device at once (LDS- and register-limited), x 8 XCDs
-    int cus = 0, per = 0;
-    HIPCHK(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
-    size_t lds = stage_lds(c, TI, TJ, TK);
-    hipError_t oe = hipErrorInvalidValue;
-    const int rs = c.sc.riemann;
-    if (c.sc.nrans == 0) {
-      if (c.sc.viscous) oe = rs == MOSE_RS_LMROE ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_stage<1, 0, true, MOSE_RS_LMROE>, 256, lds)
-                        : rs == MOSE_RS_AUSMP ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_stage<1, 0, true, MOSE_RS_AUSMP>, 256, lds)
-                                              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_stage<1, 0, true, MOSE_RS_HLLC>, 256, lds);
-      else oe = rs == MOSE_RS_LMROE ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_stage<1, 0, false, MOSE_RS_LMROE>, 256, lds)
-                : rs == MOSE_RS_AUSMP ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_stage<1, 0, false, MOSE_RS_AUSMP>, 256, lds)
-                                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_stage<1, 0, false, MOSE_RS_HLLC>, 256, lds);
-    } else {
-      oe = rs == MOSE_RS_LMROE ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_stage<1, 1, true, MOSE_RS_LMROE>, 256, lds)
-           : rs == MOSE_RS_AUSMP ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_stage<1, 1, true, MOSE_RS_AUSMP>, 256, lds)
-                                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_stage<1, 1, true, MOSE_RS_HLLC>, 256, lds);
-    }
-    if (oe != hipSuccess || per < 1) per = 1;
-    c.pgrid = ((cus * per + 7) / 8) * 8;
-  }
   return ensure_rpart(ctx, c, c.ntiles);
 }
 
@@ -670,10 +645,9 @@ static void launch_stage(Case& c, const CaseDev& D, hipStream_t s, double rkc, b
   A.nhs = hs_slots(c.TI, c.TJ, c.TK);
   A.ncache = (c.TI + 2) * (c.TJ + 2) * (c.TK + 2);
   A.r0 = stage_r0(c, c.TI, c.TJ, c.TK);
-  A.last = last ? 1 : 0; A.rkc = rkc; A.dtnext = c.dtnext; A.rpart = c.rpart; A.queue = c.tq;
+  A.last = last ? 1 : 0; A.rkc = rkc; A.dtnext = c.dtnext; A.rpart = c.rpart;
   size_t lds = stage_lds(c, c.TI, c.TJ, c.TK);
-  k_queue_reset<<<1, 64, 0, s>>>(c.tq, c.ntiles);
-  unsigned grid = (unsigned)std::max(8, std::min(((c.ntiles + 7) / 8) * 8, c.pgrid));
+  unsigned grid = (unsigned)(((c.ntiles + 7) / 8) * 8);
   const int rs = c.sc.riemann;
   if (c.sc.viscous) {
     if (rs == MOSE_RS_LMROE) k_stage<NS, NR, true, MOSE_RS_LMROE><<<grid, 256, lds, s>>>(D, A);

@@ -10,8 +10,8 @@
 //
 // Each face is computed once per tile (the faces on a tile edge twice: once by each tile), no face buffer goes to
 // HBM, and every sum keeps the reference's order, so the result is bitwise that of the per-face kernels.
-// The tiles of a case (of every block, of any size) are listed in device memory (block, i0, j0, k0) and handed out by
-// per-XCD atomic queues to a persistent grid (k_stage below).
+// The tiles of a case (of every block, of any size) are listed in device memory (block, i0, j0, k0) and served by
+// one launch per stage; the workgroup -> tile map deals a contiguous run of tiles to each XCD (k_stage below).
 #pragma once
 
 #include "mose_kernels.hpp"
@@ -34,7 +34,6 @@ struct StageArgs {
   double rkc;
   unsigned long long* dtnext;
   double* rpart;      // last stage: per-tile sums of the squared step changes [ntiles][nprim - NS + 1]
-  unsigned int* queue;   // [8] per-XCD queue heads (k_queue_reset before every launch)
 };
 
 // One interior viscous face of direction D (diff_cuf_i/j/k + diff_physics.inc) with the cell values (rho, 1/rho, T)
@@ -316,40 +315,24 @@ __device__ __forceinline__ void stage_tile(const CaseDev& C, const StageArgs& A,
   }
 }
 
-// The next tile for a workgroup of XCD x: from XCD x's own contiguous range first, then from the other XCDs'
-// ranges in turn (work stealing); -1 when every range is used up.  q[y] counts up from the first tile of range y.
-__device__ __forceinline__ int next_tile(unsigned int* q, int n, int x) {
-  for (int d = 0; d < 8; ++d) {
-    int y = (x + d) & 7;
-    int hi = (int)(((long long)(y + 1) * n) >> 3);
-    if ((int)q[y] >= hi) continue;
-    unsigned int t = atomicAdd(&q[y], 1u);
-    if ((int)t < hi) return (int)t;
-  }
-  return -1;
+// The workgroup -> tile map: workgroup w runs on XCD w % 8 (workgroups are dealt to the 8 XCDs round robin); XCD x
+// gets the contiguous run of tiles [x n / 8, (x + 1) n / 8), so neighbouring tiles share that XCD's L2.  A run-time
+// atomic tile queue here (dynamic assignment with stealing) raised the kernel's register allocation from 230 to
+// 263 VGPRs and halved its occupancy, so the map is static; the tiles of every block of the case, of any size,
+// are still served by this one launch.
+__device__ __forceinline__ int xcd_tile(int wg, int n) {
+  int x = wg & 7, q = wg >> 3;
+  int per = n >> 3, rem = n & 7;
+  int base = x * per + (x < rem ? x : rem);
+  int cnt = per + (x < rem ? 1 : 0);
+  return q < cnt ? base + q : -1;
 }
 
-// per-XCD queue heads <- the first tile of each XCD's range
-__global__ void k_queue_reset(unsigned int* q, int n) {
-  int x = threadIdx.x;
-  if (blockIdx.x == 0 && x < 8) q[x] = (unsigned int)(((long long)x * n) >> 3);
-}
-
-// The fused stage kernel, persistent: each workgroup takes tiles from the atomic queues until none is left.
-// Workgroups are dealt to the 8 XCDs round robin (workgroup w runs on XCD w % 8), and XCD x's queue holds a contiguous
-// run of tiles, so neighbouring tiles share that XCD's L2; a workgroup whose XCD has run out steals from the others.
 template <int NS, int NR, bool VISC, int RS>
 __global__ void __launch_bounds__(256) k_stage(CaseDev C, StageArgs A) {
-  __shared__ int s_tile;
-  const int x = blockIdx.x & 7;
-  while (true) {
-    if (threadIdx.x == 0) s_tile = next_tile(A.queue, A.ntiles, x);
-    __syncthreads();
-    const int t = s_tile;
-    __syncthreads();
-    if (t < 0) break;
-    stage_tile<NS, NR, VISC, RS>(C, A, t);
-  }
+  const int t = xcd_tile(blockIdx.x, A.ntiles);
+  if (t < 0) return;
+  stage_tile<NS, NR, VISC, RS>(C, A, t);
 }
 
 }  // namespace mose
