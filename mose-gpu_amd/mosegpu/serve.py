@@ -80,6 +80,8 @@ class ServeStats:
     rounds: int
     max_resident: int
     cell_updates: float
+    max_bytes: int = 0
+    budget: int = 0
     meta: dict = field(default_factory=dict)
 
 
@@ -105,6 +107,7 @@ def serve(reqs: list[Request], budget_bytes: int | None = None, quantum: int = 1
         probe.close()
     rounds = 0
     max_res = 0
+    max_used = 0
     cu = 0.0
     while pending or resident:
         now = time.perf_counter() - t0
@@ -129,6 +132,7 @@ def serve(reqs: list[Request], budget_bytes: int | None = None, quantum: int = 1
                 time.sleep(max(0.0, pending[0].arrival - (time.perf_counter() - t0)))
             continue
         max_res = max(max_res, len(resident))
+        max_used = max(max_used, used)
         # one round: every resident case advances min(quantum, remaining) steps; cases of the same context are
         # queued together (each on its own stream) and collected together
         by_ctx = {}
@@ -166,7 +170,7 @@ def serve(reqs: list[Request], budget_bytes: int | None = None, quantum: int = 1
         c.close()
     lat = np.array([r.done - r.arrival for r in reqs])
     return ServeStats(reqs, wall, len(reqs) / wall, float(np.percentile(lat, 50)), float(np.percentile(lat, 99)),
-                      rounds, max_res, cu)
+                      rounds, max_res, cu, max_used, budget_bytes)
 
 
 def run_alone(r: Request, path: int = PATH_FUSED):

@@ -29,7 +29,9 @@ def test_served_equals_alone(gpu_ready):
 
 def test_budget_limits_residency(gpu_ready):
     reqs = small_stream(6, seed=9)
-    one = max(serve.case_bytes(serve.build_case(r)) for r in reqs)
-    st = serve.serve(reqs, budget_bytes=int(1.5 * one), quantum=5, realtime=False)
-    assert st.max_resident <= 2
+    sizes = [serve.case_bytes(serve.build_case(r)) for r in reqs]
+    budget = int(1.5 * max(sizes))
+    st = serve.serve(reqs, budget_bytes=budget, quantum=5, realtime=False)
     assert all(r.done > 0 for r in reqs)
+    assert st.max_bytes <= budget                  # admission never exceeds the HBM budget
+    assert st.max_resident < len(reqs)             # and the budget did hold requests back
