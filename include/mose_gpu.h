@@ -216,7 +216,8 @@ int mose_bandwidth_probe(mose_ctx* ctx, int64_t bytes, int reps, double* gbs);
 /* ---------------------------------------------------------------------------------------------------------
  * Stiff chemistry (FLINT): per-cell constant-volume ODE integration over dt of independent cells
  * (Chemistry_Newstate_blk, src/lib/physics/Lib_Chemistry.f90:33-76) with RODAS (hairerRodas.f ROSCOR, METH = 1,
- * the host integrator wrap_rodas, OSlo.f90:485-527).
+ * the host integrator wrap_rodas, OSlo.f90:485-527) or RADAU5 (hairerRadau5.f RADCOR, wrap_radau5,
+ * OSlo.f90:349-412).
  * --------------------------------------------------------------------------------------------------------- */
 
 /* A reaction mechanism in FLINT's table form (read_chemistry, Load_Chemistry.f90:7-164; the general loop of
@@ -241,11 +242,15 @@ typedef struct {
   const double* lind;     /* [n_lind][3][rows]: k_inf, k_0, K_c */
 } mose_mechanism;
 
-enum { MOSE_ODE_RODAS = 1 };
+/* RODAS: hairerRodas.f ROSCOR (METH = 1) as wrap_rodas runs it (OSlo.f90:485-527).
+ * RADAU5: hairerRadau5.f RADCOR as wrap_radau5 runs it (OSlo.f90:349-412: IJAC = 0, ITOL = 1), in the statement
+ * order of the device radau5_dev (Lib_Radau5_dev.f90:199-652). */
+enum { MOSE_ODE_RODAS = 1, MOSE_ODE_RADAU5 = 2 };
 typedef struct {
-  int32_t method;         /* MOSE_ODE_RODAS */
+  int32_t method;         /* MOSE_ODE_RODAS or MOSE_ODE_RADAU5 */
   int32_t reserved[3];
-  double rtol[MOSE_MAX_SPECIES + 1];   /* per ODE component (species..., T): raw tolerances, as wrap_rodas */
+  double rtol[MOSE_MAX_SPECIES + 1];   /* per ODE component (species..., T): raw tolerances, as the wrappers take
+                                          them (RADAU5 applies its ITOL = 1 transform itself) */
   double atol[MOSE_MAX_SPECIES + 1];
 } mose_ode_opts;
 
@@ -264,9 +269,10 @@ int mose_chem_create(mose_ctx* ctx, const mose_mechanism* mech, int64_t ncell_ma
 int mose_chem_destroy(mose_ctx* ctx, int chem_id);
 int mose_chem_upload(mose_ctx* ctx, int chem_id, int64_t ncell, const double* state);
 int mose_chem_download(mose_ctx* ctx, int chem_id, double* state, double* hr, int32_t* idid, int32_t* nstep);
-/* Every cell: T = p / (rho R), RODAS over [0, dt] on (rho_1..rho_ns, T), species floored at 1e-20, p = T rho R
- * (Chemistry_Newstate_blk); hr = |T_new - T_old| / T_old.  One persistent launch: each lane takes the next cell
- * from an atomic work queue when its cell is done, so cells of uneven stiffness share the device evenly. */
+/* Every cell: T = p / (rho R), RODAS or RADAU5 over [0, dt] on (rho_1..rho_ns, T), species floored at 1e-20,
+ * p = T rho R (Chemistry_Newstate_blk); hr = |T_new - T_old| / T_old.  One persistent launch: each team of 16 lanes
+ * takes the next cell from an atomic work queue when its cell is done, so cells of uneven stiffness share the device
+ * evenly. */
 int mose_chem_advance(mose_ctx* ctx, int chem_id, double dt, const mose_ode_opts* opts, mose_chem_report* report);
 
 #ifdef __cplusplus

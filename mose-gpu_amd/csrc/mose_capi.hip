@@ -1032,6 +1032,7 @@ extern "C" int mose_bandwidth_probe(mose_ctx* ctx, int64_t bytes, int reps, doub
 #include "mose_chem.hpp"
 
 namespace {
+constexpr int kChemTeamsPerWg = 16;
 struct ChemObj {
   ChemDev M{};
   void* tabs = nullptr;          // device copies of the mechanism tables (one allocation)
@@ -1039,9 +1040,8 @@ struct ChemObj {
   double* hr = nullptr;
   int* idid = nullptr;
   int* nstep = nullptr;
-  double* fjac = nullptr;        // [NZ*NZ][nlanes]
   unsigned long long* ctr = nullptr;   // [0] queue, [1..5] stats
-  int64_t cap = 0, n = 0, nlanes = 0;
+  int64_t cap = 0, n = 0;
   int grid = 0;
   hipEvent_t ev[2] = {nullptr, nullptr};
   bool valid = false;
@@ -1058,10 +1058,9 @@ static void chem_free(ChemObj* o) {
   if (!o) return;
   if (o->tabs) (void)hipFree(o->tabs);
   if (o->state) (void)hipFree(o->state);
-  if (o->fjac) (void)hipFree(o->fjac);
   for (auto& e : o->ev)
     if (e) (void)hipEventDestroy(e);
-  o->tabs = nullptr; o->state = nullptr; o->fjac = nullptr;
+  o->tabs = nullptr; o->state = nullptr;
   o->valid = false;
 }
 
@@ -1072,6 +1071,8 @@ extern "C" int mose_chem_create(mose_ctx* ctx, const mose_mechanism* m, int64_t 
       !m->ni1 || !m->ni2 || !m->eps || (m->n_arrh > 0 && (!m->kf || !m->kb)) || (m->n_troe > 0 && !m->troe) ||
       (m->n_lind > 0 && !m->lind))
     return fail(ctx, MOSE_ERR_ARG, "mose_chem_create: incomplete mechanism");
+  if (m->n_arrh + m->n_troe + m->n_lind > NRMAX)
+    return fail(ctx, MOSE_ERR_ARG, "mose_chem_create: the chemistry kernel takes at most 32 reactions");
   HIPCHK(ctx, hipSetDevice(ctx->device));
   auto o = std::make_unique<ChemObj>();
   const int ns = m->ns, rows = m->tmax - m->tmin + 1, nr = m->n_arrh + m->n_troe + m->n_lind;
@@ -1096,10 +1097,9 @@ extern "C" int mose_chem_create(mose_ctx* ctx, const mose_mechanism* m, int64_t 
   HIPCHK(ctx, hipGetLastError());
   int cus = 0;
   HIPCHK(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
-  // persistent grid: enough 64-lane workgroups to fill every SIMD a few times over
-  int64_t want = (ncell_max + 63) / 64;
-  o->grid = (int)std::min<int64_t>(want, (int64_t)cus * 16);
-  o->nlanes = (int64_t)o->grid * 64;
+  // persistent grid of 256-lane workgroups (16 teams of 16 lanes, one cell per team at a time): enough to give
+  // every SIMD several waves
+  o->grid = (int)std::min<int64_t>((ncell_max + kChemTeamsPerWg - 1) / kChemTeamsPerWg, (int64_t)cus * 8);
   o->cap = ncell_max;
   const int nz = ns + 1;
   size_t sbytes = (size_t)nz * ncell_max * sizeof(double) + (size_t)ncell_max * (sizeof(double) + 2 * sizeof(int)) + 1024;
@@ -1108,7 +1108,6 @@ extern "C" int mose_chem_create(mose_ctx* ctx, const mose_mechanism* m, int64_t 
   o->idid = (int*)(o->hr + ncell_max);
   o->nstep = o->idid + ncell_max;
   o->ctr = (unsigned long long*)(((uintptr_t)(o->nstep + ncell_max) + 255) & ~(uintptr_t)255);
-  HIPCHK(ctx, hipMalloc(&o->fjac, (size_t)nz * nz * o->nlanes * sizeof(double)));
   for (auto& e : o->ev) HIPCHK(ctx, hipEventCreate(&e));
   o->valid = true;
   g_chem.push_back(std::move(o));
@@ -1164,14 +1163,14 @@ extern "C" int mose_chem_advance(mose_ctx* ctx, int id, double dt, const mose_od
   HIPCHK(ctx, hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
   ChemArgs A{};
-  A.state = o->state; A.hr = o->hr; A.idid = o->idid; A.nstep = o->nstep; A.fjac = o->fjac;
-  A.queue = o->ctr; A.stats = o->ctr + 1; A.ncell = o->n; A.nlanes = o->nlanes; A.dt = dt;
+  A.state = o->state; A.hr = o->hr; A.idid = o->idid; A.nstep = o->nstep;
+  A.queue = o->ctr; A.stats = o->ctr + 1; A.ncell = o->n; A.dt = dt;
   for (int q = 0; q < nz; ++q) { A.rtol[q] = opts->rtol[q]; A.atol[q] = opts->atol[q]; }
   A.stride = o->cap;
   HIPCHK(ctx, hipMemsetAsync(o->ctr, 0, 8 * sizeof(unsigned long long), s));
-  int grid = (int)std::min<int64_t>(o->grid, (o->n + 63) / 64);
+  int grid = (int)std::min<int64_t>(o->grid, (o->n + kChemTeamsPerWg - 1) / kChemTeamsPerWg);
   HIPCHK(ctx, hipEventRecord(o->ev[0], s));
-  k_chem<9><<<grid, 64, 0, s>>>(o->M, A);
+  k_chem<9><<<grid, kChemTeamsPerWg * TEAM, 0, s>>>(o->M, A);
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipEventRecord(o->ev[1], s));
   unsigned long long st[6];

@@ -50,6 +50,20 @@ def oracle_lib():
     L.oc_sol.argtypes = [C.c_int, dp, dp, C.POINTER(C.c_int)]
     L.oc_rodas_robertson.restype = C.c_int
     L.oc_rodas_robertson.argtypes = [dp, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_int)]
+    ip_ = C.POINTER(C.c_int)
+    L.oc_decc.restype = C.c_int
+    L.oc_decc.argtypes = [C.c_int, dp, dp, ip_]
+    L.oc_solc.argtypes = [C.c_int, dp, dp, dp, dp, ip_]
+    L.oc_radau5_robertson.restype = C.c_int
+    L.oc_radau5_robertson.argtypes = [dp, C.c_double, C.c_double, C.c_double, ip_]
+    L.oc_ode_linear.restype = C.c_int
+    L.oc_ode_linear.argtypes = [C.c_int, C.c_int, dp, dp, C.c_double, C.c_double, C.c_double, ip_]
+    L.oc_radau5_tols.argtypes = [C.c_int, dp, dp, dp, dp, dp]
+    L.oc_radau5_consts.argtypes = [dp]
+    L.oc_advance_m.restype = C.c_int
+    L.oc_advance_m.argtypes = [vp, C.c_int, C.c_int64, dp, C.c_double, dp, dp, dp, C.POINTER(C.c_int32),
+                               C.POINTER(C.c_int32), C.c_int]
+    L.oc_set_reference_pow.argtypes = [C.c_int]
     L.oc_advance.restype = C.c_int
     L.oc_advance.argtypes = [vp, C.c_int64, dp, C.c_double, dp, dp, dp, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                              C.c_int]
@@ -138,7 +152,8 @@ class OracleChem:
         oracle_lib().oc_wdot_pub(self.h, _d(roi), T, _d(om))
         return om
 
-    def advance(self, state, dt, rtol=1e-8, atol=1e-12):
+    def advance(self, state, dt, rtol=1e-8, atol=1e-12, method=1):
+        """method 1: RODAS, 2: RADAU5 (the MOSE_ODE_* ids)"""
         st = np.ascontiguousarray(state, dtype=np.float64).copy()
         n = st.shape[1]
         nz = self.mech.ns + 1
@@ -146,7 +161,29 @@ class OracleChem:
         hr = np.zeros(n)
         idid = np.zeros(n, dtype=np.int32)
         nstep = np.zeros(n, dtype=np.int32)
-        oracle_lib().oc_advance(self.h, n, _d(st), dt, _d(rt), _d(at), _d(hr),
-                                idid.ctypes.data_as(C.POINTER(C.c_int32)), nstep.ctypes.data_as(C.POINTER(C.c_int32)),
-                                self.nthreads)
+        oracle_lib().oc_advance_m(self.h, method, n, _d(st), dt, _d(rt), _d(at), _d(hr),
+                                  idid.ctypes.data_as(C.POINTER(C.c_int32)),
+                                  nstep.ctypes.data_as(C.POINTER(C.c_int32)), self.nthreads)
         return st, hr, idid, nstep
+
+
+REF_LIB = ORACLE_DIR / "_ref" / "libhairer_ref.so"
+_R = None
+
+
+def ref_lib():
+    """The reference's own Hairer RODAS / RADAU5, compiled from the reference sources by `make -C oracle ref` (absent
+    where the reference tree is not: the tests that need it skip)."""
+    global _R
+    if _R is not None:
+        return _R
+    if not REF_LIB.exists():
+        return None
+    R = C.CDLL(str(REF_LIB))
+    dp, ip = C.POINTER(C.c_double), C.POINTER(C.c_int)
+    R.ref_robertson.restype = C.c_int
+    R.ref_robertson.argtypes = [C.c_int, dp, C.c_double, C.c_double, C.c_double, ip]
+    R.ref_advance_cell.restype = C.c_int
+    R.ref_advance_cell.argtypes = [C.c_void_p, C.c_int, C.c_int, dp, dp, C.c_double, dp, dp, dp, ip]
+    _R = R
+    return R

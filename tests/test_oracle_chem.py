@@ -178,3 +178,112 @@ def test_cell_keeps_mass_and_energy():
     assert np.max(np.abs(e1 - e0) / np.abs(e0).max()) <= 1e-6
     assert np.allclose(hr, np.abs(T1 - T0) / T0, rtol=1e-9)
     assert np.max(hr) > 1e-3                     # some cells react
+
+
+# ---- RADAU5 -------------------------------------------------------------------------------------------------------
+
+def test_radau5_robertson():
+    L = oracle_lib()
+    y = np.array([1.0, 0.0, 0.0])
+    st = (C.c_int * 4)()
+    assert L.oc_radau5_robertson(_d(y), 40.0, 1e-10, 1e-14, st) == 1
+    assert np.allclose(y, ROBERTSON_T40, rtol=1e-7, atol=0)
+    assert abs(y.sum() - 1.0) < 1e-13
+    assert st[0] < 400                       # an order-5 method needs few steps here
+
+
+def test_radau5_constants():
+    """c1, c2 are the Radau IIA nodes (roots of the degree-3 Radau polynomial); 1/u1 and alph +- i beta are the
+    eigenvalues of the inverse of the Radau IIA(5) coefficient matrix."""
+    L = oracle_lib()
+    k = np.zeros(11)
+    L.oc_radau5_consts(_d(k))
+    c1, c2 = k[0], k[1]
+    assert np.isclose(c1, (4 - np.sqrt(6)) / 10, rtol=0, atol=1e-16) and np.isclose(c2, (4 + np.sqrt(6)) / 10)
+    s6 = np.sqrt(6.0)
+    A = np.array([[(88 - 7 * s6) / 360, (296 - 169 * s6) / 1800, (-2 + 3 * s6) / 225],
+                  [(296 + 169 * s6) / 1800, (88 + 7 * s6) / 360, (-2 - 3 * s6) / 225],
+                  [(16 - s6) / 36, (16 + s6) / 36, 1 / 9]])
+    ev = np.linalg.eigvals(np.linalg.inv(A))
+    real = ev[np.argmin(np.abs(ev.imag))].real
+    cplx = ev[np.argmax(ev.imag)]
+    assert np.isclose(k[8], real, rtol=1e-13)                      # u1 (after the inversion)
+    assert np.isclose(k[9], cplx.real, rtol=1e-13) and np.isclose(k[10], cplx.imag, rtol=1e-13)
+    assert np.isclose(k[5], -(13 + 7 * s6) / 3) and np.isclose(k[7], -1 / 3)
+
+
+def test_radau5_tolerance_transform():
+    L = oracle_lib()
+    rt = np.array([1e-8, 1e-6]); at = np.array([1e-12, 1e-10])
+    rtt = np.zeros(2); att = np.zeros(2); fn = np.zeros(1)
+    L.oc_radau5_tols(2, _d(rt), _d(at), _d(rtt), _d(att), _d(fn))
+    assert np.allclose(rtt, 0.1 * rt ** (2 / 3), rtol=1e-15)
+    assert np.allclose(att, rtt * at / rt, rtol=1e-15)
+    assert np.isclose(fn[0], max(10 * 1.1e-19 / rtt[0], min(0.03, np.sqrt(rtt[0]))), rtol=1e-15)
+
+
+def test_complex_lu_against_numpy():
+    L = oracle_lib()
+    rng = np.random.default_rng(4)
+    for n in (1, 2, 10):
+        for trial in range(5):
+            Ar = rng.normal(size=(n, n)); Ai = rng.normal(size=(n, n))
+            if trial == 1:
+                Ai[:] = 0.0                     # the ti = 0 branch of the elimination
+            if trial == 2:
+                Ar[:, 1:] = 0.0                 # the tr = 0 branch
+                Ar[:, 0] = rng.normal(size=n)
+            Ar[0, 0] = 1e-9; Ai[0, 0] = 0.0     # forces a row exchange
+            ar = np.asfortranarray(Ar).ravel(order="F").copy(); ai = np.asfortranarray(Ai).ravel(order="F").copy()
+            ip = (C.c_int * n)()
+            assert L.oc_decc(n, _d(ar), _d(ai), ip) == 0
+            b = rng.normal(size=n) + 1j * rng.normal(size=n)
+            br = b.real.copy(); bi = b.imag.copy()
+            L.oc_solc(n, _d(ar), _d(ai), _d(br), _d(bi), ip)
+            ref = np.linalg.solve(Ar + 1j * Ai, b)
+            assert np.allclose(br + 1j * bi, ref, rtol=1e-9, atol=1e-11)
+    z = np.zeros(4)
+    ip = (C.c_int * 2)()
+    assert L.oc_decc(2, _d(z), _d(z.copy()), ip) != 0     # singular
+
+
+@pytest.mark.parametrize("method", [1, 2])
+def test_integrators_on_a_linear_system(method):
+    """y' = A y with A = V diag(lam) V^-1, eigenvalues from -1 to -1e4 (stiff): y(1) = exp(A) y0 in closed form."""
+    from scipy.linalg import expm
+    L = oracle_lib()
+    rng = np.random.default_rng(7)
+    n = 6
+    V = rng.normal(size=(n, n)) + 3 * np.eye(n)
+    lam = -np.logspace(0, 4, n)
+    A = V @ np.diag(lam) @ np.linalg.inv(V)
+    y0 = rng.normal(size=n)
+    y = y0.copy()
+    a = np.asfortranarray(A).ravel(order="F").copy()
+    st = (C.c_int * 4)()
+    assert L.oc_ode_linear(method, n, _d(a), _d(y), 1.0, 1e-10, 1e-12, st) == 1
+    ref = expm(A) @ y0
+    assert np.allclose(y, ref, rtol=1e-6, atol=1e-9)
+
+
+def test_radau5_and_rodas_agree_on_cells():
+    """The two integrators on the same cells agree to the scale of the tolerance (the reference's cross-method gate
+    is 5e-2, single_cell_rodas.f90:201; this one is far tighter).  At rtol 1e-8 RADAU5's simplified Newton iteration
+    diverges on a few cold cells whose radicals start at zero (the clipped rates are not differentiable there) and
+    the integration stops at NMAX steps; the reference's own RADAU5 stops on the same cells (the comparison with
+    the reference build checks that), so those cells are left out here."""
+    o = OracleChem(MECH)
+    st = chem.h2_air_cells(48, seed=31)
+    a, hra, ida, _ = o.advance(st, 1e-6, method=1)
+    b, hrb, idb, nsb = o.advance(st, 1e-6, method=2)
+    assert np.all(ida == 1)
+    ok = idb == 1
+    assert ok.mean() >= 0.85 and np.all(idb[~ok] == -2)
+    a, b, st = a[:, ok], b[:, ok], st[:, ok]
+    T = lambda s: s[9] / (s[:9].sum(0) * ((RU / chem.WM)[:, None] * s[:9]).sum(0) / s[:9].sum(0))
+    assert np.max(np.abs(T(a) - T(b)) / T(a)) < 5e-5
+    major = st[:9].max(1) > 1e-3 * st[:9].sum(0).max()
+    rel = np.abs(a[:9] - b[:9]) / np.maximum(np.abs(a[:9]), 1e-6 * a[:9].sum(0))
+    assert np.max(rel[major]) < 1e-3
+    m0, m1 = st[:9].sum(0), b[:9].sum(0)
+    assert np.max(np.abs(m1 - m0) / m0) <= 1e-10
