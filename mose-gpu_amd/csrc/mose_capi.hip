@@ -1030,6 +1030,7 @@ extern "C" int mose_bandwidth_probe(mose_ctx* ctx, int64_t bytes, int reps, doub
 // stiff chemistry: batches of independent cells (Chemistry_Newstate_blk with the host RODAS of the general path)
 // ----------------------------------------------------------------------------------------------------------------
 #include "mose_chem.hpp"
+#include "mose_radau.hpp"
 
 namespace {
 constexpr int kChemTeamsPerWg = 16;
@@ -1048,6 +1049,29 @@ struct ChemObj {
 };
 std::vector<std::unique_ptr<ChemObj>> g_chem;   // per process; ids index this list (contexts share it)
 }  // namespace
+
+// RADAU5's ITOL = 1 tolerance transform and FNEWT (hairerRadau5.f:430-505; set_radau5_dev_tols,
+// Lib_Radau5_dev.f90:71-107) and RADCOR's method constants (hairerRadau5.f:713-728), once per call on the host:
+// k = [c1, c2, c1 - 1, c2 - 1, c1 - c2, dd1, dd2, dd3, 1/u1, alph/cno, beta/cno, FNEWT]
+static void radau5_host_setup(int n, const double* rt, const double* at, double* rtolt, double* atolt, double* k) {
+  const double expm = 2.0 / 3.0, uround = 1.1e-19;
+  for (int i = 0; i < n; ++i) {
+    const double quot = at[i] / rt[i];
+    rtolt[i] = 0.1 * std::pow(rt[i], expm);
+    atolt[i] = rtolt[i] * quot;
+  }
+  const double tolst = rtolt[0];
+  const double sq6 = std::sqrt(6.0);
+  const double c1 = (4.0 - sq6) / 10.0, c2 = (4.0 + sq6) / 10.0;
+  const double u1 = (6.0 + std::pow(81.0, 1.0 / 3.0) - std::pow(9.0, 1.0 / 3.0)) / 30.0;
+  const double alph = (12.0 - std::pow(81.0, 1.0 / 3.0) + std::pow(9.0, 1.0 / 3.0)) / 60.0;
+  const double beta = (std::pow(81.0, 1.0 / 3.0) + std::pow(9.0, 1.0 / 3.0)) * std::sqrt(3.0) / 60.0;
+  const double cno = alph * alph + beta * beta;
+  k[0] = c1; k[1] = c2; k[2] = c1 - 1.0; k[3] = c2 - 1.0; k[4] = c1 - c2;
+  k[5] = -(13.0 + 7.0 * sq6) / 3.0; k[6] = (-13.0 + 7.0 * sq6) / 3.0; k[7] = -1.0 / 3.0;
+  k[8] = 1.0 / u1; k[9] = alph / cno; k[10] = beta / cno;
+  k[11] = std::fmax(10 * uround / tolst, std::fmin(0.03, std::pow(tolst, 0.5)));
+}
 
 static ChemObj* get_chem(int id) {
   if (id < 0 || id >= (int)g_chem.size() || !g_chem[id] || !g_chem[id]->valid) return nullptr;
@@ -1155,7 +1179,8 @@ extern "C" int mose_chem_download(mose_ctx* ctx, int id, double* state, double* 
 extern "C" int mose_chem_advance(mose_ctx* ctx, int id, double dt, const mose_ode_opts* opts, mose_chem_report* rep) {
   ChemObj* o = get_chem(id);
   if (!ctx || !o || !opts || !(dt > 0.0)) return fail(ctx, MOSE_ERR_ARG, "mose_chem_advance: bad arguments");
-  if (opts->method != MOSE_ODE_RODAS) return fail(ctx, MOSE_ERR_ARG, "mose_chem_advance: only RODAS is built");
+  if (opts->method != MOSE_ODE_RODAS && opts->method != MOSE_ODE_RADAU5)
+    return fail(ctx, MOSE_ERR_ARG, "mose_chem_advance: method must be MOSE_ODE_RODAS or MOSE_ODE_RADAU5");
   if (o->n < 1) return fail(ctx, MOSE_ERR_ARG, "mose_chem_advance: no cells uploaded");
   const int nz = o->M.ns + 1;
   for (int q = 0; q < nz; ++q)
@@ -1166,11 +1191,13 @@ extern "C" int mose_chem_advance(mose_ctx* ctx, int id, double dt, const mose_od
   A.state = o->state; A.hr = o->hr; A.idid = o->idid; A.nstep = o->nstep;
   A.queue = o->ctr; A.stats = o->ctr + 1; A.ncell = o->n; A.dt = dt;
   for (int q = 0; q < nz; ++q) { A.rtol[q] = opts->rtol[q]; A.atol[q] = opts->atol[q]; }
+  if (opts->method == MOSE_ODE_RADAU5) radau5_host_setup(nz, opts->rtol, opts->atol, A.rtol, A.atol, A.rk);
   A.stride = o->cap;
   HIPCHK(ctx, hipMemsetAsync(o->ctr, 0, 8 * sizeof(unsigned long long), s));
   int grid = (int)std::min<int64_t>(o->grid, (o->n + kChemTeamsPerWg - 1) / kChemTeamsPerWg);
   HIPCHK(ctx, hipEventRecord(o->ev[0], s));
-  k_chem<9><<<grid, kChemTeamsPerWg * TEAM, 0, s>>>(o->M, A);
+  if (opts->method == MOSE_ODE_RADAU5) k_radau<9><<<grid, kChemTeamsPerWg * TEAM, 0, s>>>(o->M, A);
+  else k_chem<9><<<grid, kChemTeamsPerWg * TEAM, 0, s>>>(o->M, A);
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipEventRecord(o->ev[1], s));
   unsigned long long st[6];

@@ -35,7 +35,8 @@ struct ChemArgs {
   long long ncell;
   long long stride;          // row stride of state (>= ncell)
   double dt;
-  double rtol[MOSE_MAX_SPECIES + 1], atol[MOSE_MAX_SPECIES + 1];
+  double rtol[MOSE_MAX_SPECIES + 1], atol[MOSE_MAX_SPECIES + 1];   // RADAU5: the transformed tolerances
+  double rk[12];             // RADAU5: c1, c2, c1 - 1, c2 - 1, c1 - c2, dd1, dd2, dd3, 1/u1, alph, beta, FNEWT
 };
 
 __device__ __forceinline__ double ctab(const ChemDev& M, const double* t, int T1, double Td) {
@@ -366,6 +367,68 @@ struct RodasC {
 // control with its accept / reject, the start of the next step, or the end of the cell.
 enum : int { EV_NONE = -1, EV_F0 = 0, EV_J = 1 };
 
+// The workgroup's LDS copies of the mechanism: the compact reactions and cw (see team_rhs)
+template <int NS>
+__device__ __forceinline__ void chem_setup(const ChemDev& M, double* cw, RxnLds& X) {
+  rxn_build<NS>(M, X);
+  const int nr = M.na + M.nt + M.nl;
+  for (int i = threadIdx.x; i < NRMAX * TEAM; i += blockDim.x) {
+    const int q = i / TEAM, s = i % TEAM;
+    double v = 0.0;
+    if (q < nr && s < NS) {
+      double dn = M.ni2[(size_t)q * (NS + 1) + s] - M.ni1[(size_t)q * (NS + 1) + s];
+      v = M.wm[s] * dn;
+    }
+    cw[i] = v;
+  }
+  __syncthreads();
+}
+
+// Chemistry_Newstate_blk, entry (src/lib/physics/Lib_Chemistry.f90:55-60): Z = (rho_i, T = p / (rho R)); returns
+// this lane's component, rho and T of the cell in rho0 / Tb
+template <int NS>
+__device__ __forceinline__ double cell_load(const ChemDev& M, const ChemArgs& A, int cell, int base, int r, int rc,
+                                            double& rho0, double& Tb) {
+  const double q = A.state[(long long)rc * A.stride + cell];
+  const double p = A.state[(long long)NS * A.stride + cell];
+  double rho = 0.0, Rm = 0.0;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) rho = rho + tsh(q, base + s);
+#pragma unroll
+  for (int s = 0; s < NS; ++s) Rm = Rm + M.Ri[s] * tsh(q, base + s) / rho;
+  const double T = p / (rho * Rm);
+  rho0 = rho;
+  Tb = T;
+  return r < NS ? q : T;
+}
+
+// Chemistry_Newstate_blk, exit (:62-70): floors, p = T rho R, hr; and the batch statistics
+template <int NS>
+__device__ __forceinline__ void cell_store(const ChemDev& M, const ChemArgs& A, int cell, int base, int r, double y,
+                                           double rho0, double Tb, int idid, int nst, int nrej, int nfcn) {
+  const double yf = (r < NS && y <= 1e-20) ? 1e-20 : y;
+  double rs = 0.0, R2 = 0.0;
+  const double T = tsh(y, base + NS);
+#pragma unroll
+  for (int s = 0; s < NS; ++s) rs = rs + tsh(yf, base + s);
+#pragma unroll
+  for (int s = 0; s < NS; ++s) R2 = R2 + M.Ri[s] * tsh(yf, base + s) / rs;
+  if (r < NS) A.state[(long long)r * A.stride + cell] = yf;
+  if (r == NS) {
+    A.state[(long long)NS * A.stride + cell] = T * rho0 * R2;
+    A.hr[cell] = fabs(T - Tb) / Tb;
+    A.idid[cell] = idid;
+    A.nstep[cell] = nst;
+  }
+  if (r == 0) {
+    atomicAdd(&A.stats[0], (unsigned long long)nst);
+    atomicAdd(&A.stats[1], (unsigned long long)nrej);
+    atomicAdd(&A.stats[2], (unsigned long long)nfcn);
+    if (idid < 0) atomicAdd(&A.stats[3], 1ull);
+    atomicMax(&A.stats[4], (unsigned long long)nst);
+  }
+}
+
 // Four waves per SIMD: the spills this forces cost less than the latency that fewer waves leave exposed (measured
 // on 262144 cells of the hydrogen-air mechanism: 1.53e4 cells/s at two waves, 1.76e4 at four).
 template <int NS>
@@ -380,20 +443,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
   __shared__ double cw[NRMAX * TEAM];
   __shared__ TeamLds tl[256 / TEAM];
   __shared__ RxnLds X;
-  rxn_build<NS>(M, X);
-  {
-    const int nr = M.na + M.nt + M.nl;
-    for (int i = threadIdx.x; i < NRMAX * TEAM; i += blockDim.x) {
-      const int q = i / TEAM, s = i % TEAM;
-      double v = 0.0;
-      if (q < nr && s < NS) {
-        double dn = M.ni2[(size_t)q * (NS + 1) + s] - M.ni1[(size_t)q * (NS + 1) + s];
-        v = M.wm[s] * dn;
-      }
-      cw[i] = v;
-    }
-  }
-  __syncthreads();
+  chem_setup<NS>(M, cw, X);
 
   const int wl = threadIdx.x & 63;
   const int base = wl & ~(TEAM - 1);     // lane (in the wave) of the team's row 0
@@ -439,18 +489,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
       c = tshi(c, base);
       if ((long long)c >= A.ncell) break;
       cell = c;
-      // Chemistry_Newstate_blk: Z = (rho_i, T = p / (rho R))
-      double q = A.state[(long long)rc * A.stride + cell];
-      double p = A.state[(long long)NS * A.stride + cell];
-      double rho = 0.0, Rm = 0.0;
-#pragma unroll
-      for (int s = 0; s < NS; ++s) rho = rho + tsh(q, base + s);
-#pragma unroll
-      for (int s = 0; s < NS; ++s) Rm = Rm + M.Ri[s] * tsh(q, base + s) / rho;
-      double T = p / (rho * Rm);
-      y = r < NS ? q : T;
-      rho0 = rho;
-      Tb = T;
+      y = cell_load<NS>(M, A, cell, base, r, rc, rho0, Tb);
       // ROSCOR initial preparations (x = 0, xend = dt, H = 0 -> 1e-6)
       x = 0.0;
       double hmax = A.dt - x;
@@ -567,27 +606,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     }
 
     if (fin) {
-      // floors, p = T rho R, hr (Chemistry_Newstate_blk)
-      double yf = (r < NS && y <= 1e-20) ? 1e-20 : y;
-      double rs = 0.0, R2 = 0.0, T = tsh(y, base + NS);
-#pragma unroll
-      for (int s = 0; s < NS; ++s) rs = rs + tsh(yf, base + s);
-#pragma unroll
-      for (int s = 0; s < NS; ++s) R2 = R2 + M.Ri[s] * tsh(yf, base + s) / rs;
-      if (r < NS) A.state[(long long)r * A.stride + cell] = yf;
-      if (r == NS) {
-        A.state[(long long)NS * A.stride + cell] = T * rho0 * R2;
-        A.hr[cell] = fabs(T - Tb) / Tb;
-        A.idid[cell] = idid;
-        A.nstep[cell] = nst;
-      }
-      if (r == 0) {
-        atomicAdd(&A.stats[0], (unsigned long long)nst);
-        atomicAdd(&A.stats[1], (unsigned long long)nrej);
-        atomicAdd(&A.stats[2], (unsigned long long)nfcn);
-        if (idid < 0) atomicAdd(&A.stats[3], 1ull);
-        atomicMax(&A.stats[4], (unsigned long long)nst);
-      }
+      cell_store<NS>(M, A, cell, base, r, y, rho0, Tb, idid, nst, nrej, nfcn);
       ev = EV_NONE;
     }
   }

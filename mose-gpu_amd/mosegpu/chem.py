@@ -212,10 +212,11 @@ def h2_air_cells(n: int, seed: int = 4, T_range=(900.0, 2500.0), phi_range=(0.5,
     return out
 
 
-def ode_opts(rtol: float = 1e-8, atol: float = 1e-12, ns: int = 9):
+def ode_opts(rtol: float = 1e-8, atol: float = 1e-12, ns: int = 9, method: int = 1):
+    """method 1: RODAS, 2: RADAU5 (MOSE_ODE_RODAS / MOSE_ODE_RADAU5); raw tolerances for every component"""
     from .core import OdeOpts
     o = OdeOpts()
-    o.method = 1
+    o.method = method
     for q in range(ns + 1):
         o.rtol[q] = rtol
         o.atol[q] = atol

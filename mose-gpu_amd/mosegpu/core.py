@@ -399,7 +399,10 @@ class DeviceChem:
         self.ctx.check(lib().mose_chem_upload(self.ctx.h, self.id, self.n, dptr(st)), "mose_chem_upload")
 
     def advance(self, dt: float, opts) -> ChemReport:
+        """Raises MoseError(MOSE_ERR_ODE) when cells end with idid < 0; the report of the call, filled either way,
+        stays in last_report and the cells' states and idid can still be downloaded."""
         rep = ChemReport()
+        self.last_report = rep
         self.ctx.check(lib().mose_chem_advance(self.ctx.h, self.id, dt, C.byref(opts), C.byref(rep)),
                        "mose_chem_advance")
         return rep

@@ -14,11 +14,16 @@ The server works in rounds:
 Requests that arrive while others run are admitted at the next round: that is continuous batching, not a queue
 in front of a single case.
 
+Host preparation of arrived requests runs on a builder thread, overlapped with the device work of the resident
+cases.
+
 Time-to-solution = retirement time - arrival time (host preparation, queueing and device time included).
 """
 from __future__ import annotations
 
 import ctypes as C
+import queue
+import threading
 import time
 from dataclasses import dataclass, field
 
@@ -86,9 +91,13 @@ class ServeStats:
 
 
 def serve(reqs: list[Request], budget_bytes: int | None = None, quantum: int = 10, path: int = PATH_FUSED,
-          realtime: bool = True, log=None) -> ServeStats:
+          realtime: bool = True, log=None, lookahead: int = 4) -> ServeStats:
     """Run the stream to completion.  realtime: requests become visible at their arrival time (wall clock);
-    otherwise all are visible at once (offline throughput)."""
+    otherwise all are visible at once (offline throughput).
+
+    A builder thread prepares arrived requests on the host (mesh, metrics, boundary records, initial field), at most
+    `lookahead` ahead of admission, while the main thread steps the resident cases: the device calls release the
+    interpreter lock, so host preparation overlaps device work."""
     ctx_cache = {}
 
     def ctx_for(case):
@@ -97,39 +106,68 @@ def serve(reqs: list[Request], budget_bytes: int | None = None, quantum: int = 1
             ctx_cache[key] = Context(case.tables)
         return ctx_cache[key]
 
-    pending = sorted(reqs, key=lambda r: r.arrival)
-    resident = []       # (req, case, dc, remaining, bytes, ctx)
+    order = sorted(reqs, key=lambda r: r.arrival)
     t0 = time.perf_counter()
-    used = 0
     if budget_bytes is None:
         probe = Context(cases.air_tables())
         budget_bytes = int(0.8 * probe.info()["hbm_free"])
         probe.close()
+    built: queue.Queue = queue.Queue(maxsize=max(1, lookahead))
+    failure = []
+
+    def builder():
+        try:
+            for r in order:
+                if realtime:
+                    wait = r.arrival - (time.perf_counter() - t0)
+                    if wait > 0:
+                        time.sleep(wait)
+                case = build_case(r)
+                built.put((r, case, case_bytes(case)))
+        except BaseException as e:     # handed to the main thread
+            failure.append(e)
+        finally:
+            built.put(None)
+
+    th = threading.Thread(target=builder, name="mose-serve-builder", daemon=True)
+    th.start()
+    waiting = []        # built, not yet admitted: (req, case, bytes), arrival order
+    resident = []       # [req, case, dc, remaining, bytes, ctx]
+    used = 0
     rounds = 0
     max_res = 0
     max_used = 0
     cu = 0.0
-    while pending or resident:
-        now = time.perf_counter() - t0
-        # admission: arrived requests that fit, oldest first (a request that does not fit waits; later smaller
-        # ones may pass it only while it cannot fit at all with nothing resident)
-        while pending and (not realtime or pending[0].arrival <= now):
-            r = pending[0]
-            case = build_case(r)
-            nb = case_bytes(case)
+    stream_done = False
+    while True:
+        # collect what the builder has ready; block only when there is nothing else to do
+        while not stream_done:
+            try:
+                item = built.get(block=not (resident or waiting))
+            except queue.Empty:
+                break
+            if item is None:
+                stream_done = True
+            else:
+                waiting.append(item)
+        if failure:
+            raise failure[0]
+        # admission: oldest first while the HBM budget allows (a request too large for the budget runs alone)
+        while waiting:
+            r, case, nb = waiting[0]
             if used + nb > budget_bytes and resident:
                 break
-            pending.pop(0)
+            waiting.pop(0)
             ctx = ctx_for(case)
             dc = DeviceCase(ctx, case)
             dc.upload(case.P)
-            dc.tune(path, (0, 0, 0) if path == PATH_PLAIN else (0, 0, 0))
+            dc.tune(path, (0, 0, 0))
             r.admitted = time.perf_counter() - t0
             resident.append([r, case, dc, r.steps, nb, ctx])
             used += nb
         if not resident:
-            if pending and realtime:
-                time.sleep(max(0.0, pending[0].arrival - (time.perf_counter() - t0)))
+            if stream_done and not waiting:
+                break
             continue
         max_res = max(max_res, len(resident))
         max_used = max(max_used, used)
@@ -165,6 +203,7 @@ def serve(reqs: list[Request], budget_bytes: int | None = None, quantum: int = 1
             else:
                 keep.append(it)
         resident = keep
+    th.join()
     wall = time.perf_counter() - t0
     for c in ctx_cache.values():
         c.close()
@@ -174,7 +213,9 @@ def serve(reqs: list[Request], budget_bytes: int | None = None, quantum: int = 1
 
 
 def run_alone(r: Request, path: int = PATH_FUSED):
-    """The same request alone on the device, start to finish: (state hash, seconds)."""
+    """The same request alone on the device, start to finish.  Returns (state hash, device seconds, request
+    seconds): the stepping alone, and the whole request (host preparation, upload, tuning, stepping, hash)."""
+    t0 = time.perf_counter()
     case = build_case(r)
     ctx = Context(case.tables)
     dc = DeviceCase(ctx, case)
@@ -186,4 +227,4 @@ def run_alone(r: Request, path: int = PATH_FUSED):
     dt = time.perf_counter() - t
     dc.destroy()
     ctx.close()
-    return h, dt
+    return h, dt, time.perf_counter() - t0

@@ -21,7 +21,7 @@ def test_served_equals_alone(gpu_ready):
     assert all(r.done > 0 for r in reqs)
     assert st.max_resident >= 2          # requests overlapped on the device
     for r in reqs:
-        h, _ = serve.run_alone(r)
+        h, _, _ = serve.run_alone(r)
         assert r.hash == h, (r.kind, r.n, r.steps)
     print(f"served {len(reqs)} cases: {st.cases_per_s:.2f} cases/s, p50 {st.p50:.3f} s, p99 {st.p99:.3f} s, "
           f"max resident {st.max_resident}")
