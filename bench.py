@@ -6,6 +6,8 @@ SSP-RK3, FP64.  One "step" = one SSP-RK3 step of all 16.8 M cells (three fused s
 Prints one JSON line: value = cell-updates/s (cells x steps / s) with the state resident in HBM; roofline of the
 fused stage kernel from HIP events on its own stream (algorithmic bytes per cell-stage 8 (3 nprim + 25) = 320 B);
 cpu_baseline = the CPU oracle (a C restatement of the reference's double-precision path) on a bounded sample.
+Beside the headline line's own fields: "serving" (continuous batching of a Poisson stream of cases, p50/p99
+time-to-solution, against the plain per-case path) and "chemistry" (batched RODAS cells/s with its CPU sample).
 """
 from __future__ import annotations
 
@@ -57,6 +59,46 @@ def cpu_baseline(n: int, steps: int, threads: int):
                       f"(gcc -O2 -ffp-contract=off, OpenMP {threads} threads), {dt:.1f} s"}
 
 
+def chemistry_leg(ncell: int, cpu_cells: int, threads: int):
+    """Batched stiff chemistry: independent cells of the 9-species hydrogen-air mechanism at 1 atm, T uniform in
+    [900, 2500] K, equivalence ratio uniform in [0.5, 2] (seeded), each integrated over dt = 1e-6 s with RODAS at
+    rtol 1e-8 / atol 1e-12 in one persistent launch.  The 2M-cell batch is sampled by `ncell` cells (the per-cell
+    work does not depend on the batch size once the device is full); cells/s from the kernel's HIP events."""
+    import numpy as np
+    from mosegpu import Context, DeviceChem, chem
+    from mosegpu.core import ThermoTables
+    M = chem.h2_air_mechanism()
+    st = chem.h2_air_cells(ncell, seed=22)
+    ctx = Context(ThermoTables(M.wm, M.cp, M.h, names=M.names))
+    dc = DeviceChem(ctx, M, ncell)
+    dc.upload(st)
+    t0 = time.perf_counter()
+    rep = dc.advance(1e-6, chem.ode_opts(1e-8, 1e-12))
+    wall = time.perf_counter() - t0
+    out, hr, idid, nstep = dc.download()
+    dc.destroy()
+    ctx.close()
+    res = {"cells": ncell, "value": ncell / (rep.ms * 1e-3), "unit": "cells/s", "kernel": "k_chem (RODAS, 16-lane team "
+           "per cell, persistent queue)", "kernel_ms": rep.ms, "wall_s": wall, "steps_mean": float(nstep.mean()),
+           "steps_max": int(nstep.max()), "rhs_evals": int(rep.rhs_total), "failed": int((idid < 0).sum()),
+           "workload": "H2-O2-N2 9 species, 17 Arrhenius + 2 Troe, 1 atm, T U[900,2500] K, phi U[0.5,2], "
+                       "dt 1e-6 s, RODAS rtol 1e-8 atol 1e-12"}
+    print(f"bench: chemistry {ncell} cells in {rep.ms / 1e3:.2f} s: {res['value']:.3e} cells/s", file=sys.stderr,
+          flush=True)
+    if cpu_cells > 0:
+        sys.path.insert(0, str(REPO / "tests"))
+        from oracle_bind import OracleChem
+        sample = st[:, :cpu_cells].copy()
+        o = OracleChem(M, nthreads=threads)
+        t0 = time.perf_counter()
+        o.advance(sample, 1e-6)
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": cpu_cells / dt, "unit": "cells/s", "cores": threads, "kind": "port",
+                               "sample": f"the first {cpu_cells} cells of the same batch, CPU oracle RODAS (gcc -O2, "
+                                         f"OpenMP {threads} threads), {dt:.1f} s"}
+    return res
+
+
 def serving_leg(count: int, plain: bool):
     """A Poisson stream (20 cases/s) of 2D four-quadrant Riemann (n log-uniform in [64, 1024]) and 3D TGV (n
     log-uniform in [48, 192]) cases of 50-500 steps, served with continuous batching; p50/p99 time-to-solution."""
@@ -98,6 +140,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--probe-bw", action="store_true", help="only run the stream-copy probe (1 GiB buffers)")
     ap.add_argument("--serve", type=int, default=16, help="requests of the serving leg (0: skip)")
+    ap.add_argument("--chem", type=int, default=262144, help="cells of the chemistry leg (0: skip)")
+    ap.add_argument("--chem-cpu", type=int, default=2048, help="cells of the chemistry CPU baseline sample")
     ap.add_argument("--serve-plain", action="store_true", help="also run the serving requests one by one on the "
                     "plain per-block path (the comparison path)")
     args = ap.parse_args()
@@ -162,6 +206,8 @@ def main():
     ctx.close()
     if args.serve > 0:
         out["serving"] = serving_leg(args.serve, args.serve_plain)
+    if args.chem > 0:
+        out["chemistry"] = chemistry_leg(args.chem, 0 if args.no_cpu else args.chem_cpu, min(16, os.cpu_count() or 1))
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_steps, min(16, os.cpu_count() or 1))
     else:

@@ -1033,7 +1033,7 @@ extern "C" int mose_bandwidth_probe(mose_ctx* ctx, int64_t bytes, int reps, doub
 #include "mose_radau.hpp"
 
 namespace {
-constexpr int kChemTeamsPerWg = 16;
+constexpr int kChemTeamsPerWg = TeamShape<9>::per_wg;   // the kernels are instantiated for 9 species
 struct ChemObj {
   ChemDev M{};
   void* tabs = nullptr;          // device copies of the mechanism tables (one allocation)
@@ -1121,7 +1121,7 @@ extern "C" int mose_chem_create(mose_ctx* ctx, const mose_mechanism* m, int64_t 
   HIPCHK(ctx, hipGetLastError());
   int cus = 0;
   HIPCHK(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
-  // persistent grid of 256-lane workgroups (16 teams of 16 lanes, one cell per team at a time): enough to give
+  // persistent grid of 256-lane workgroups (24 teams of 10 lanes, one cell per team at a time): enough to give
   // every SIMD several waves
   o->grid = (int)std::min<int64_t>((ncell_max + kChemTeamsPerWg - 1) / kChemTeamsPerWg, (int64_t)cus * 8);
   o->cap = ncell_max;
@@ -1196,8 +1196,8 @@ extern "C" int mose_chem_advance(mose_ctx* ctx, int id, double dt, const mose_od
   HIPCHK(ctx, hipMemsetAsync(o->ctr, 0, 8 * sizeof(unsigned long long), s));
   int grid = (int)std::min<int64_t>(o->grid, (o->n + kChemTeamsPerWg - 1) / kChemTeamsPerWg);
   HIPCHK(ctx, hipEventRecord(o->ev[0], s));
-  if (opts->method == MOSE_ODE_RADAU5) k_radau<9><<<grid, kChemTeamsPerWg * TEAM, 0, s>>>(o->M, A);
-  else k_chem<9><<<grid, kChemTeamsPerWg * TEAM, 0, s>>>(o->M, A);
+  if (opts->method == MOSE_ODE_RADAU5) k_radau<9><<<grid, 256, 0, s>>>(o->M, A);
+  else k_chem<9><<<grid, 256, 0, s>>>(o->M, A);
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipEventRecord(o->ev[1], s));
   unsigned long long st[6];

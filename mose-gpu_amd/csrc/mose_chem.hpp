@@ -1,6 +1,6 @@
 // Per-cell stiff chemistry on the device: FLINT's general law-of-mass-action source, the constant-volume ODE of
 // rhs_native, and the RODAS integrator with the numerical Jacobian of the host path (wrap_rodas -> hairerRodas.f),
-// one team of 16 lanes per cell.
+// one team of NS + 1 lanes per cell.
 //
 //   net_rate    general()                 lib/FLINT/src/lib/Lib_Chemistry_wdot.f90:163-288 (+ falloff :12-80)
 //   team_rhs    rhs_native                lib/FLINT/src/lib/Lib_Chemistry_rhs.f90:29-81
@@ -8,7 +8,7 @@
 //   k_chem      ROSCOR (METH = 1, IJAC = 0, autonomous) hairerRodas.f:540-880 inside Chemistry_Newstate_blk
 //               (src/lib/physics/Lib_Chemistry.f90:33-76)
 //
-// A team of 16 lanes integrates one cell (row r of every matrix and component r of every vector on lane r).
+// A team of NS + 1 lanes integrates one cell (row r of every matrix and component r of every vector on lane r).
 // Work distribution: a persistent grid; each team takes its next cell from an atomic queue when its cell is done
 // (uneven stiffness -> uneven step counts), and every iteration of the team loop does one step attempt.
 #pragma once
@@ -100,15 +100,22 @@ __device__ __forceinline__ double rpow(double c, double nu) {
 }
 
 // ---------------------------------------------------------------------------------------------------------------
-// A team of 16 lanes integrates one cell: lane r of the team owns row r of the Jacobian and of the LU factors and
-// component r of every vector (rows 0..NS; lanes NS+1..15 compute on dummy data and keep every shuffle uniform).
-// Reactions are spread over the lanes (reaction q on lane q % 16); every sum the sequential statements form (species
+// A team of NS + 1 lanes integrates one cell: lane r of the team owns row r of the Jacobian and of the LU factors and
+// component r of every vector.  Reactions are spread over the lanes (reaction q on lane q % (NS + 1)); every sum the sequential statements form (species
 // production over the reactions, the energy sums, the pivot search, the error norm) is gathered into each lane
 // (through the team's LDS area or by shuffles) and added in the same order, so the team reproduces the sequential
 // statements bit for bit.
 // ---------------------------------------------------------------------------------------------------------------
-constexpr int TEAM = 16;
-constexpr int NRMAX = 32;   // reactions per mechanism in this kernel (two per lane)
+constexpr int SLOT = 16;    // species slots of the LDS tables (ns <= 16)
+constexpr int NRMAX = 32;   // reactions per mechanism in these kernels
+
+// A team is NS + 1 lanes (one per ODE component); a wavefront holds 64 / (NS + 1) teams and its remaining lanes idle.
+template <int NS>
+struct TeamShape {
+  static constexpr int size = NS + 1;
+  static constexpr int per_wave = 64 / size;
+  static constexpr int per_wg = 4 * per_wave;   // 256-lane workgroups
+};
 
 __device__ __forceinline__ double tsh(double v, int src) { return __shfl(v, src, 64); }
 __device__ __forceinline__ int tshi(int v, int src) { return __shfl(v, src, 64); }
@@ -120,7 +127,7 @@ __device__ __forceinline__ int tshi(int v, int src) { return __shfl(v, src, 64);
 struct RxnLds {
   unsigned long long o1[NRMAX], o2[NRMAX];
   int flags[NRMAX];          // bit 0: third body; bit 1: the orders are packed in o1 / o2
-  double eps[NRMAX * TEAM];  // eps(q, s) at [q * TEAM + s]
+  double eps[NRMAX * SLOT];  // eps(q, s) at [q * SLOT + s]
 };
 
 template <int NS>
@@ -150,8 +157,8 @@ __device__ __forceinline__ void rxn_build(const ChemDev& M, RxnLds& X) {
     X.o2[q] = o2;
     X.flags[q] = fl;
   }
-  for (int i = threadIdx.x; i < NRMAX * TEAM; i += blockDim.x) {
-    const int q = i / TEAM, s = i % TEAM;
+  for (int i = threadIdx.x; i < NRMAX * SLOT; i += blockDim.x) {
+    const int q = i / SLOT, s = i % SLOT;
     X.eps[i] = (q < nr && s < NS) ? M.eps[(size_t)q * W + s] : 0.0;
   }
 }
@@ -167,7 +174,7 @@ __device__ __forceinline__ double net_rate(const ChemDev& M, const RxnLds& X, in
   if (fl & 1) {
     coM = 0.0;
 #pragma unroll
-    for (int s = 0; s < NS; ++s) coM = coM + coi[s] * X.eps[q * TEAM + s];
+    for (int s = 0; s < NS; ++s) coM = coM + coi[s] * X.eps[q * SLOT + s];
   } else {
     coM = 1.0;
   }
@@ -216,12 +223,12 @@ __device__ __forceinline__ double net_rate(const ChemDev& M, const RxnLds& X, in
 // The exchange area of one team in LDS.  A team's lanes are in one wavefront, so a write by one lane is seen by the
 // others after a wavefront-scope fence (no workgroup barrier).
 struct TeamLds {
-  double z[TEAM];          // the argument Z of the rhs, one component per lane
-  double c[TEAM];          // the concentrations max(Z_s, 0) / W_s
+  double z[SLOT];          // the argument Z of the rhs, one component per lane
+  double c[SLOT];          // the concentrations max(Z_s, 0) / W_s
   double net[NRMAX];       // rates of progress, reaction q at [q]
-  double ei[TEAM];         // per-species terms of the energy sums
-  double rc[TEAM];
-  double q[TEAM];          // squared scaled errors
+  double ei[SLOT];         // per-species terms of the energy sums
+  double rc[SLOT];
+  double q[SLOT];          // squared scaled errors
 };
 
 __device__ __forceinline__ void wsync() {
@@ -231,7 +238,7 @@ __device__ __forceinline__ void wsync() {
 }
 
 // rhs_native (Lib_Chemistry_rhs.f90:29-81) for the team: z = this lane's component of Z, returns this lane's F.
-// cw[q * TEAM + s] = wm(s) * (nu''(q, s) - nu'(q, s)), the first product of the production statement of general()
+// cw[q * SLOT + s] = wm(s) * (nu''(q, s) - nu'(q, s)), the first product of the production statement of general()
 // (Lib_Chemistry_wdot.f90:280-284), so that omega_s = sum over q in order of cw * net_q rounds as that statement.
 template <int NS>
 __device__ __forceinline__ double team_rhs(const ChemDev& M, const RxnLds& X, const double* cw, TeamLds& L, int r,
@@ -250,16 +257,16 @@ __device__ __forceinline__ double team_rhs(const ChemDev& M, const RxnLds& X, co
   for (int s = 0; s < NS; ++s) coi[s] = L.c[s];
   const int nr = M.na + M.nt + M.nl;
 #pragma unroll 1
-  for (int p = 0; p < NRMAX / TEAM; ++p) {
-    const int q = r + p * TEAM;
-    if (p * TEAM < nr) L.net[q] = q < nr ? net_rate<NS>(M, X, q, coi, Ti, Tf) : 0.0;
+  for (int q0 = 0; q0 < nr; q0 += NZ) {
+    const int q = q0 + r;
+    if (q < nr) L.net[q] = net_rate<NS>(M, X, q, coi, Ti, Tf);
   }
   wsync();
   // omega_s on lane s: production summed over the reactions in order
   double om = 0.0;
 #pragma unroll
   for (int q = 0; q < NRMAX; ++q)
-    if (q < nr) om = om + cw[q * TEAM + rs] * L.net[q];
+    if (q < nr) om = om + cw[q * SLOT + rs] * L.net[q];
   // the energy sums over the species, in order, on the lane of the T equation
   const int row = Ti - M.tmin;
   const double* ht = M.h + (size_t)rs * M.rows;
@@ -357,7 +364,7 @@ struct RodasC {
   static constexpr double C64 = 0.1631930543123136e+02, C65 = -0.6058818238834054e+01;
 };
 
-// One team of 16 lanes per cell, persistent over the cell queue.  The team loop is a state machine in which every
+// One team of NS + 1 lanes per cell, persistent over the cell queue.  The team loop is a state machine in which every
 // iteration makes exactly one rhs evaluation (the only call site, so all teams of a wavefront share it whatever
 // phase of the step they are in):
 //   EV_F0        f(y) at the start of a step
@@ -372,8 +379,8 @@ template <int NS>
 __device__ __forceinline__ void chem_setup(const ChemDev& M, double* cw, RxnLds& X) {
   rxn_build<NS>(M, X);
   const int nr = M.na + M.nt + M.nl;
-  for (int i = threadIdx.x; i < NRMAX * TEAM; i += blockDim.x) {
-    const int q = i / TEAM, s = i % TEAM;
+  for (int i = threadIdx.x; i < NRMAX * SLOT; i += blockDim.x) {
+    const int q = i / SLOT, s = i % SLOT;
     double v = 0.0;
     if (q < nr && s < NS) {
       double dn = M.ni2[(size_t)q * (NS + 1) + s] - M.ni1[(size_t)q * (NS + 1) + s];
@@ -435,21 +442,24 @@ template <int NS>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_chem(ChemDev M, ChemArgs A) {
   constexpr int NZ = NS + 1;
   constexpr int EV_S = EV_J + NZ;   // EV_S + (k - 2): the rhs of stage k
-  static_assert(NZ <= TEAM, "one lane per ODE component");
+  static_assert(NZ <= SLOT, "one lane per ODE component");
+  using TSh = TeamShape<NS>;
   using R = RodasC;
   const double uround = 1.1e-19, safe = 0.9, fac1 = 5.0, fac2 = 1.0 / 6.0;
   const int nmax = 100000;
 
-  __shared__ double cw[NRMAX * TEAM];
-  __shared__ TeamLds tl[256 / TEAM];
+  __shared__ double cw[NRMAX * SLOT];
+  __shared__ TeamLds tl[TSh::per_wg];
   __shared__ RxnLds X;
   chem_setup<NS>(M, cw, X);
 
   const int wl = threadIdx.x & 63;
-  const int base = wl & ~(TEAM - 1);     // lane (in the wave) of the team's row 0
-  const int r = wl & (TEAM - 1);         // this lane's row / component
-  const int rc = r < NZ ? r : NZ - 1;    // a valid component index for the dummy lanes
-  TeamLds& L = tl[threadIdx.x / TEAM];
+  const int tw = wl / NZ;                // the team within the wavefront
+  if (tw >= TSh::per_wave) return;       // the lanes past the last whole team (no barrier follows)
+  const int base = tw * NZ;              // lane (in the wave) of the team's row 0
+  const int r = wl - base;               // this lane's row / component
+  const int rc = r;
+  TeamLds& L = tl[(threadIdx.x >> 6) * TSh::per_wave + tw];
 
   int cell = -1, ev = EV_NONE;
   bool reject = false, last = false;

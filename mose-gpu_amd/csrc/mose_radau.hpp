@@ -7,7 +7,7 @@
 //   team_decc / team_solc   DECC / SOLC   Lib_Radau5_dev.f90:1042-1177 (hairerToolbox.f)
 //   k_radau                 RADCOR        lib/FLINT/lib/OSlo/src/lib/hairerRadau5.f:671-1000
 //
-// Same team layout as k_chem: 16 lanes per cell, row / component r on lane r.  The tolerance transform
+// Same team layout as k_chem: NS + 1 lanes per cell, row / component r on lane r.  The tolerance transform
 // (rtol' = 0.1 rtol^(2/3), atol' = rtol' atol / rtol, FNEWT; set_radau5_dev_tols, Lib_Radau5_dev.f90:71-107) and the
 // method constants (the cube roots of RADCOR's preparation) are computed once on the host and arrive in ChemArgs.
 #pragma once
@@ -161,7 +161,8 @@ template <int NS>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1))) k_radau(ChemDev M, ChemArgs A) {
   constexpr int NZ = NS + 1;
   constexpr int RV_N1 = RV_J + NZ, RV_N2 = RV_N1 + 1, RV_N3 = RV_N1 + 2, RV_EST = RV_N1 + 3;
-  static_assert(NZ <= TEAM, "one lane per ODE component");
+  static_assert(NZ <= SLOT, "one lane per ODE component");
+  using TSh = TeamShape<NS>;
   using T = RadauT;
   const double uround = 1.1e-19, safe = 0.9, thet = 0.001, quot1 = 1.0, quot2 = 1.2, facl = 5.0, facr = 1.0 / 8.0;
   const int nmax = 100000, nit = 7;
@@ -170,18 +171,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1))) k
   const double fnewt = A.rk[11];
   const double cfac = safe * (1 + 2 * nit);
 
-  __shared__ double cw[NRMAX * TEAM];
-  __shared__ TeamLds tl[256 / TEAM];
+  __shared__ double cw[NRMAX * SLOT];
+  __shared__ TeamLds tl[TSh::per_wg];
   __shared__ RxnLds X;
-  __shared__ double fjs[256 / TEAM][NZ * TEAM];   // the Jacobian of each team: a(r, j) at [j * TEAM + r]
+  __shared__ double fjs[TSh::per_wg][NZ * SLOT];   // the Jacobian of each team: a(r, j) at [j * SLOT + r]
   chem_setup<NS>(M, cw, X);
 
   const int wl = threadIdx.x & 63;
-  const int base = wl & ~(TEAM - 1);
-  const int r = wl & (TEAM - 1);
-  const int rc = r < NZ ? r : NZ - 1;
-  TeamLds& L = tl[threadIdx.x / TEAM];
-  double* fj = fjs[threadIdx.x / TEAM];
+  const int tw = wl / NZ;
+  if (tw >= TSh::per_wave) return;       // the lanes past the last whole team (no barrier follows)
+  const int base = tw * NZ;
+  const int r = wl - base;
+  const int rc = r;
+  const int team = (threadIdx.x >> 6) * TSh::per_wave + tw;
+  TeamLds& L = tl[team];
+  double* fj = fjs[team];
 
   int cell = -1, ev = RV_NONE, after = LB_NONE;
   int nsing = 0, ier = 0, newt = 0, nstep = 0, naccpt = 0, nrejct = 0, nfcn = 0, idid = 0;
@@ -239,7 +243,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1))) k
       lab = after;
     } else if (ev < RV_N1) {
       const int i = ev - RV_J;
-      fj[i * TEAM + r] = (f - y0) / delt;
+      fj[i * SLOT + r] = (f - y0) / delt;
       if (i + 1 < NZ) {
         column_arg(i + 1);
         ev++;
@@ -277,7 +281,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1))) k
         betan = beta / h;
 #pragma unroll
         for (int j = 0; j < NZ; ++j) {
-          const double a = -fj[j * TEAM + r];
+          const double a = -fj[j * SLOT + r];
           e1[j] = (j == r) ? a + fac1 : a;
           e2r[j] = (j == r) ? a + alphn : a;
           e2i[j] = (j == r) ? betan : 0.0;
