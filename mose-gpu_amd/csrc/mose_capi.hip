@@ -594,10 +594,11 @@ static int tile_slots(int TI, int TJ, int TK) { return (TI + 1) * TJ * TK + TI *
 
 static int hs_slots(int TI, int TJ, int TK) { return (TI + 2) * TJ * TK + TI * (TJ + 2) * TK + TI * TJ * (TK + 2); }
 
-// LDS doubles of region 0 (half-slopes, then the viscous cell cache) and the bytes of the whole tile
+// LDS doubles of region 0 (half-slopes, then the viscous cell cache: rho, 1/rho, T and the primitives of the tile
+// and its one-cell halo) and the bytes of the whole tile
 static int stage_r0(const Case& c, int TI, int TJ, int TK) {
   int r0 = hs_slots(TI, TJ, TK) * c.nprim;
-  if (c.sc.viscous) r0 = std::max(r0, 3 * (TI + 2) * (TJ + 2) * (TK + 2));
+  if (c.sc.viscous) r0 = std::max(r0, (3 + c.nprim) * (TI + 2) * (TJ + 2) * (TK + 2));
   return (r0 + 31) & ~31;
 }
 static size_t stage_lds(const Case& c, int TI, int TJ, int TK) {
@@ -611,17 +612,25 @@ static size_t stage_lds_(int nprim, bool visc, int TI, int TJ, int TK) {
 }
 
 static int set_tiles(mose_ctx* ctx, Case& c, int TI, int TJ, int TK) {
-  if (TI < 1 || TJ < 1 || TK < 1 || TI * TJ * TK != 256) return fail(ctx, MOSE_ERR_ARG, "tile must hold 256 cells");
+  if (TI < 1 || TJ < 1 || TK < 1 || (TI * TJ * TK != 256 && TI * TJ * TK != 512))
+    return fail(ctx, MOSE_ERR_ARG, "a tile holds 256 or 512 cells");
   if (stage_lds(c, TI, TJ, TK) + kStaticLds > (size_t)ctx->max_lds)
     return fail(ctx, MOSE_ERR_ARG, "tile " + std::to_string(TI) + "x" + std::to_string(TJ) + "x" + std::to_string(TK) +
                                        " needs " + std::to_string(stage_lds(c, TI, TJ, TK)) + " B of LDS, the device allows " +
                                        std::to_string(ctx->max_lds));
+  // tiles block by block, in super-tiles of 4 x 4 x 4 tiles (i fastest inside a super-tile and between them):
+  // neighbouring tiles run close together in time and share the L2 of their XCD (1.4 % faster than plain i-fastest
+  // order on the 256^3 Taylor-Green case)
   std::vector<Tile> t;
   for (int b = 0; b < c.nb; ++b) {
     const mose_blk& B = c.blk[b];
-    for (int k0 = 1; k0 <= B.n3; k0 += TK)
-      for (int j0 = 1; j0 <= B.n2; j0 += TJ)
-        for (int i0 = 1; i0 <= B.n1; i0 += TI) t.push_back(Tile{b, i0, j0, k0});
+    const int na = (B.n1 + TI - 1) / TI, nbj = (B.n2 + TJ - 1) / TJ, nc = (B.n3 + TK - 1) / TK;
+    for (int sc = 0; sc < nc; sc += 4)
+      for (int sb = 0; sb < nbj; sb += 4)
+        for (int sa = 0; sa < na; sa += 4)
+          for (int cc = sc; cc < std::min(sc + 4, nc); ++cc)
+            for (int bb = sb; bb < std::min(sb + 4, nbj); ++bb)
+              for (int a = sa; a < std::min(sa + 4, na); ++a) t.push_back(Tile{b, 1 + a * TI, 1 + bb * TJ, 1 + cc * TK});
   }
   if ((int)t.size() > c.tiles_cap) {
     if (c.dtiles) HIPCHK(ctx, hipFree(c.dtiles));
@@ -637,6 +646,20 @@ static int set_tiles(mose_ctx* ctx, Case& c, int TI, int TJ, int TK) {
   return ensure_rpart(ctx, c, c.ntiles);
 }
 
+template <int NS, int NR, int BS>
+static void launch_stage_bs(Case& c, const CaseDev& D, hipStream_t s, const StageArgs& A, unsigned grid, size_t lds) {
+  const int rs = c.sc.riemann;
+  if (c.sc.viscous) {
+    if (rs == MOSE_RS_LMROE) k_stage<NS, NR, true, MOSE_RS_LMROE, BS><<<grid, BS, lds, s>>>(D, A);
+    else if (rs == MOSE_RS_AUSMP) k_stage<NS, NR, true, MOSE_RS_AUSMP, BS><<<grid, BS, lds, s>>>(D, A);
+    else k_stage<NS, NR, true, MOSE_RS_HLLC, BS><<<grid, BS, lds, s>>>(D, A);
+  } else {
+    if (rs == MOSE_RS_LMROE) k_stage<NS, NR, false, MOSE_RS_LMROE, BS><<<grid, BS, lds, s>>>(D, A);
+    else if (rs == MOSE_RS_AUSMP) k_stage<NS, NR, false, MOSE_RS_AUSMP, BS><<<grid, BS, lds, s>>>(D, A);
+    else k_stage<NS, NR, false, MOSE_RS_HLLC, BS><<<grid, BS, lds, s>>>(D, A);
+  }
+}
+
 template <int NS, int NR>
 static void launch_stage(Case& c, const CaseDev& D, hipStream_t s, double rkc, bool last) {
   StageArgs A;
@@ -648,16 +671,8 @@ static void launch_stage(Case& c, const CaseDev& D, hipStream_t s, double rkc, b
   A.last = last ? 1 : 0; A.rkc = rkc; A.dtnext = c.dtnext; A.rpart = c.rpart;
   size_t lds = stage_lds(c, c.TI, c.TJ, c.TK);
   unsigned grid = (unsigned)(((c.ntiles + 7) / 8) * 8);
-  const int rs = c.sc.riemann;
-  if (c.sc.viscous) {
-    if (rs == MOSE_RS_LMROE) k_stage<NS, NR, true, MOSE_RS_LMROE><<<grid, 256, lds, s>>>(D, A);
-    else if (rs == MOSE_RS_AUSMP) k_stage<NS, NR, true, MOSE_RS_AUSMP><<<grid, 256, lds, s>>>(D, A);
-    else k_stage<NS, NR, true, MOSE_RS_HLLC><<<grid, 256, lds, s>>>(D, A);
-  } else {
-    if (rs == MOSE_RS_LMROE) k_stage<NS, NR, false, MOSE_RS_LMROE><<<grid, 256, lds, s>>>(D, A);
-    else if (rs == MOSE_RS_AUSMP) k_stage<NS, NR, false, MOSE_RS_AUSMP><<<grid, 256, lds, s>>>(D, A);
-    else k_stage<NS, NR, false, MOSE_RS_HLLC><<<grid, 256, lds, s>>>(D, A);
-  }
+  if (c.TI * c.TJ * c.TK == 512) launch_stage_bs<NS, NR, 512>(c, D, s, A, grid, lds);
+  else launch_stage_bs<NS, NR, 256>(c, D, s, A, grid, lds);
 }
 
 // records the step's dt in the history at the device-side position (the same kernel serves eager and graph steps)
@@ -759,10 +774,11 @@ static int run_steps(mose_ctx* ctx, Case& c, int nsteps) {
 
 template <int NS, int NR>
 static int autotune(mose_ctx* ctx, Case& c, mose_tuning* chosen) {
-  // candidates of 256 cells; each runs the ghost, record and stage kernels of stage 1 (into the scratch buffer PA,
-  // the state P0 untouched) a few times, the fastest is kept
+  // candidates of 256 and 512 cells; each runs the ghost, record and stage kernels of stage 1 (into the scratch
+  // buffer PA, the state P0 untouched) a few times, the fastest is kept
   static const int cand[][3] = {{32, 4, 2}, {64, 2, 2}, {16, 4, 4}, {16, 8, 2}, {8, 8, 4}, {64, 4, 1}, {32, 8, 1},
-                                {128, 2, 1}, {256, 1, 1}};
+                                {128, 2, 1}, {256, 1, 1}, {8, 8, 8}, {16, 8, 4}, {16, 4, 8}, {32, 4, 4}, {16, 16, 2},
+                                {32, 16, 1}, {64, 8, 1}};
   hipStream_t s = c.stream;
   hipEvent_t e0, e1;
   HIPCHK(ctx, hipEventCreate(&e0));

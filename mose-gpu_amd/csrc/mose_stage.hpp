@@ -1,7 +1,9 @@
-// The fused stage kernel: one workgroup per tile of TI x TJ x TK cells of any block of the case.
+// The fused stage kernel: one workgroup per tile of TI x TJ x TK cells (256 or 512, one thread per cell) of any block
+// of the case.
 //
 //   phase A  every interior face of the tile (TI+1)TJTK + TI(TJ+1)TK + TITJ(TK+1): the convective flux into LDS;
-//            the per-cell (rho, 1/rho, T) of the tile and its one-cell halo into LDS (viscous cases)
+//            the per-cell (rho, 1/rho, T) and the primitives of the tile and its one-cell halo into LDS (viscous
+//            cases), so that the viscous faces read their stencils from LDS
 //   phase B  each cell sums its six convective faces in the fixed order of pass2_body.inc
 //   phase C  every interior face of the tile: the viscous flux into the same LDS buffer
 //   phase D  each cell subtracts its six viscous faces (pass2_body.inc), adds the boundary-record increments in face
@@ -28,7 +30,7 @@ struct StageArgs {
   int TI, TJ, TK;
   int nslot;          // faces per tile (LDS slots)
   int nhs;            // half-slope slots: (TI+2)TJTK + TI(TJ+2)TK + TITJ(TK+2)
-  int ncache;         // (TI+2)(TJ+2)(TK+2) for the viscous cell cache
+  int ncache;         // (TI+2)(TJ+2)(TK+2) cells of the viscous cell cache
   int r0;             // doubles of LDS region 0 (half slopes, then the viscous cell cache)
   int last;           // 1 on the last RK stage: take the dt limit of the new state
   double rkc;
@@ -37,7 +39,7 @@ struct StageArgs {
 };
 
 // One interior viscous face of direction D (diff_cuf_i/j/k + diff_physics.inc) with the cell values (rho, 1/rho, T)
-// taken from the tile's LDS cache; (i,j,k) = the face's left cell, (a,b,c) = its tile-local face coordinates.
+// and the primitives of its stencil taken from the tile's LDS cache; (i,j,k) = the face's left cell, (a,b,c) = its tile-local face coordinates.
 template <int NS, int NR, int D>
 __device__ __forceinline__ void visc_face(const CaseDev& C, const mose_blk& B, const Tile& tl, const StageArgs& A,
                                           const double* cache, int i, int j, int k, int a, int b, int c, double* res) {
@@ -56,9 +58,11 @@ __device__ __forceinline__ void visc_face(const CaseDev& C, const mose_blk& B, c
   int k1 = (a + 1 - (D == 0)) + CI * ((b + 1 - (D == 1)) + CJ * (c + 1 - (D == 2)));
   int k2 = k1 + kst;
   int64_t c0 = row(B, i, j, k), c1 = c0 + st;
+  // the primitives of the tile and its one-cell halo, copied into LDS after the cell scalars: P_s at [(3 + s) NCa]
+  const double* Pc = cache + 3 * NCa;
   double q0[NP], q1[NP], G[NP][3];
 #pragma unroll
-  for (int s = 0; s < NP; ++s) { q0[s] = P[s * NC + c0]; q1[s] = P[s * NC + c1]; }
+  for (int s = 0; s < NP; ++s) { q0[s] = Pc[s * NCa + k1]; q1[s] = Pc[s * NCa + k2]; }
   double ir1 = cache[NCa + k1], ir2 = cache[NCa + k2];
 #pragma unroll
   for (int s = 0; s < NS; ++s) G[s][D] = q1[s] * ir2 - q0[s] * ir1;
@@ -72,19 +76,18 @@ __device__ __forceinline__ void visc_face(const CaseDev& C, const mose_blk& B, c
     const int ax = pass ? ic : ib;
     const int64_t sd = pass ? sc : sb;
     const int kd = pass ? kc : kb;
-    int64_t a3 = c0 - sd, a4 = c0 + sd, a5 = c1 - sd, a6 = c1 + sd;
     int k3 = k1 - kd, k4 = k1 + kd, k5 = k2 - kd, k6 = k2 + kd;
     double i3 = cache[NCa + k3], i4 = cache[NCa + k4], i5 = cache[NCa + k5], i6 = cache[NCa + k6];
 #pragma unroll
     for (int s = 0; s < NS; ++s)
-      G[s][ax] = (P[s * NC + a4] * i4 - P[s * NC + a3] * i3 + P[s * NC + a6] * i6 - P[s * NC + a5] * i5) * 0.25;
+      G[s][ax] = (Pc[s * NCa + k4] * i4 - Pc[s * NCa + k3] * i3 + Pc[s * NCa + k6] * i6 - Pc[s * NCa + k5] * i5) * 0.25;
     G[V::np][ax] = (cache[2 * NCa + k4] - cache[2 * NCa + k3] + cache[2 * NCa + k6] - cache[2 * NCa + k5]) * 0.25;
 #pragma unroll
     for (int s = V::nu; s <= V::nw; ++s)
-      G[s][ax] = (P[s * NC + a4] - P[s * NC + a3] + P[s * NC + a6] - P[s * NC + a5]) * 0.25;
+      G[s][ax] = (Pc[s * NCa + k4] - Pc[s * NCa + k3] + Pc[s * NCa + k6] - Pc[s * NCa + k5]) * 0.25;
 #pragma unroll
     for (int s = V::nt; s < NP; ++s)
-      G[s][ax] = (P[s * NC + a4] * i4 - P[s * NC + a3] * i3 + P[s * NC + a6] * i6 - P[s * NC + a5] * i5) * 0.25;
+      G[s][ax] = (Pc[s * NCa + k4] * i4 - Pc[s * NCa + k3] * i3 + Pc[s * NCa + k6] * i6 - Pc[s * NCa + k5] * i5) * 0.25;
   }
   double Mav[3][3];
 #pragma unroll
@@ -122,7 +125,7 @@ __device__ __forceinline__ void stage_tile(const CaseDev& C, const StageArgs& A,
   const int NI = (TI + 1) * TJ * TK, NJ = TI * (TJ + 1) * TK;
   const int NF = A.nslot;
   double* Hs = sm;                       // region 0: [NP][nhs] half-slopes (phase A)
-  double* cache = sm;                    // region 0: [3][ncache] rho, 1/rho, T (phases B-D)
+  double* cache = sm;                    // region 0: [3 + NP][ncache] rho, 1/rho, T, P (phases B-D)
   double* Fb = sm + A.r0;                // region 1: [NP][NF] face fluxes
   const int64_t si = 1, sj = B.pi, sk = (int64_t)B.pi * B.pj;
 
@@ -196,6 +199,8 @@ __device__ __forceinline__ void stage_tile(const CaseDev& C, const StageArgs& A,
       cache[q] = rho;
       cache[A.ncache + q] = 1.0 / rho;
       cache[2 * A.ncache + q] = P[V::np * NC + r0] / (rho * f_Rtot<NS>(T, y));
+#pragma unroll
+      for (int s = 0; s < NP; ++s) cache[(3 + s) * A.ncache + q] = P[s * NC + r0];
     }
   }
 
@@ -328,8 +333,11 @@ __device__ __forceinline__ int xcd_tile(int wg, int n) {
   return q < cnt ? base + q : -1;
 }
 
-template <int NS, int NR, bool VISC, int RS>
-__global__ void __launch_bounds__(256) k_stage(CaseDev C, StageArgs A) {
+// BS = the workgroup size = the cells of a tile (256 or 512).  Larger tiles recompute fewer halo half-slopes and
+// faces and re-read less of their neighbours; MI355X's 160 KB of LDS per workgroup holds an 8 x 8 x 8 tile's
+// half-slopes and face fluxes.
+template <int NS, int NR, bool VISC, int RS, int BS>
+__global__ void __launch_bounds__(BS) k_stage(CaseDev C, StageArgs A) {
   const int t = xcd_tile(blockIdx.x, A.ntiles);
   if (t < 0) return;
   stage_tile<NS, NR, VISC, RS>(C, A, t);
