@@ -5,6 +5,8 @@
 !> (src/lib/numerics/time/explicit/Mod_Explicit.f90:155-294): Fill_Ghost_Cell, Refresh_Thermo_Cache, Fluxes,
 !> BC_Fluxes, RANS_Source_Terms and RK_Newstate become one mose_case_step per step batch, Compute_dt becomes
 !> mose_case_dt, Compute_Residual_blk becomes mose_case_residual_norm and field_state_hash mose_case_state_hash.
+!> Chemistry_Newstate_blk (src/lib/physics/Lib_Chemistry.f90:33-76) becomes mose_chem_upload / mose_chem_advance /
+!> mose_chem_download on a batch of cells.
 module mose_gpu_c
   use iso_c_binding
   implicit none
@@ -57,6 +59,27 @@ module mose_gpu_c
     integer(c_int32_t) :: path, ti, tj, tk, timing, no_graph
     integer(c_int32_t) :: reserved(2)
   end type mose_tuning
+
+  ! stiff chemistry (Chemistry_Newstate_blk, src/lib/physics/Lib_Chemistry.f90:33-76)
+  integer(c_int), parameter :: MOSE_ODE_RODAS = 1, MOSE_ODE_RADAU5 = 2
+
+  type, bind(C) :: mose_mechanism
+    integer(c_int32_t) :: ns, n_arrh, n_troe, n_lind, tmin, tmax
+    type(c_ptr) :: wm, cp, h, ni1, ni2, eps, kf, kb, troe, lind
+  end type mose_mechanism
+
+  type, bind(C) :: mose_ode_opts
+    integer(c_int32_t) :: method
+    integer(c_int32_t) :: reserved(3)
+    real(c_double) :: rtol(MOSE_MAX_SPECIES + 1), atol(MOSE_MAX_SPECIES + 1)
+  end type mose_ode_opts
+
+  type, bind(C) :: mose_chem_report
+    integer(c_int64_t) :: cells, failed, steps_total, rejected_total, rhs_total
+    integer(c_int32_t) :: steps_max, first_failed_idid
+    integer(c_int64_t) :: first_failed_cell
+    real(c_double) :: ms
+  end type mose_chem_report
 
   interface
     integer(c_int) function mose_device_count(n) bind(C, name="mose_device_count")
@@ -172,6 +195,40 @@ module mose_gpu_c
       integer(c_int), value :: case_id
       type(mose_tuning), intent(in) :: tuning
       type(mose_tuning), intent(out) :: chosen
+    end function
+    integer(c_int) function mose_chem_create(ctx, mech, ncell_max, chem_id) bind(C, name="mose_chem_create")
+      import :: c_int, c_ptr, c_int64_t, mose_mechanism
+      type(c_ptr), value :: ctx
+      type(mose_mechanism), intent(in) :: mech
+      integer(c_int64_t), value :: ncell_max
+      integer(c_int), intent(out) :: chem_id
+    end function
+    integer(c_int) function mose_chem_destroy(ctx, chem_id) bind(C, name="mose_chem_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: chem_id
+    end function
+    integer(c_int) function mose_chem_upload(ctx, chem_id, ncell, state) bind(C, name="mose_chem_upload")
+      import :: c_int, c_ptr, c_int64_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: chem_id
+      integer(c_int64_t), value :: ncell
+      real(c_double), intent(in) :: state(*)
+    end function
+    integer(c_int) function mose_chem_download(ctx, chem_id, state, hr, idid, nstep) bind(C, name="mose_chem_download")
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: chem_id
+      real(c_double), intent(out) :: state(*), hr(*)
+      integer(c_int32_t), intent(out) :: idid(*), nstep(*)
+    end function
+    integer(c_int) function mose_chem_advance(ctx, chem_id, dt, opts, report) bind(C, name="mose_chem_advance")
+      import :: c_int, c_ptr, c_double, mose_ode_opts, mose_chem_report
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: chem_id
+      real(c_double), value :: dt
+      type(mose_ode_opts), intent(in) :: opts
+      type(mose_chem_report), intent(out) :: report
     end function
   end interface
 

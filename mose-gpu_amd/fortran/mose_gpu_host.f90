@@ -27,6 +27,39 @@ contains
     end if
   end subroutine MOSE_GPU_Explicit_Step
 
+  !> Chemistry_Newstate_blk (src/lib/physics/Lib_Chemistry.f90:33-76) on a batch of ncell cells resident in the
+  !> chemistry object chem_id: state(ncell, ns + 1) holds the species densities and, in column ns + 1, the pressure
+  !> (the chemistry columns of P_flat).  Every cell: T = p / (rho R), the integrator (MOSE_ODE_RODAS or
+  !> MOSE_ODE_RADAU5, raw tolerances per component) over [0, dt], floors, p = T rho R; hr(c) = |T_new - T_old| / T_old.
+  !> Like the reference's loop, a cell whose integration fails (idid < 0) is written back as the integrator left it;
+  !> the count of such cells is returned in nfailed and reported once.
+  subroutine MOSE_GPU_Chemistry_Newstate(ctx, chem_id, ncell, ns, state, dt, method, rtol, atol, hr, nfailed)
+    type(c_ptr), intent(in) :: ctx
+    integer(c_int), intent(in) :: chem_id, ns, method
+    integer(c_int64_t), intent(in) :: ncell
+    real(c_double), intent(inout) :: state(ncell, ns + 1)
+    real(c_double), intent(in) :: dt, rtol(ns + 1), atol(ns + 1)
+    real(c_double), intent(out) :: hr(ncell)
+    integer(c_int64_t), intent(out) :: nfailed
+    type(mose_ode_opts) :: opts
+    type(mose_chem_report) :: rep
+    integer(c_int32_t), allocatable :: idid(:), nstep(:)
+    integer(c_int) :: rc
+    call mose_check(ctx, mose_chem_upload(ctx, chem_id, ncell, state), 'mose_chem_upload')
+    opts%method = method
+    opts%reserved = 0
+    opts%rtol = 0.0d0
+    opts%atol = 0.0d0
+    opts%rtol(1:ns + 1) = rtol
+    opts%atol(1:ns + 1) = atol
+    rc = mose_chem_advance(ctx, chem_id, dt, opts, rep)
+    if (rc /= MOSE_OK .and. rc /= MOSE_ERR_ODE) call mose_check(ctx, rc, 'mose_chem_advance')
+    nfailed = rep%failed
+    if (nfailed > 0) write(*,'(a,i0,a)') ' [WARNING] chemistry: ', nfailed, ' cells ended with idid < 0'
+    allocate(idid(ncell), nstep(ncell))
+    call mose_check(ctx, mose_chem_download(ctx, chem_id, state, hr, idid, nstep), 'mose_chem_download')
+  end subroutine MOSE_GPU_Chemistry_Newstate
+
   !> Stops with the library's message when a call failed.
   subroutine mose_check(ctx, rc, what)
     type(c_ptr), intent(in) :: ctx

@@ -1,5 +1,6 @@
-"""The Fortran host (mose-gpu_amd/fortran: the ISO_C_BINDING module, the host sequencer and the shock-tube driver)
-drives the library through the C ABI and gets the oracle's state: the same field_state_hash, time and profile."""
+"""The Fortran host (mose-gpu_amd/fortran: the ISO_C_BINDING module, the host sequencer, the shock-tube and chemistry
+drivers) drives the library through the C ABI and gets the oracle's state: the same field_state_hash, time and
+profile, and the same chemistry state bit for bit."""
 import subprocess
 
 import numpy as np
@@ -33,3 +34,36 @@ def test_fortran_sod_driver(gpu_ready, tmp_path, path):
     assert np.max(np.abs(prof[:, 2] - V[4])) <= 1e-12 * np.max(np.abs(V[4]))
     print(f"fortran host path {path}: hash {(lo, hi)}, oracle {mosegpu.state_hash(case.layout, Po)}")
     assert (lo, hi) == mosegpu.state_hash(case.layout, Po)
+
+
+def write_mechanism(path, m):
+    with open(path, "wb") as f:
+        np.array([m.ns, m.n_arrh, m.n_troe, m.n_lind, 0, m.cp.shape[1] - 1], dtype=np.int32).tofile(f)
+        for a in (m.wm, m.cp, m.h, m.ni1, m.ni2, m.eps, m.kf, m.kb, m.troe, m.lind):
+            np.ascontiguousarray(a, dtype=np.float64).tofile(f)
+
+
+def test_fortran_chemistry_driver(gpu_ready, tmp_path):
+    """mose_chem_run (MOSE_GPU_Chemistry_Newstate through the ISO_C_BINDING module) on 96 hydrogen-air cells: the
+    oracle's RODAS state and hr, bit for bit."""
+    from mosegpu import chem
+    from oracle_bind import OracleChem
+    drv = mosegpu.LIB_PATH.parent / "mose_chem_run"
+    assert drv.exists(), "build the Fortran host first (make -C mose-gpu_amd)"
+    mech = chem.h2_air_mechanism()
+    st = chem.h2_air_cells(96, seed=41)
+    write_mechanism(tmp_path / "mech.bin", mech)
+    with open(tmp_path / "cells.bin", "wb") as f:
+        np.array([st.shape[1]], dtype=np.int64).tofile(f)
+        np.ascontiguousarray(st).tofile(f)          # [ns+1][ncell] = Fortran state(ncell, ns+1)
+    r = subprocess.run([str(drv), "mech.bin", "cells.bin", "out.bin", "1e-6", "1", "1e-8", "1e-12"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = np.fromfile(tmp_path / "out.bin", dtype=np.float64)
+    n = st.shape[1]
+    dev, hr = raw[:10 * n].reshape(10, n), raw[10 * n:]
+    ref, hr_o, idid_o, _ = OracleChem(mech, nthreads=16).advance(st, 1e-6)
+    assert np.all(idid_o == 1)
+    print(f"fortran chemistry driver: {r.stdout.strip()}; max |dev - oracle| {np.abs(dev - ref).max():.3e}")
+    assert np.array_equal(dev, ref)
+    assert np.array_equal(hr, hr_o)
