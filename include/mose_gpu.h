@@ -165,12 +165,20 @@ int mose_state_download(mose_ctx* ctx, int case_id, double* P);
  * chemistry): per step the global dt (Compute_dt, Mod_dt.f90:15-243), PO <- P, then n_RK stages of
  * ghosts + halo (Fill_Ghost_Cell, Lib_Extrapolation.f90:1370-1524), fluxes (Fluxes, Mod_Fluxes.f90:10;
  * BC_Fluxes, Mod_BC.f90:646) and the conservative update (Newstate_Conservative_blk, Lib_Newstate.f90:257).
- * The call is asynchronous on the context's stream except for the report. */
+ * The steps run on the case's own stream; the call returns when they are done, with the report. */
 int mose_case_step(mose_ctx* ctx, int case_id, int nsteps, mose_report* report);
 
-/* The same on several cases at once: one launch per kernel over all the blocks of all the cases
- * (the block pack of MOSE_Pack_m.f90:70-251 as a persistent work queue); each case keeps its own dt. */
+/* The same on several cases at once: the steps of every case are queued on its own stream before any is waited
+ * on, so the cases' kernels overlap on the device; each case keeps its own dt.  (Within one case, all of its
+ * blocks already share one launch per stage, the block pack of MOSE_Pack_m.f90:70-251.) */
 int mose_cases_step(mose_ctx* ctx, const int* case_ids, int ncases, int nsteps, mose_report* reports);
+
+/* Asynchronous stepping for a serving loop.  mose_case_launch queues nsteps steps of the case on its stream and
+ * returns at once; mose_case_poll returns at once too, with done = 0 while they run and done = 1 (and the report of
+ * those steps, as mose_case_step gives it) when they have finished.  One launch per case may be in flight; poll it
+ * to done before any other call on the case.  No reference counterpart (the serving layer). */
+int mose_case_launch(mose_ctx* ctx, int case_id, int nsteps);
+int mose_case_poll(mose_ctx* ctx, int case_id, int* done, mose_report* report);
 
 /* Residual of the current state (ghosts, fluxes and boundary fluxes of one stage, no update): R[s*NC+row]
  * on the interior rows.  For checks against Fluxes + BC_Fluxes. */

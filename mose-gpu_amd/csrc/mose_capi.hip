@@ -72,6 +72,7 @@ struct Case {
   unsigned long long* stage_fb = nullptr;
   void* stage_host = nullptr;
   int pending = 0;
+  bool launched = false;          // a mose_case_launch awaits its mose_case_poll
   // one fused RK step captured as a hipGraph (kernels only), replayed per step
   bool use_graph = true;
   hipGraph_t graph = nullptr;
@@ -894,6 +895,35 @@ extern "C" int mose_case_step(mose_ctx* ctx, int id, int nsteps, mose_report* re
   HIPCHK(ctx, hipSetDevice(ctx->device));
   int rc = case_enqueue(ctx, c, nsteps);
   if (rc != MOSE_OK) return rc;
+  return case_collect(ctx, c, rep);
+}
+
+// Asynchronous stepping for a serving loop: queue now, collect when the case's event has fired.
+extern "C" int mose_case_launch(mose_ctx* ctx, int id, int nsteps) {
+  Case* c = get_case(ctx, id);
+  if (!c || nsteps < 0) return MOSE_ERR_ARG;
+  if (c->pending > 0) return fail(ctx, MOSE_ERR_ARG, "mose_case_launch: a launch of this case is still in flight");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  int rc = case_enqueue(ctx, c, nsteps);
+  if (rc != MOSE_OK) return rc;
+  HIPCHK(ctx, hipEventRecord(c->ev[0], c->stream));
+  c->launched = true;
+  return MOSE_OK;
+}
+
+extern "C" int mose_case_poll(mose_ctx* ctx, int id, int* done, mose_report* rep) {
+  Case* c = get_case(ctx, id);
+  if (!c || !done) return MOSE_ERR_ARG;
+  if (!c->launched) return fail(ctx, MOSE_ERR_ARG, "mose_case_poll: no launch of this case is in flight");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  hipError_t q = hipEventQuery(c->ev[0]);
+  if (q == hipErrorNotReady) {
+    *done = 0;
+    return MOSE_OK;
+  }
+  if (q != hipSuccess) return fail(ctx, MOSE_ERR_HIP, std::string("mose_case_poll: ") + hipGetErrorString(q));
+  *done = 1;
+  c->launched = false;
   return case_collect(ctx, c, rep);
 }
 

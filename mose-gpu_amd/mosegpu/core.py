@@ -143,6 +143,8 @@ def lib() -> C.CDLL:
     L.mose_state_download.argtypes = [vp, ip, dp]
     L.mose_case_step.argtypes = [vp, ip, ip, C.POINTER(Report)]
     L.mose_cases_step.argtypes = [vp, C.POINTER(C.c_int), ip, ip, C.POINTER(Report)]
+    L.mose_case_launch.argtypes = [vp, ip, ip]
+    L.mose_case_poll.argtypes = [vp, ip, C.POINTER(C.c_int), C.POINTER(Report)]
     L.mose_case_residual.argtypes = [vp, ip, dp]
     L.mose_case_dt.argtypes = [vp, ip, dp]
     L.mose_case_residual_norm.argtypes = [vp, ip, dp]
@@ -158,7 +160,8 @@ def lib() -> C.CDLL:
     L.mose_chem_advance.argtypes = [vp, ip, C.c_double, C.POINTER(OdeOpts), C.POINTER(ChemReport)]
     for name in ("mose_ctx_create", "mose_ctx_destroy", "mose_metrics_build", "mose_records_tangent", "mose_case_create",
                  "mose_case_destroy", "mose_case_bytes", "mose_state_upload", "mose_state_download",
-                 "mose_case_step", "mose_cases_step", "mose_case_residual", "mose_case_dt",
+                 "mose_case_step", "mose_cases_step", "mose_case_launch", "mose_case_poll", "mose_case_residual",
+                 "mose_case_dt",
                  "mose_case_residual_norm", "mose_case_state_hash", "mose_case_timing", "mose_device_count",
                  "mose_case_tune", "mose_bandwidth_probe", "mose_ctx_info", "mose_chem_create", "mose_chem_destroy",
                  "mose_chem_upload", "mose_chem_download", "mose_chem_advance"):
@@ -338,6 +341,17 @@ class DeviceCase:
         rep = Report()
         self.ctx.check(lib().mose_case_step(self.ctx.h, self.id, nsteps, C.byref(rep)), "mose_case_step")
         return rep
+
+    def launch(self, nsteps: int):
+        """Queue nsteps steps and return at once (mose_case_launch); poll() until it reports them done."""
+        self.ctx.check(lib().mose_case_launch(self.ctx.h, self.id, nsteps), "mose_case_launch")
+
+    def poll(self) -> Report | None:
+        """The report of the launched steps once they have finished, None while they run (mose_case_poll)."""
+        done = C.c_int(0)
+        rep = Report()
+        self.ctx.check(lib().mose_case_poll(self.ctx.h, self.id, C.byref(done), C.byref(rep)), "mose_case_poll")
+        return rep if done.value else None
 
     def residual(self) -> np.ndarray:
         R = np.zeros((self.case.nprim, self.case.layout.NC))

@@ -5,14 +5,16 @@ Requests arrive as a seeded Poisson stream. Each request is an independent singl
 - a 3D viscous Taylor-Green vortex at n^3, n log-uniform in [48, 192].
 Each runs 50..500 steps.
 
-The server works in rounds:
-1. It admits every arrived request whose device footprint (mose_case_bytes) fits the HBM budget left, oldest first.
-2. It steps all resident cases together, a quantum of steps at a time (mose_cases_step: every case on its own
-   stream, so their kernels overlap on the device).
+The server loop:
+1. It admits every prepared request whose device footprint (mose_case_bytes) fits the HBM budget left, oldest
+   first.
+2. It keeps every resident case stepping on its own stream, a quantum of steps at a time: mose_case_launch queues
+   the next quantum as soon as mose_case_poll has collected the last one, so the cases' kernels overlap on the
+   device and no case waits for a slower one.
 3. It retires finished cases. A retired case gives its state hash, and its memory goes back to the budget.
 
-Requests that arrive while others run are admitted at the next round: that is continuous batching, not a queue
-in front of a single case.
+Requests that arrive while others run are admitted as soon as they are prepared and fit: that is continuous
+batching, not a queue in front of a single case.
 
 Host preparation of arrived requests runs on a builder thread, overlapped with the device work of the resident
 cases.
@@ -82,12 +84,23 @@ class ServeStats:
     cases_per_s: float
     p50: float
     p99: float
-    rounds: int
+    launches: int           # quanta launched
     max_resident: int
     cell_updates: float
     max_bytes: int = 0
     budget: int = 0
     meta: dict = field(default_factory=dict)
+
+
+def serve_tile(case, path: int):
+    """The stage tile of a served case, by rule instead of the autotuner (which would time candidate tiles on every
+    admitted case while the others wait): 8 x 8 x 4 for 3D blocks, 32 x 8 x 1 for 2D ones."""
+    if path == PATH_PLAIN:
+        return (0, 0, 0)
+    n1, n2, n3 = case.layout.dims[0]
+    if n3 == 1:
+        return (32, 8, 1) if n2 > 1 else (256, 1, 1)
+    return (8, 8, 4)
 
 
 def serve(reqs: list[Request], budget_bytes: int | None = None, quantum: int = 10, path: int = PATH_FUSED,
@@ -96,8 +109,9 @@ def serve(reqs: list[Request], budget_bytes: int | None = None, quantum: int = 1
     otherwise all are visible at once (offline throughput).
 
     A builder thread prepares arrived requests on the host (mesh, metrics, boundary records, initial field), at most
-    `lookahead` ahead of admission, while the main thread steps the resident cases: the device calls release the
-    interpreter lock, so host preparation overlaps device work."""
+    `lookahead` ahead of admission.  The main thread keeps every resident case busy: each has its own stream, gets its
+    next quantum of steps (mose_case_launch) as soon as its last one is collected (mose_case_poll), and retires when
+    its steps are done, so no case waits for a slower one and host preparation overlaps device work."""
     ctx_cache = {}
 
     def ctx_for(case):
@@ -132,9 +146,9 @@ def serve(reqs: list[Request], budget_bytes: int | None = None, quantum: int = 1
     th = threading.Thread(target=builder, name="mose-serve-builder", daemon=True)
     th.start()
     waiting = []        # built, not yet admitted: (req, case, bytes), arrival order
-    resident = []       # [req, case, dc, remaining, bytes, ctx]
+    resident = []       # {"r", "case", "dc", "rem", "nb", "fly"}: fly = steps in flight
     used = 0
-    rounds = 0
+    launches = 0
     max_res = 0
     max_used = 0
     cu = 0.0
@@ -161,9 +175,9 @@ def serve(reqs: list[Request], budget_bytes: int | None = None, quantum: int = 1
             ctx = ctx_for(case)
             dc = DeviceCase(ctx, case)
             dc.upload(case.P)
-            dc.tune(path, (0, 0, 0))
+            dc.tune(path, serve_tile(case, path))
             r.admitted = time.perf_counter() - t0
-            resident.append([r, case, dc, r.steps, nb, ctx])
+            resident.append({"r": r, "case": case, "dc": dc, "rem": r.steps, "nb": nb, "fly": 0})
             used += nb
         if not resident:
             if stream_done and not waiting:
@@ -171,45 +185,42 @@ def serve(reqs: list[Request], budget_bytes: int | None = None, quantum: int = 1
             continue
         max_res = max(max_res, len(resident))
         max_used = max(max_used, used)
-        # one round: every resident case advances min(quantum, remaining) steps; cases of the same context are
-        # queued together (each on its own stream) and collected together
-        by_ctx = {}
-        for item in resident:
-            by_ctx.setdefault(id(item[5]), []).append(item)
-        for items in by_ctx.values():
-            ctx = items[0][5]
-            groups = {}
-            for it in items:
-                groups.setdefault(min(quantum, it[3]), []).append(it)
-            for k, its in groups.items():
-                ids = np.array([it[2].id for it in its], dtype=np.int32)
-                reps = (Report * len(its))()
-                ctx.check(lib().mose_cases_step(ctx.h, ids.ctypes.data_as(C.POINTER(C.c_int)), len(its), k, reps),
-                          "mose_cases_step")
-                for it in its:
-                    it[3] -= k
-                    cu += k * it[1].interior_cells()
-        rounds += 1
+        # every resident case with steps left and nothing in flight gets its next quantum on its own stream
+        for it in resident:
+            if it["fly"] == 0 and it["rem"] > 0:
+                k = min(quantum, it["rem"])
+                it["dc"].launch(k)
+                it["fly"] = k
+                launches += 1
+        # collect the quanta that have finished; retire the cases that are done
+        progressed = False
         keep = []
         for it in resident:
-            if it[3] <= 0:
-                r = it[0]
-                r.hash = it[2].state_hash()
+            if it["fly"] and it["dc"].poll() is not None:
+                it["rem"] -= it["fly"]
+                cu += it["fly"] * it["case"].interior_cells()
+                it["fly"] = 0
+                progressed = True
+            if it["rem"] <= 0 and it["fly"] == 0:
+                r = it["r"]
+                r.hash = it["dc"].state_hash()
                 r.done = time.perf_counter() - t0
-                it[2].destroy()
-                used -= it[4]
+                it["dc"].destroy()
+                used -= it["nb"]
                 if log:
                     log(f"served {r.kind} n={r.n} steps={r.steps}: {r.done - r.arrival:.3f} s")
             else:
                 keep.append(it)
         resident = keep
+        if not progressed:
+            time.sleep(2e-4)
     th.join()
     wall = time.perf_counter() - t0
     for c in ctx_cache.values():
         c.close()
     lat = np.array([r.done - r.arrival for r in reqs])
     return ServeStats(reqs, wall, len(reqs) / wall, float(np.percentile(lat, 50)), float(np.percentile(lat, 99)),
-                      rounds, max_res, cu, max_used, budget_bytes)
+                      launches, max_res, cu, max_used, budget_bytes)
 
 
 def run_alone(r: Request, path: int = PATH_FUSED):

@@ -119,7 +119,7 @@ def run_fused(case, nsteps, tile):
     return dc.download(), rep, chosen
 
 
-@pytest.mark.parametrize("tile", [(32, 4, 2), (16, 4, 4), (64, 2, 2), (0, 0, 0)])
+@pytest.mark.parametrize("tile", [(32, 4, 2), (16, 4, 4), (64, 2, 2), (0, 0, 0), (8, 8, 8), (32, 4, 4)])
 def test_fused_taylor_green_parity(gpu_ready, tile):
     case = cases.taylor_green(20, 6, dims=(20, 18, 21))
     case.meta["gamma"] = 1004.5 / (1004.5 - 8314.46261815324 / 28.970418)
@@ -151,7 +151,8 @@ def test_fused_equals_plain_at_bench_size(gpu_ready):
     case = cases.taylor_green(256, 2)
     ctx = Context(case.tables)
     out = {}
-    for name, path, tile in (("plain", mosegpu.PATH_PLAIN, (0, 0, 0)), ("fused", mosegpu.PATH_FUSED, (32, 4, 2))):
+    for name, path, tile in (("plain", mosegpu.PATH_PLAIN, (0, 0, 0)), ("fused", mosegpu.PATH_FUSED, (32, 4, 2)),
+                             ("fused512", mosegpu.PATH_FUSED, (8, 8, 8))):
         dc = DeviceCase(ctx, case)
         dc.upload(case.P)
         dc.tune(path, tile)
@@ -161,6 +162,7 @@ def test_fused_equals_plain_at_bench_size(gpu_ready):
     a = case.layout.interior(out["plain"], 0)
     b = case.layout.interior(out["fused"], 0)
     assert np.array_equal(a, b)
+    assert np.array_equal(a, case.layout.interior(out["fused512"], 0))
     vol = case.layout.interior(case.geom.vol, 0)
     m0 = np.sum(case.layout.interior(case.P, 0)[0] * vol)
     m1 = np.sum(b[0] * vol)

@@ -35,3 +35,32 @@ def test_budget_limits_residency(gpu_ready):
     assert all(r.done > 0 for r in reqs)
     assert st.max_bytes <= budget                  # admission never exceeds the HBM budget
     assert st.max_resident < len(reqs)             # and the budget did hold requests back
+
+
+def test_launch_poll_equals_step(gpu_ready):
+    """mose_case_launch + mose_case_poll give the state and report of mose_case_step; polling without a launch in
+    flight and a second launch before the poll are errors."""
+    import time
+
+    from mosegpu import Context, DeviceCase, MoseError, cases
+    case = cases.taylor_green(24, 4)
+    ctx = Context(case.tables)
+    a, b = DeviceCase(ctx, case), DeviceCase(ctx, case)
+    a.upload(case.P)
+    b.upload(case.P)
+    ra = a.step(4)
+    b.launch(4)
+    with pytest.raises(MoseError):
+        b.launch(1)
+    rb = None
+    t0 = time.perf_counter()
+    while rb is None and time.perf_counter() - t0 < 60:
+        rb = b.poll()
+    assert rb is not None
+    assert (rb.steps, rb.time, rb.dt) == (ra.steps, ra.time, ra.dt)
+    assert a.state_hash() == b.state_hash()
+    with pytest.raises(MoseError):
+        b.poll()
+    a.destroy()
+    b.destroy()
+    ctx.close()

@@ -94,3 +94,36 @@ def test_farfield_wall_residual(gpu_ready, rs):
     Rg = dc.residual()
     Ro = OracleCase(case).residual(case.P.copy(), 0.0)
     assert compare(case, Rg, Ro, f"far field + wall residual ({rs})") <= TOL
+
+
+@pytest.mark.parametrize("rs", ["HLLC", "LMRoe"])
+def test_walls_farfield_steps_on_512_cell_tiles(gpu_ready, rs):
+    """Five steps of a viscous box with far field on five faces and an isothermal wall on one, without RANS (a
+    512-cell tile's LDS holds five primitives, not six), on the 8 x 8 x 8 tiles of 512-thread workgroups."""
+    rho, p, M = 1.15, 9.0e4, 0.3
+    T = p / (rho * cases.R_AIR)
+    gam = cases.CP_AIR / (cases.CP_AIR - cases.R_AIR)
+    U = M * np.sqrt(gam * cases.R_AIR * T)
+    T0 = T * (1.0 + 0.5 * (gam - 1.0) * M * M)
+    ff = np.zeros(1, dtype=FARFIELD_DTYPE)
+    ff["t0"] = T0; ff["p0"] = p * (T0 / T) ** (gam / (gam - 1.0)); ff["pamb"] = p; ff["rel_fac"] = 1.0
+    ff["ci"][0, 0] = 1.0
+    faces = {1: BC_FARFIELD, 2: BC_FARFIELD, 3: BC_WALL_ISOTHERMAL, 4: BC_FARFIELD, 5: BC_FARFIELD, 6: BC_FARFIELD}
+    case = cases.box_case((19, 13, 11), faces, extent=(1.0, 0.6, 0.5), farfield=ff, riemann=rs, T_wall=290.0,
+                          limiter="vanalbada", cfl=0.4)
+    rng = np.random.Generator(np.random.PCG64(23))
+    V = case.layout.interior(case.P, 0)
+    sh = V.shape[1:]
+    V[0] = rho * (1 + 0.02 * rng.uniform(-1, 1, sh))
+    V[1] = U * (1 + 0.1 * rng.uniform(-1, 1, sh))
+    V[2] = 0.05 * U * rng.uniform(-1, 1, sh)
+    V[3] = 0.05 * U * rng.uniform(-1, 1, sh)
+    V[4] = p * (1 + 0.02 * rng.uniform(-1, 1, sh))
+    cases.fill_ghosts_nearest(case.layout, case.P)
+    Pg, rep, chosen = run_device(case, 5, mosegpu.PATH_FUSED, (8, 8, 8))
+    assert tuple(chosen) == (8, 8, 8)
+    Po = case.P.copy()
+    dts = OracleCase(case).step(Po, 5)
+    assert abs(rep.time - dts.sum()) <= 1e-13 * dts.sum()
+    assert compare(case, Pg, Po, f"walls + far field, {rs}, 512-cell tiles x 5") <= TOL
+
