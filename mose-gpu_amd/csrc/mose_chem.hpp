@@ -4,7 +4,7 @@
 //
 //   net_rate    general()                 lib/FLINT/src/lib/Lib_Chemistry_wdot.f90:163-288 (+ falloff :12-80)
 //   team_rhs    rhs_native                lib/FLINT/src/lib/Lib_Chemistry_rhs.f90:29-81
-//   team_dec/sol DEC / SOL                lib/FLINT/src/lib/Lib_Radau5_dev.f90:950-1035
+//   team_dec_lds / team_sol_lds  DEC / SOL  lib/FLINT/src/lib/Lib_Radau5_dev.f90:950-1035
 //   k_chem      ROSCOR (METH = 1, IJAC = 0, autonomous) hairerRodas.f:540-880 inside Chemistry_Newstate_blk
 //               (src/lib/physics/Lib_Chemistry.f90:33-76)
 //
@@ -289,51 +289,66 @@ __device__ __forceinline__ double team_rhs(const ChemDev& M, const RxnLds& X, co
   return r < NS ? om : 0.0;
 }
 
-// DEC (Moler, ACM 423; Lib_Radau5_dev.f90:950-997) with row r of the matrix on lane r: e[j] = a(r, j).
-// Returns ier (0: regular); the pivot row of column k is ip(k) = (ipk >> 4k) & 15, the same on every lane of the team
-// (ip(N), the sign of the determinant, is not kept: SOL does not read it).
+// DEC (Moler, ACM 423; Lib_Radau5_dev.f90:950-997) with row r of the matrix on lane r: e[j] = a(r, j).  Returns
+// ier (0: regular); the pivot row of column k is ip(k) = (ipk >> 4k) & 15, the same on every lane of the team
+// (ip(N), the sign of the determinant, is not kept: SOL does not read it).  The pivot column and the two exchanged
+// rows pass through the team's LDS area (one 64-bit LDS access per value; a shuffle of a double costs two
+// bpermutes): L.q holds a column, L.ei the pivot row m and L.rc row k (none of them is live across a DEC).
 template <int N>
-__device__ __forceinline__ int team_dec(double (&e)[N], unsigned long long& ipk, int base, int r) {
+__device__ __forceinline__ int team_dec_lds(double (&e)[N], unsigned long long& ipk, TeamLds& L, int r) {
   static_assert(N <= 16, "4-bit pivot indices");
   ipk = 0ull;
 #pragma unroll
   for (int k = 0; k < N - 1; ++k) {
+    L.q[r] = e[k];
+    wsync();
     int m = k;
-    double am = fabs(tsh(e[k], base + k));
+    double am = fabs(L.q[k]);
 #pragma unroll
     for (int i = k + 1; i < N; ++i) {
-      double v = fabs(tsh(e[k], base + i));
+      double v = fabs(L.q[i]);
       if (v > am) { m = i; am = v; }
     }
     ipk |= (unsigned long long)m << (4 * k);
-    double t = tsh(e[k], base + m);
-    if (m != k) {
-      double akk = tsh(e[k], base + k);
-      e[k] = (r == k) ? t : ((r == m) ? akk : e[k]);
+    double t = L.q[m];
+    const double akk = L.q[k];
+    if (r == m) {
+#pragma unroll
+      for (int j = k + 1; j < N; ++j) L.ei[j] = e[j];
     }
+    if (r == k) {
+#pragma unroll
+      for (int j = k + 1; j < N; ++j) L.rc[j] = e[j];
+    }
+    wsync();
+    if (m != k) e[k] = (r == k) ? t : ((r == m) ? akk : e[k]);
     if (t == 0.0) return k + 1;
     t = 1.0 / t;
     if (r > k) e[k] = -e[k] * t;
 #pragma unroll
     for (int j = k + 1; j < N; ++j) {
-      double amj = tsh(e[j], base + m), akj = tsh(e[j], base + k);
+      const double amj = L.ei[j], akj = L.rc[j];
       e[j] = (r == k) ? amj : ((r == m) ? akj : e[j]);
-      double tj = amj;
-      if (tj != 0.0 && r > k) e[j] = e[j] + e[k] * tj;
+      if (amj != 0.0 && r > k) e[j] = e[j] + e[k] * amj;
     }
   }
-  if (tsh(e[N - 1], base + N - 1) == 0.0) return N;
+  L.q[r] = e[N - 1];
+  wsync();
+  if (L.q[N - 1] == 0.0) return N;
   return 0;
 }
 
-// SOL (Lib_Radau5_dev.f90:1003-1035) on the team: b = this lane's component
+// SOL (Lib_Radau5_dev.f90:1003-1035) on the team: b = this lane's component; the exchanges go through L.q
 template <int N>
-__device__ __forceinline__ double team_sol(const double (&e)[N], unsigned long long ipk, double b, int base, int r) {
+__device__ __forceinline__ double team_sol_lds(const double (&e)[N], unsigned long long ipk, double b, TeamLds& L,
+                                               int r) {
 #pragma unroll
   for (int k = 0; k < N - 1; ++k) {
     const int m = (int)((ipk >> (4 * k)) & 15ull);
-    double bm = tsh(b, base + m), bk = tsh(b, base + k);
-    double t = bm;
+    L.q[r] = b;
+    wsync();
+    const double bm = L.q[m], bk = L.q[k];
+    const double t = bm;
     b = (r == m) ? bk : b;
     b = (r == k) ? t : b;
     if (r > k) b = b + e[k] * t;
@@ -342,7 +357,9 @@ __device__ __forceinline__ double team_sol(const double (&e)[N], unsigned long l
   for (int kb = 1; kb <= N - 1; ++kb) {
     const int k = N - kb;
     if (r == k) b = b / e[k];
-    double t = -tsh(b, base + k);
+    L.q[r] = b;
+    wsync();
+    const double t = -L.q[k];
     if (r < k) b = b + e[k] * t;
   }
   if (r == 0) b = b / e[0];
@@ -544,7 +561,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
       else if (k == 5) b = f + ((R::C52 / h) * ak2 + (R::C54 / h) * ak4 + (R::C51 / h) * ak1 + (R::C53 / h) * ak3);
       else b = f + ((R::C61 / h) * ak1 + (R::C62 / h) * ak2 + (R::C65 / h) * ak5 + (R::C64 / h) * ak4 +
                     (R::C63 / h) * ak3);
-      const double ak = team_sol<NZ>(e, ipk, b, base, r);
+      const double ak = team_sol_lds<NZ>(e, ipk, b, L, r);
       if (k == 2) { ak2 = ak; yn = y + R::A31 * ak1 + R::A32 * ak2; }
       else if (k == 3) { ak3 = ak; yn = y + R::A41 * ak1 + R::A42 * ak2 + R::A43 * ak3; }
       else if (k == 4) { ak4 = ak; yn = y + R::A51 * ak1 + R::A52 * ak2 + R::A53 * ak3 + R::A54 * ak4; }
@@ -600,7 +617,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         const double fac = 1.0 / (h * R::GAMMA);
 #pragma unroll
         for (int j = 0; j < NZ; ++j) e[j] = (j == r) ? -fjr[j] + fac : -fjr[j];
-        if (team_dec<NZ>(e, ipk, base, r) == 0) break;
+        if (team_dec_lds<NZ>(e, ipk, L, r) == 0) break;
         nsing++;
         if (nsing >= 5) { idid = -4; fin = true; break; }
         h = h * 0.5;
@@ -608,7 +625,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         last = false;
       }
       if (!fin) {
-        ak1 = team_sol<NZ>(e, ipk, dy1, base, r);
+        ak1 = team_sol_lds<NZ>(e, ipk, dy1, L, r);
         yn = y + R::A21 * ak1;
         zin = yn;
         ev = EV_S;
