@@ -647,18 +647,26 @@ static int set_tiles(mose_ctx* ctx, Case& c, int TI, int TJ, int TK) {
   return ensure_rpart(ctx, c, c.ntiles);
 }
 
-template <int NS, int NR, int BS>
-static void launch_stage_bs(Case& c, const CaseDev& D, hipStream_t s, const StageArgs& A, unsigned grid, size_t lds) {
+template <int NS, int NR, int BS, int SH>
+static void launch_stage_sh(Case& c, const CaseDev& D, hipStream_t s, const StageArgs& A, unsigned grid, size_t lds) {
   const int rs = c.sc.riemann;
   if (c.sc.viscous) {
-    if (rs == MOSE_RS_LMROE) k_stage<NS, NR, true, MOSE_RS_LMROE, BS><<<grid, BS, lds, s>>>(D, A);
-    else if (rs == MOSE_RS_AUSMP) k_stage<NS, NR, true, MOSE_RS_AUSMP, BS><<<grid, BS, lds, s>>>(D, A);
-    else k_stage<NS, NR, true, MOSE_RS_HLLC, BS><<<grid, BS, lds, s>>>(D, A);
+    if (rs == MOSE_RS_LMROE) k_stage<NS, NR, true, MOSE_RS_LMROE, BS, SH><<<grid, BS, lds, s>>>(D, A);
+    else if (rs == MOSE_RS_AUSMP) k_stage<NS, NR, true, MOSE_RS_AUSMP, BS, SH><<<grid, BS, lds, s>>>(D, A);
+    else k_stage<NS, NR, true, MOSE_RS_HLLC, BS, SH><<<grid, BS, lds, s>>>(D, A);
   } else {
-    if (rs == MOSE_RS_LMROE) k_stage<NS, NR, false, MOSE_RS_LMROE, BS><<<grid, BS, lds, s>>>(D, A);
-    else if (rs == MOSE_RS_AUSMP) k_stage<NS, NR, false, MOSE_RS_AUSMP, BS><<<grid, BS, lds, s>>>(D, A);
-    else k_stage<NS, NR, false, MOSE_RS_HLLC, BS><<<grid, BS, lds, s>>>(D, A);
+    if (rs == MOSE_RS_LMROE) k_stage<NS, NR, false, MOSE_RS_LMROE, BS, SH><<<grid, BS, lds, s>>>(D, A);
+    else if (rs == MOSE_RS_AUSMP) k_stage<NS, NR, false, MOSE_RS_AUSMP, BS, SH><<<grid, BS, lds, s>>>(D, A);
+    else k_stage<NS, NR, false, MOSE_RS_HLLC, BS, SH><<<grid, BS, lds, s>>>(D, A);
   }
+}
+
+// the two preferred tile shapes run kernels with the shape fixed at compile time; the others read it at run time
+template <int NS, int NR, int BS>
+static void launch_stage_bs(Case& c, const CaseDev& D, hipStream_t s, const StageArgs& A, unsigned grid, size_t lds) {
+  if (BS == 256 && c.TI == 8 && c.TJ == 8 && c.TK == 4) launch_stage_sh<NS, NR, BS, 1>(c, D, s, A, grid, lds);
+  else if (BS == 512 && c.TI == 8 && c.TJ == 8 && c.TK == 8) launch_stage_sh<NS, NR, BS, 2>(c, D, s, A, grid, lds);
+  else launch_stage_sh<NS, NR, BS, 0>(c, D, s, A, grid, lds);
 }
 
 template <int NS, int NR>

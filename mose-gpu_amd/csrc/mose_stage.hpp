@@ -38,10 +38,37 @@ struct StageArgs {
   double* rpart;      // last stage: per-tile sums of the squared step changes [ntiles][nprim - NS + 1]
 };
 
+// The index geometry of a tile.  STI/STJ/STK > 0 fix the shape at compile time (the decode of slot indices then
+// divides by constants: multiply and shift instead of the integer division by a run-time value), 0 takes it from the
+// launch arguments.  r0 repeats the host's sizing of LDS region 0 (stage_r0).
+struct TileGeo {
+  int TI, TJ, TK, HI, HJ, nhs, NI, NJ, nslot, CI, CJ, ncache, r0;
+};
+template <int NP, bool VISC, int STI, int STJ, int STK>
+__device__ __forceinline__ TileGeo tile_geo(const StageArgs& A) {
+  TileGeo g;
+  g.TI = STI ? STI : A.TI;
+  g.TJ = STJ ? STJ : A.TJ;
+  g.TK = STK ? STK : A.TK;
+  g.HI = (g.TI + 2) * g.TJ * g.TK;
+  g.HJ = g.TI * (g.TJ + 2) * g.TK;
+  g.nhs = g.HI + g.HJ + g.TI * g.TJ * (g.TK + 2);
+  g.NI = (g.TI + 1) * g.TJ * g.TK;
+  g.NJ = g.TI * (g.TJ + 1) * g.TK;
+  g.nslot = g.NI + g.NJ + g.TI * g.TJ * (g.TK + 1);
+  g.CI = g.TI + 2;
+  g.CJ = g.TJ + 2;
+  g.ncache = g.CI * g.CJ * (g.TK + 2);
+  int r0 = g.nhs * NP;
+  if (VISC) r0 = r0 > (3 + NP) * g.ncache ? r0 : (3 + NP) * g.ncache;
+  g.r0 = (r0 + 31) & ~31;
+  return g;
+}
+
 // One interior viscous face of direction D (diff_cuf_i/j/k + diff_physics.inc) with the cell values (rho, 1/rho, T)
 // and the primitives of its stencil taken from the tile's LDS cache; (i,j,k) = the face's left cell, (a,b,c) = its tile-local face coordinates.
 template <int NS, int NR, int D>
-__device__ __forceinline__ void visc_face(const CaseDev& C, const mose_blk& B, const Tile& tl, const StageArgs& A,
+__device__ __forceinline__ void visc_face(const CaseDev& C, const mose_blk& B, const Tile& tl, const TileGeo& G_,
                                           const double* cache, int i, int j, int k, int a, int b, int c, double* res) {
   using V = Vars<NS, NR>;
   constexpr int NP = V::nprim;
@@ -49,7 +76,7 @@ __device__ __forceinline__ void visc_face(const CaseDev& C, const mose_blk& B, c
   const double* P = C.Pin;
   const int64_t NC = C.NC;
   const mose_dtab& T = *C.tab;
-  const int CI = A.TI + 2, CJ = A.TJ + 2, NCa = A.ncache;
+  const int CI = G_.CI, CJ = G_.CJ, NCa = G_.ncache;
   const int64_t sa[3] = {1, (int64_t)B.pi, (int64_t)B.pi * B.pj};
   const int64_t st = sa[D], sb = sa[ib], sc = sa[ic];
   const int ca[3] = {1, CI, CI * CJ};
@@ -110,7 +137,7 @@ __device__ __forceinline__ void visc_face(const CaseDev& C, const mose_blk& B, c
 }
 
 // One tile of one RK stage (the phases above).  Every thread of the workgroup calls it with the same tile.
-template <int NS, int NR, bool VISC, int RS>
+template <int NS, int NR, bool VISC, int RS, int STI, int STJ, int STK>
 __device__ __forceinline__ void stage_tile(const CaseDev& C, const StageArgs& A, int tidx) {
   using V = Vars<NS, NR>;
   constexpr int NP = V::nprim;
@@ -121,17 +148,18 @@ __device__ __forceinline__ void stage_tile(const CaseDev& C, const StageArgs& A,
   const mose_dtab& T = *C.tab;
   const int64_t NC = C.NC;
   const double* P = C.Pin;
-  const int TI = A.TI, TJ = A.TJ, TK = A.TK;
-  const int NI = (TI + 1) * TJ * TK, NJ = TI * (TJ + 1) * TK;
-  const int NF = A.nslot;
+  const TileGeo G = tile_geo<NP, VISC, STI, STJ, STK>(A);
+  const int TI = G.TI, TJ = G.TJ, TK = G.TK;
+  const int NI = G.NI, NJ = G.NJ;
+  const int NF = G.nslot;
   double* Hs = sm;                       // region 0: [NP][nhs] half-slopes (phase A)
   double* cache = sm;                    // region 0: [3 + NP][ncache] rho, 1/rho, T, P (phases B-D)
-  double* Fb = sm + A.r0;                // region 1: [NP][NF] face fluxes
+  double* Fb = sm + G.r0;                // region 1: [NP][NF] face fluxes
   const int64_t si = 1, sj = B.pi, sk = (int64_t)B.pi * B.pj;
 
   // ---------------- phase A0: limited half-slopes of the tile's cells (+ one cell along each direction) --------
-  const int HI = (TI + 2) * TJ * TK, HJ = TI * (TJ + 2) * TK;
-  for (int q = tid; q < A.nhs; q += blockDim.x) {
+  const int HI = G.HI, HJ = G.HJ;
+  for (int q = tid; q < G.nhs; q += blockDim.x) {
     int d, a, b, c;
     if (q < HI) { d = 0; a = q % (TI + 2); int r = q / (TI + 2); b = r % TJ; c = r / TJ; }
     else if (q < HI + HJ) { int g = q - HI; d = 1; a = g % TI; int r = g / TI; b = r % (TJ + 2); c = r / (TJ + 2); }
@@ -147,7 +175,7 @@ __device__ __forceinline__ void stage_tile(const CaseDev& C, const StageArgs& A,
     double idm = 1.0 / (0.5 * (dm + d0)), idp = 1.0 / (0.5 * (d0 + dp)), hd = 0.5 * d0;
 #pragma unroll
     for (int s = 0; s < NP; ++s)
-      Hs[s * A.nhs + q] = half_slope(C.lim, P[s * NC + c0 - st], P[s * NC + c0], P[s * NC + c0 + st], idm, idp, hd);
+      Hs[s * G.nhs + q] = half_slope(C.lim, P[s * NC + c0 - st], P[s * NC + c0], P[s * NC + c0 + st], idm, idp, hd);
   }
   __syncthreads();
 
@@ -171,8 +199,8 @@ __device__ __forceinline__ void stage_tile(const CaseDev& C, const StageArgs& A,
     double L[NP], R[NP], res[NP];
 #pragma unroll
     for (int s = 0; s < NP; ++s) {
-      L[s] = P[s * NC + c0] + Hs[s * A.nhs + hl];
-      R[s] = P[s * NC + c0 + st] - Hs[s * A.nhs + hr];
+      L[s] = P[s * NC + c0] + Hs[s * G.nhs + hl];
+      R[s] = P[s * NC + c0 + st] - Hs[s * G.nhs + hr];
     }
     const double* fm = d == 0 ? C.fm1 : (d == 1 ? C.fm2 : C.fm3);
     int64_t NFg = d == 0 ? C.NF1 : (d == 1 ? C.NF2 : C.NF3);
@@ -187,7 +215,7 @@ __device__ __forceinline__ void stage_tile(const CaseDev& C, const StageArgs& A,
   if (VISC) {
     // the half-slopes are dead: region 0 now holds (rho, 1/rho, T) of the tile and its one-cell halo
     const int CI = TI + 2, CJ = TJ + 2;
-    for (int q = tid; q < A.ncache; q += blockDim.x) {
+    for (int q = tid; q < G.ncache; q += blockDim.x) {
       int a = q % CI, r = q / CI, b = r % CJ, c = r / CJ;
       int i = tl.i0 + a - 1, j = tl.j0 + b - 1, k = tl.k0 + c - 1;
       if (i < 1 - GC || i > B.n1 + GC || j < 1 - GC || j > B.n2 + GC || k < 1 - GC || k > B.n3 + GC) continue;
@@ -197,10 +225,10 @@ __device__ __forceinline__ void stage_tile(const CaseDev& C, const StageArgs& A,
       for (int s = 0; s < NS; ++s) y[s] = P[s * NC + r0];
       double rho = rho_sum<NS>(y);
       cache[q] = rho;
-      cache[A.ncache + q] = 1.0 / rho;
-      cache[2 * A.ncache + q] = P[V::np * NC + r0] / (rho * f_Rtot<NS>(T, y));
+      cache[G.ncache + q] = 1.0 / rho;
+      cache[2 * G.ncache + q] = P[V::np * NC + r0] / (rho * f_Rtot<NS>(T, y));
 #pragma unroll
-      for (int s = 0; s < NP; ++s) cache[(3 + s) * A.ncache + q] = P[s * NC + r0];
+      for (int s = 0; s < NP; ++s) cache[(3 + s) * G.ncache + q] = P[s * NC + r0];
     }
   }
 
@@ -242,9 +270,9 @@ __device__ __forceinline__ void stage_tile(const CaseDev& C, const StageArgs& A,
       if (!ok) continue;
       // local cache coordinates of the left cell c1 (tile-local + 1)
       double res[NP];
-      if (d == 0) visc_face<NS, NR, 0>(C, B, tl, A, cache, i, j, k, a, b, c, res);
-      else if (d == 1) visc_face<NS, NR, 1>(C, B, tl, A, cache, i, j, k, a, b, c, res);
-      else visc_face<NS, NR, 2>(C, B, tl, A, cache, i, j, k, a, b, c, res);
+      if (d == 0) visc_face<NS, NR, 0>(C, B, tl, G, cache, i, j, k, a, b, c, res);
+      else if (d == 1) visc_face<NS, NR, 1>(C, B, tl, G, cache, i, j, k, a, b, c, res);
+      else visc_face<NS, NR, 2>(C, B, tl, G, cache, i, j, k, a, b, c, res);
 #pragma unroll
       for (int s = 0; s < NP; ++s) Fb[s * NF + f] = res[s];
     }
@@ -288,7 +316,7 @@ __device__ __forceinline__ void stage_tile(const CaseDev& C, const StageArgs& A,
     double Tseed;
     if (VISC) {
       const int CI = TI + 2, CJ = TJ + 2;
-      Tseed = cache[2 * A.ncache + (ca + 1) + CI * ((cb + 1) + CJ * (cc + 1))];
+      Tseed = cache[2 * G.ncache + (ca + 1) + CI * ((cb + 1) + CJ * (cc + 1))];
       if (C.guard && !(Tseed > 1.0 && Tseed < 9990.0)) {
         flag_bad(C, tl.b, ci, cj, ck, 1);
         if (Tseed != Tseed) Tseed = 300.0;
@@ -336,11 +364,13 @@ __device__ __forceinline__ int xcd_tile(int wg, int n) {
 // BS = the workgroup size = the cells of a tile (256 or 512).  Larger tiles recompute fewer halo half-slopes and
 // faces and re-read less of their neighbours; MI355X's 160 KB of LDS per workgroup holds an 8 x 8 x 8 tile's
 // half-slopes and face fluxes.
-template <int NS, int NR, bool VISC, int RS, int BS>
+// SH selects a tile shape fixed at compile time: 0 = from the launch arguments, 1 = 8 x 8 x 4, 2 = 8 x 8 x 8.
+template <int NS, int NR, bool VISC, int RS, int BS, int SH>
 __global__ void __launch_bounds__(BS) k_stage(CaseDev C, StageArgs A) {
   const int t = xcd_tile(blockIdx.x, A.ntiles);
   if (t < 0) return;
-  stage_tile<NS, NR, VISC, RS>(C, A, t);
+  constexpr int STI = SH ? 8 : 0, STJ = SH ? 8 : 0, STK = SH == 1 ? 4 : (SH == 2 ? 8 : 0);
+  stage_tile<NS, NR, VISC, RS, STI, STJ, STK>(C, A, t);
 }
 
 }  // namespace mose
