@@ -96,7 +96,7 @@ def test_farfield_wall_residual(gpu_ready, rs):
     assert compare(case, Rg, Ro, f"far field + wall residual ({rs})") <= TOL
 
 
-@pytest.mark.parametrize("rs", ["HLLC", "LMRoe"])
+@pytest.mark.parametrize("rs", ["HLLC", "LMRoe", "AUSM+"])
 def test_walls_farfield_steps_on_512_cell_tiles(gpu_ready, rs):
     """Five steps of a viscous box with far field on five faces and an isothermal wall on one, without RANS (a
     512-cell tile's LDS holds five primitives, not six), on the 8 x 8 x 8 tiles of 512-thread workgroups."""
