@@ -409,8 +409,8 @@ class DeviceChem:
 
     def upload(self, state: np.ndarray):
         st = np.ascontiguousarray(state, dtype=np.float64)
+        self.ctx.check(lib().mose_chem_upload(self.ctx.h, self.id, st.shape[1], dptr(st)), "mose_chem_upload")
         self.n = st.shape[1]
-        self.ctx.check(lib().mose_chem_upload(self.ctx.h, self.id, self.n, dptr(st)), "mose_chem_upload")
 
     def advance(self, dt: float, opts) -> ChemReport:
         """Raises MoseError(MOSE_ERR_ODE) when cells end with idid < 0; the report of the call, filled either way,

@@ -89,3 +89,32 @@ def run_device_allow_fail(state, dt, rtol=1e-8, atol=1e-12, method=2):
     dc.destroy()
     ctx.close()
     return out, rep
+
+
+@pytest.mark.parametrize("n", [1, 13, 61])
+def test_chem_ragged_batches(gpu_ready, n):
+    """Batches that fill neither a team row nor a workgroup (1, 13 and 61 cells; 6 teams per wavefront, 24 per
+    workgroup): the oracle's states bit for bit."""
+    st = chem.h2_air_cells(n, seed=100 + n)
+    (dev, hr, idid, nstep), rep = run_device(st, 1e-6)
+    ref, hr_o, idid_o, nstep_o = OracleChem(MECH, nthreads=4).advance(st, 1e-6)
+    assert rep.cells == n
+    assert np.array_equal(idid, idid_o) and np.array_equal(nstep, nstep_o)
+    assert np.array_equal(dev, ref)
+
+
+def test_chem_argument_errors(gpu_ready):
+    from mosegpu.core import MoseError
+    ctx = Context(chem_tables())
+    dc = DeviceChem(ctx, MECH, 8)
+    with pytest.raises(MoseError):
+        dc.advance(1e-6, chem.ode_opts())          # nothing uploaded
+    with pytest.raises(MoseError):
+        dc.upload(chem.h2_air_cells(9, seed=1))    # more cells than the object holds
+    dc.upload(chem.h2_air_cells(8, seed=1))
+    with pytest.raises(MoseError):
+        dc.advance(1e-6, chem.ode_opts(method=7))  # no such integrator
+    with pytest.raises(MoseError):
+        dc.advance(-1e-6, chem.ode_opts())         # dt must be positive
+    dc.destroy()
+    ctx.close()
