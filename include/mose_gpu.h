@@ -267,8 +267,8 @@ typedef struct {
   int64_t cells;
   int64_t failed;         /* cells with idid < 0 */
   int64_t steps_total, rejected_total, rhs_total;
-  int32_t steps_max, first_failed_idid;
-  int64_t first_failed_cell;
+  int32_t steps_max, first_failed_idid;   /* the idid of first_failed_cell (0 if none failed) */
+  int64_t first_failed_cell;   /* the lowest-index cell that ended with idid < 0, -1 if none */
   double ms;              /* device time of the integration kernel */
 } mose_chem_report;
 

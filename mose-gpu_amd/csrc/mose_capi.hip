@@ -900,6 +900,7 @@ static int case_collect(mose_ctx* ctx, Case* c, mose_report* rep) {
 extern "C" int mose_case_step(mose_ctx* ctx, int id, int nsteps, mose_report* rep) {
   Case* c = get_case(ctx, id);
   if (!c || nsteps < 0) return MOSE_ERR_ARG;
+  if (c->launched) return fail(ctx, MOSE_ERR_ARG, "mose_case_step: a mose_case_launch of this case awaits its poll");
   HIPCHK(ctx, hipSetDevice(ctx->device));
   int rc = case_enqueue(ctx, c, nsteps);
   if (rc != MOSE_OK) return rc;
@@ -939,8 +940,11 @@ extern "C" int mose_case_poll(mose_ctx* ctx, int id, int* done, mose_report* rep
 // concurrently on the device (small cases fill the CUs that one case alone leaves idle).
 extern "C" int mose_cases_step(mose_ctx* ctx, const int* ids, int n, int nsteps, mose_report* reps) {
   if (!ctx || !ids || n < 0 || nsteps < 0) return MOSE_ERR_ARG;
-  for (int q = 0; q < n; ++q)
-    if (!get_case(ctx, ids[q])) return fail(ctx, MOSE_ERR_ARG, "mose_cases_step: bad case id");
+  for (int q = 0; q < n; ++q) {
+    Case* c = get_case(ctx, ids[q]);
+    if (!c) return fail(ctx, MOSE_ERR_ARG, "mose_cases_step: bad case id");
+    if (c->launched) return fail(ctx, MOSE_ERR_ARG, "mose_cases_step: a mose_case_launch of a case awaits its poll");
+  }
   HIPCHK(ctx, hipSetDevice(ctx->device));
   int worst = MOSE_OK;
   std::vector<char> queued(n, 0);
@@ -1095,7 +1099,7 @@ struct ChemObj {
   double* hr = nullptr;
   int* idid = nullptr;
   int* nstep = nullptr;
-  unsigned long long* ctr = nullptr;   // [0] queue, [1..5] stats
+  unsigned long long* ctr = nullptr;   // [0] queue, [1..6] stats
   int64_t cap = 0, n = 0;
   int grid = 0;
   hipEvent_t ev[2] = {nullptr, nullptr};
@@ -1254,9 +1258,15 @@ extern "C" int mose_chem_advance(mose_ctx* ctx, int id, double dt, const mose_od
   else k_chem<9><<<grid, 256, 0, s>>>(o->M, A);
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipEventRecord(o->ev[1], s));
-  unsigned long long st[6];
+  unsigned long long st[7];
   HIPCHK(ctx, hipMemcpyAsync(st, o->ctr, sizeof st, hipMemcpyDeviceToHost, s));
   HIPCHK(ctx, hipStreamSynchronize(s));
+  int64_t first_failed = st[6] ? (int64_t)(~0ull - st[6]) : -1;
+  int32_t first_idid = 0;
+  if (first_failed >= 0) {
+    HIPCHK(ctx, hipMemcpyAsync(&first_idid, o->idid + first_failed, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+  }
   float ms = 0.f;
   HIPCHK(ctx, hipEventElapsedTime(&ms, o->ev[0], o->ev[1]));
   if (rep) {
@@ -1266,10 +1276,12 @@ extern "C" int mose_chem_advance(mose_ctx* ctx, int id, double dt, const mose_od
     rep->rhs_total = (int64_t)st[3];
     rep->failed = (int64_t)st[4];
     rep->steps_max = (int32_t)st[5];
-    rep->first_failed_idid = 0;
-    rep->first_failed_cell = -1;
+    rep->first_failed_idid = first_idid;
+    rep->first_failed_cell = first_failed;
     rep->ms = ms;
   }
-  if (st[4] > 0) return fail(ctx, MOSE_ERR_ODE, "mose_chem_advance: " + std::to_string(st[4]) + " cells failed (idid < 0)");
+  if (st[4] > 0)
+    return fail(ctx, MOSE_ERR_ODE, std::to_string(st[4]) + " cells failed (idid < 0), the first is cell " +
+                                       std::to_string(first_failed) + " with idid " + std::to_string(first_idid));
   return MOSE_OK;
 }

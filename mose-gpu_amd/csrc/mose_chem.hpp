@@ -31,7 +31,8 @@ struct ChemArgs {
   int* idid;                // [ncell]
   int* nstep;               // [ncell]
   unsigned long long* queue;   // [0] next cell
-  unsigned long long* stats;   // [0] steps, [1] rejected, [2] rhs calls, [3] failed, [4] max steps
+  unsigned long long* stats;   // [0] steps, [1] rejected, [2] rhs calls, [3] failed, [4] max steps,
+                               // [5] ~0 - (first failed cell), 0 if none
   long long ncell;
   long long stride;          // row stride of state (>= ncell)
   double dt;
@@ -448,7 +449,10 @@ __device__ __forceinline__ void cell_store(const ChemDev& M, const ChemArgs& A, 
     atomicAdd(&A.stats[0], (unsigned long long)nst);
     atomicAdd(&A.stats[1], (unsigned long long)nrej);
     atomicAdd(&A.stats[2], (unsigned long long)nfcn);
-    if (idid < 0) atomicAdd(&A.stats[3], 1ull);
+    if (idid < 0) {
+      atomicAdd(&A.stats[3], 1ull);
+      atomicMax(&A.stats[5], ~0ull - (unsigned long long)cell);   // the lowest failed cell gives the largest value
+    }
     atomicMax(&A.stats[4], (unsigned long long)nst);
   }
 }

@@ -65,6 +65,8 @@ def test_radau5_parity_with_oracle(gpu_ready):
     assert np.array_equal(idid, idid_o)
     ok = idid == 1
     assert ok.mean() > 0.8
+    first = int(np.argmax(~ok))
+    assert rep.failed == int((~ok).sum()) and rep.first_failed_cell == first and rep.first_failed_idid == idid[first]
     scale = np.abs(ref[:, ok]).max(axis=1)
     err = (np.abs(dev[:, ok] - ref[:, ok]).max(axis=1) / scale).max()
     same_steps = float(np.mean(nstep == nstep_o))

@@ -52,6 +52,8 @@ def test_launch_poll_equals_step(gpu_ready):
     b.launch(4)
     with pytest.raises(MoseError):
         b.launch(1)
+    with pytest.raises(MoseError):
+        b.step(1)                  # a launch awaits its poll
     rb = None
     t0 = time.perf_counter()
     while rb is None and time.perf_counter() - t0 < 60:
