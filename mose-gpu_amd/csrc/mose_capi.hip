@@ -661,11 +661,17 @@ static void launch_stage_sh(Case& c, const CaseDev& D, hipStream_t s, const Stag
   }
 }
 
-// the two preferred tile shapes run kernels with the shape fixed at compile time; the others read it at run time
-template <int NS, int NR, int BS>
+// Every tile runs as one workgroup of kStageThreads threads: two per cell of a 256-cell tile, one per cell of a
+// 512-cell tile.  The kernel takes 127 VGPRs, so two 256-cell tiles (76 KB of LDS each at nprim 5, viscous) run on
+// a CU at 4 waves per SIMD; with one thread per cell they ran at 2, and the stage took 16 % longer (7.22 against
+// 5.96 ms on the 256^3 Taylor-Green case).  The two preferred tile shapes run kernels with the shape fixed at compile
+// time; the others read it at run time.
+static constexpr int kStageThreads = 512;
+template <int NS, int NR>
 static void launch_stage_bs(Case& c, const CaseDev& D, hipStream_t s, const StageArgs& A, unsigned grid, size_t lds) {
-  if (BS == 256 && c.TI == 8 && c.TJ == 8 && c.TK == 4) launch_stage_sh<NS, NR, BS, 1>(c, D, s, A, grid, lds);
-  else if (BS == 512 && c.TI == 8 && c.TJ == 8 && c.TK == 8) launch_stage_sh<NS, NR, BS, 2>(c, D, s, A, grid, lds);
+  constexpr int BS = kStageThreads;
+  if (c.TI == 8 && c.TJ == 8 && c.TK == 4) launch_stage_sh<NS, NR, BS, 1>(c, D, s, A, grid, lds);
+  else if (c.TI == 8 && c.TJ == 8 && c.TK == 8) launch_stage_sh<NS, NR, BS, 2>(c, D, s, A, grid, lds);
   else launch_stage_sh<NS, NR, BS, 0>(c, D, s, A, grid, lds);
 }
 
@@ -680,8 +686,7 @@ static void launch_stage(Case& c, const CaseDev& D, hipStream_t s, double rkc, b
   A.last = last ? 1 : 0; A.rkc = rkc; A.dtnext = c.dtnext; A.rpart = c.rpart;
   size_t lds = stage_lds(c, c.TI, c.TJ, c.TK);
   unsigned grid = (unsigned)(((c.ntiles + 7) / 8) * 8);
-  if (c.TI * c.TJ * c.TK == 512) launch_stage_bs<NS, NR, 512>(c, D, s, A, grid, lds);
-  else launch_stage_bs<NS, NR, 256>(c, D, s, A, grid, lds);
+  launch_stage_bs<NS, NR>(c, D, s, A, grid, lds);
 }
 
 // records the step's dt in the history at the device-side position (the same kernel serves eager and graph steps)

@@ -1,5 +1,6 @@
-// The fused stage kernel: one workgroup per tile of TI x TJ x TK cells (256 or 512, one thread per cell) of any block
-// of the case.
+// The fused stage kernel: one workgroup of 512 threads per tile of TI x TJ x TK cells (256 or 512 cells: two threads
+// per cell of a 256-cell tile) of any block of the case.  The face and half-slope loops spread over all 512
+// threads; the per-cell phases run on the first TI TJ TK.
 //
 //   phase A  every interior face of the tile (TI+1)TJTK + TI(TJ+1)TK + TITJ(TK+1): the convective flux into LDS;
 //            the per-cell (rho, 1/rho, T) and the primitives of the tile and its one-cell halo into LDS (viscous
@@ -66,73 +67,114 @@ __device__ __forceinline__ TileGeo tile_geo(const StageArgs& A) {
 }
 
 // One interior viscous face of direction D (diff_cuf_i/j/k + diff_physics.inc) with the cell values (rho, 1/rho, T)
-// and the primitives of its stencil taken from the tile's LDS cache; (i,j,k) = the face's left cell, (a,b,c) = its tile-local face coordinates.
+// and the primitives of its stencil taken from the tile's LDS cache; (i,j,k) = the face's left cell, (a,b,c) = its
+// tile-local face coordinates.  The face's averaged state, thermo and metric come first; then the gradient of one
+// variable at a time is assembled in computational space, taken to physical space and reduced to its flux terms, so
+// that only the three velocity gradients are held at once.  Every value is the same expression as in
+// diff_flux_interior (diff_physics.inc:59-161), evaluated in the same order.
 template <int NS, int NR, int D>
 __device__ __forceinline__ void visc_face(const CaseDev& C, const mose_blk& B, const Tile& tl, const TileGeo& G_,
                                           const double* cache, int i, int j, int k, int a, int b, int c, double* res) {
   using V = Vars<NS, NR>;
   constexpr int NP = V::nprim;
   constexpr int ib = D == 0 ? 1 : 0, ic = D == 2 ? 1 : 2;
-  const double* P = C.Pin;
   const int64_t NC = C.NC;
   const mose_dtab& T = *C.tab;
   const int CI = G_.CI, CJ = G_.CJ, NCa = G_.ncache;
   const int64_t sa[3] = {1, (int64_t)B.pi, (int64_t)B.pi * B.pj};
-  const int64_t st = sa[D], sb = sa[ib], sc = sa[ic];
   const int ca[3] = {1, CI, CI * CJ};
   const int kst = ca[D], kb = ca[ib], kc = ca[ic];
   // cache index of the left cell: tile-local + 1 (halo), one back along D
-  int k1 = (a + 1 - (D == 0)) + CI * ((b + 1 - (D == 1)) + CJ * (c + 1 - (D == 2)));
-  int k2 = k1 + kst;
-  int64_t c0 = row(B, i, j, k), c1 = c0 + st;
+  const int k1 = (a + 1 - (D == 0)) + CI * ((b + 1 - (D == 1)) + CJ * (c + 1 - (D == 2)));
+  const int k2 = k1 + kst;
+  const int64_t c0 = row(B, i, j, k), c1 = c0 + sa[D];
   // the primitives of the tile and its one-cell halo, copied into LDS after the cell scalars: P_s at [(3 + s) NCa]
   const double* Pc = cache + 3 * NCa;
-  double q0[NP], q1[NP], G[NP][3];
+  const double* Ir = cache + NCa;
+  const double* Tc = cache + 2 * NCa;
+  double Pf[NP];
 #pragma unroll
-  for (int s = 0; s < NP; ++s) { q0[s] = Pc[s * NCa + k1]; q1[s] = Pc[s * NCa + k2]; }
-  double ir1 = cache[NCa + k1], ir2 = cache[NCa + k2];
-#pragma unroll
-  for (int s = 0; s < NS; ++s) G[s][D] = q1[s] * ir2 - q0[s] * ir1;
-  G[V::np][D] = cache[2 * NCa + k2] - cache[2 * NCa + k1];
-#pragma unroll
-  for (int s = V::nu; s <= V::nw; ++s) G[s][D] = q1[s] - q0[s];
-#pragma unroll
-  for (int s = V::nt; s < NP; ++s) G[s][D] = q1[s] * ir2 - q0[s] * ir1;
-#pragma unroll
-  for (int pass = 0; pass < 2; ++pass) {
-    const int ax = pass ? ic : ib;
-    const int64_t sd = pass ? sc : sb;
-    const int kd = pass ? kc : kb;
-    int k3 = k1 - kd, k4 = k1 + kd, k5 = k2 - kd, k6 = k2 + kd;
-    double i3 = cache[NCa + k3], i4 = cache[NCa + k4], i5 = cache[NCa + k5], i6 = cache[NCa + k6];
-#pragma unroll
-    for (int s = 0; s < NS; ++s)
-      G[s][ax] = (Pc[s * NCa + k4] * i4 - Pc[s * NCa + k3] * i3 + Pc[s * NCa + k6] * i6 - Pc[s * NCa + k5] * i5) * 0.25;
-    G[V::np][ax] = (cache[2 * NCa + k4] - cache[2 * NCa + k3] + cache[2 * NCa + k6] - cache[2 * NCa + k5]) * 0.25;
-#pragma unroll
-    for (int s = V::nu; s <= V::nw; ++s)
-      G[s][ax] = (Pc[s * NCa + k4] - Pc[s * NCa + k3] + Pc[s * NCa + k6] - Pc[s * NCa + k5]) * 0.25;
-#pragma unroll
-    for (int s = V::nt; s < NP; ++s)
-      G[s][ax] = (Pc[s * NCa + k4] * i4 - Pc[s * NCa + k3] * i3 + Pc[s * NCa + k6] * i6 - Pc[s * NCa + k5] * i5) * 0.25;
-  }
-  double Mav[3][3];
-#pragma unroll
-  for (int x = 0; x < 3; ++x)
-#pragma unroll
-    for (int y = 0; y < 3; ++y) Mav[x][y] = 0.5 * (C.m[(int64_t)(x + 3 * y) * NC + c0] + C.m[(int64_t)(x + 3 * y) * NC + c1]);
-  double Pf[NP], ps[NS];
-#pragma unroll
-  for (int s = 0; s < NP; ++s) Pf[s] = 0.5 * (q0[s] + q1[s]);
-#pragma unroll
-  for (int s = 0; s < NS; ++s) ps[s] = Pf[s];
+  for (int s = 0; s < NP; ++s) Pf[s] = 0.5 * (Pc[s * NCa + k1] + Pc[s * NCa + k2]);
+  const FaceThermo ft = face_thermo<NS>(T, Pf, Pf[V::np], C.vc.prl);
   const double* fm = D == 0 ? C.fm1 : (D == 1 ? C.fm2 : C.fm3);
   const int64_t NFg = D == 0 ? C.NF1 : (D == 1 ? C.NF2 : C.NF3);
   const int64_t fg = D == 0 ? B.f1off + i + (int64_t)(B.n1 + 1) * ((j - 1) + (int64_t)B.n2 * (k - 1))
                    : D == 1 ? B.f2off + (i - 1) + (int64_t)B.n1 * (j + (int64_t)(B.n2 + 1) * (k - 1))
                             : B.f3off + (i - 1) + (int64_t)B.n1 * ((j - 1) + (int64_t)B.n2 * k);
-  FaceThermo ft = face_thermo<NS>(T, ps, Pf[V::np], C.vc.prl);
-  diff_flux_interior<NS, NR>(T, C.vc, G, Mav, Pf, ft, fm[fg], fm[NFg + fg], fm[2 * NFg + fg], fm[3 * NFg + fg], res);
+  const double nx = fm[fg], ny = fm[NFg + fg], nz = fm[2 * NFg + fg], area = fm[3 * NFg + fg];
+  double Mav[3][3];
+#pragma unroll
+  for (int x = 0; x < 3; ++x)
+#pragma unroll
+    for (int y = 0; y < 3; ++y) Mav[x][y] = 0.5 * (C.m[(int64_t)(x + 3 * y) * NC + c0] + C.m[(int64_t)(x + 3 * y) * NC + c1]);
+  const double drho = ft.rho, dmil = ft.mil;
+  const int Ti = ftrunc(ft.T);
+  const double Tdf = ft.T - Ti;
+  double mie = 0.0;
+  if (NR > 0) {
+    double chi = Pf[V::nt] / dmil, c3 = chi * chi * chi;
+    mie = c3 / (c3 + SA_CV1 * SA_CV1 * SA_CV1) * Pf[V::nt];
+  }
+  const double Dm = (dmil / C.vc.sc + mie / C.vc.sct) / drho;
+  const double kap = ft.kl + mie * ft.cp / C.vc.prt;
+  const double dmi = dmil + mie;
+  // the physical-space gradient of variable s: kind 0 = P_s / rho, 1 = T, 2 = P_s
+  auto grad = [&](int s, int kind, double* g) {
+    const double* Q = kind == 1 ? Tc : Pc + s * NCa;
+    double r[3];
+    if (kind == 0) r[D] = Q[k2] * Ir[k2] - Q[k1] * Ir[k1];
+    else r[D] = Q[k2] - Q[k1];
+#pragma unroll
+    for (int pass = 0; pass < 2; ++pass) {
+      const int kd = pass ? kc : kb;
+      const int k3 = k1 - kd, k4 = k1 + kd, k5 = k2 - kd, k6 = k2 + kd;
+      double t;
+      if (kind == 0) t = (Q[k4] * Ir[k4] - Q[k3] * Ir[k3] + Q[k6] * Ir[k6] - Q[k5] * Ir[k5]) * 0.25;
+      else t = (Q[k4] - Q[k3] + Q[k6] - Q[k5]) * 0.25;
+      r[pass ? ic : ib] = t;
+    }
+#pragma unroll
+    for (int y = 0; y < 3; ++y) g[y] = r[0] * Mav[0][y] + r[1] * Mav[1][y] + r[2] * Mav[2][y];
+  };
+  double DiffH = 0.0, sf = 0.0;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    double g[3];
+    grad(s, 0, g);
+    double DmG = Dm * (g[0] * nx + g[1] * ny + g[2] * nz);
+    res[s] = drho * area * DmG;
+    double h1 = tab(T.h, s, Ti), h2 = tab(T.h, s, Ti + 1);
+    DiffH = DiffH + DmG * (h1 + Tdf * (h2 - h1));
+  }
+#pragma unroll
+  for (int s = 0; s < NS; ++s) sf = sf + res[s];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) res[s] = res[s] - sf * Pf[s] / drho;
+  double qT;
+  {
+    double g[3];
+    grad(0, 1, g);
+    qT = kap * (g[0] * nx + g[1] * ny + g[2] * nz);
+  }
+  if (NR > 0) {
+    double g[3];
+    grad(V::nt, 0, g);
+    double gn = g[0] * nx + g[1] * ny + g[2] * nz;
+    res[V::nt] = (dmil + Pf[V::nt]) * gn * area / SA_SIGMA;
+  }
+  double gu[3], gv[3], gw[3];
+  grad(V::nu, 2, gu);
+  grad(V::nv, 2, gv);
+  grad(V::nw, 2, gw);
+  double lamDiv = NTT * dmi * (gu[0] + gv[1] + gw[2]);
+  double Txx = lamDiv + 2.0 * dmi * gu[0] + 0.0;
+  double Tyy = lamDiv + 2.0 * dmi * gv[1] + 0.0;
+  double Tzz = lamDiv + 2.0 * dmi * gw[2] + 0.0;
+  double Txy = dmi * (gu[1] + gv[0]), Txz = dmi * (gu[2] + gw[0]);
+  double Tyz = dmi * (gv[2] + gw[1]);
+  double S1 = Txx * nx + Txy * ny + Txz * nz, S2 = Txy * nx + Tyy * ny + Tyz * nz, S3 = Txz * nx + Tyz * ny + Tzz * nz;
+  res[V::nu] = area * S1; res[V::nv] = area * S2; res[V::nw] = area * S3;
+  res[V::np] = area * ((S1 * Pf[V::nu] + S2 * Pf[V::nv] + S3 * Pf[V::nw]) + qT + DiffH * drho);
   (void)tl;
 }
 
@@ -361,10 +403,13 @@ __device__ __forceinline__ int xcd_tile(int wg, int n) {
   return q < cnt ? base + q : -1;
 }
 
-// BS = the workgroup size = the cells of a tile (256 or 512).  Larger tiles recompute fewer halo half-slopes and
-// faces and re-read less of their neighbours; MI355X's 160 KB of LDS per workgroup holds an 8 x 8 x 8 tile's
-// half-slopes and face fluxes.
+// BS = the workgroup size (at least the cells of the tile).  Larger tiles recompute fewer halo half-slopes and faces
+// and re-read less of their neighbours; MI355X's 160 KB of LDS per workgroup holds an 8 x 8 x 8 tile's half-slopes
+// and face fluxes.  Smaller tiles leave room for two workgroups per CU.
 // SH selects a tile shape fixed at compile time: 0 = from the launch arguments, 1 = 8 x 8 x 4, 2 = 8 x 8 x 8.
+// Without RANS the 8 x 8 x 4 kernels take at most 128 VGPRs (127 for the viscous LMRoe one), so two 512-thread
+// workgroups share a CU (4 waves per SIMD).  An amdgpu_waves_per_eu(4) attribute gives the same count but a schedule
+// that hides less latency: 6.41 against 5.96 ms per stage on the 256^3 Taylor-Green case.
 template <int NS, int NR, bool VISC, int RS, int BS, int SH>
 __global__ void __launch_bounds__(BS) k_stage(CaseDev C, StageArgs A) {
   const int t = xcd_tile(blockIdx.x, A.ntiles);
