@@ -7,7 +7,8 @@ Prints one JSON line: value = cell-updates/s (cells x steps / s) with the state 
 fused stage kernel from HIP events on its own stream (algorithmic bytes per cell-stage 8 (3 nprim + 25) = 320 B);
 cpu_baseline = the CPU oracle (a C restatement of the reference's double-precision path) on a bounded sample.
 Beside the headline line's own fields: "serving" (continuous batching of a Poisson stream of cases, p50/p99
-time-to-solution, against the plain per-case path) and "chemistry" (batched RODAS cells/s with its CPU sample).
+time-to-solution, against the plain per-case path) and "chemistry" (batched RODAS cells/s with its CPU sample, and
+RADAU5 beside it).
 """
 from __future__ import annotations
 
@@ -59,11 +60,12 @@ def cpu_baseline(n: int, steps: int, threads: int):
                       f"(gcc -O2 -ffp-contract=off, OpenMP {threads} threads), {dt:.1f} s"}
 
 
-def chemistry_leg(ncell: int, cpu_cells: int, threads: int):
+def chemistry_leg(ncell: int, cpu_cells: int, threads: int, radau_cells: int = 0, radau_cpu: int = 0):
     """Batched stiff chemistry: independent cells of the 9-species hydrogen-air mechanism at 1 atm, T uniform in
     [900, 2500] K, equivalence ratio uniform in [0.5, 2] (seeded), each integrated over dt = 1e-6 s with RODAS at
     rtol 1e-8 / atol 1e-12 in one persistent launch.  The 2M-cell batch is sampled by `ncell` cells (the per-cell
-    work does not depend on the batch size once the device is full); cells/s from the kernel's HIP events."""
+    work does not depend on the batch size once the device is full); cells/s from the kernel's HIP events.
+    `radau_cells` > 0 adds the same cells under RADAU5 (its own launch) as chemistry["radau5"]."""
     import numpy as np
     from mosegpu import Context, DeviceChem, chem
     from mosegpu.core import ThermoTables
@@ -85,6 +87,8 @@ def chemistry_leg(ncell: int, cpu_cells: int, threads: int):
                        "dt 1e-6 s, RODAS rtol 1e-8 atol 1e-12"}
     print(f"bench: chemistry {ncell} cells in {rep.ms / 1e3:.2f} s: {res['value']:.3e} cells/s", file=sys.stderr,
           flush=True)
+    if radau_cells > 0:
+        res["radau5"] = radau5_leg(M, chem.h2_air_cells(radau_cells, seed=22), radau_cpu, threads)
     if cpu_cells > 0:
         sys.path.insert(0, str(REPO / "tests"))
         from oracle_bind import OracleChem
@@ -95,6 +99,44 @@ def chemistry_leg(ncell: int, cpu_cells: int, threads: int):
         dt = time.perf_counter() - t0
         res["cpu_baseline"] = {"value": cpu_cells / dt, "unit": "cells/s", "cores": threads, "kind": "port",
                                "sample": f"the first {cpu_cells} cells of the same batch, CPU oracle RODAS (gcc -O2, "
+                                         f"OpenMP {threads} threads), {dt:.1f} s"}
+    return res
+
+
+def radau5_leg(M, st, cpu_cells: int, threads: int):
+    """The chemistry cells under RADAU5 (rtol 1e-8 / atol 1e-12).  Cells on which the simplified Newton iteration
+    stalls run to NMAX = 100000 steps and end with idid -2, as in the reference's own RADAU5; one such cell takes a
+    team the whole launch, so the launch time is set by that tail."""
+    from mosegpu import Context, DeviceChem, MoseError, chem
+    from mosegpu.core import ThermoTables
+    n = st.shape[1]
+    ctx = Context(ThermoTables(M.wm, M.cp, M.h, names=M.names))
+    dc = DeviceChem(ctx, M, n)
+    dc.upload(st)
+    try:
+        rep = dc.advance(1e-6, chem.ode_opts(1e-8, 1e-12, method=2))
+    except MoseError:
+        rep = dc.last_report     # cells that stop at NMAX are reported as an error; the launch itself completed
+    out, hr, idid, nstep = dc.download()
+    dc.destroy()
+    ctx.close()
+    stalled = int((idid == -2).sum())
+    res = {"cells": n, "value": n / (rep.ms * 1e-3), "unit": "cells/s", "kernel": "k_radau (RADAU5, a team of ns+1 "
+           "lanes per cell, persistent queue)", "kernel_ms": rep.ms, "steps_mean": float(nstep.mean()),
+           "steps_max": int(nstep.max()), "stopped_at_nmax": stalled, "failed_other": int((idid < 0).sum()) - stalled,
+           "steps_mean_completed": float(nstep[idid >= 0].mean()) if (idid >= 0).any() else None}
+    print(f"bench: radau5 {n} cells in {rep.ms / 1e3:.2f} s: {res['value']:.3e} cells/s, {stalled} stopped at NMAX",
+          file=sys.stderr, flush=True)
+    if cpu_cells > 0:
+        sys.path.insert(0, str(REPO / "tests"))
+        from oracle_bind import OracleChem
+        sample = st[:, :cpu_cells].copy()
+        o = OracleChem(M, nthreads=threads)
+        t0 = time.perf_counter()
+        o.advance(sample, 1e-6, method=2)
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": cpu_cells / dt, "unit": "cells/s", "cores": threads, "kind": "port",
+                               "sample": f"the first {cpu_cells} cells of the same batch, CPU oracle RADAU5 (gcc -O2, "
                                          f"OpenMP {threads} threads), {dt:.1f} s"}
     return res
 
@@ -142,6 +184,8 @@ def main():
     ap.add_argument("--serve", type=int, default=16, help="requests of the serving leg (0: skip)")
     ap.add_argument("--chem", type=int, default=262144, help="cells of the chemistry leg (0: skip)")
     ap.add_argument("--chem-cpu", type=int, default=2048, help="cells of the chemistry CPU baseline sample")
+    ap.add_argument("--chem-radau", type=int, default=65536, help="cells of the RADAU5 chemistry leg (0: skip)")
+    ap.add_argument("--chem-radau-cpu", type=int, default=1024, help="cells of the RADAU5 CPU baseline sample")
     ap.add_argument("--serve-plain", action="store_true", help="also run the serving requests one by one on the "
                     "plain per-block path (the comparison path)")
     args = ap.parse_args()
@@ -207,7 +251,8 @@ def main():
     if args.serve > 0:
         out["serving"] = serving_leg(args.serve, args.serve_plain)
     if args.chem > 0:
-        out["chemistry"] = chemistry_leg(args.chem, 0 if args.no_cpu else args.chem_cpu, min(16, os.cpu_count() or 1))
+        out["chemistry"] = chemistry_leg(args.chem, 0 if args.no_cpu else args.chem_cpu, min(16, os.cpu_count() or 1),
+                                         args.chem_radau, 0 if args.no_cpu else args.chem_radau_cpu)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_steps, min(16, os.cpu_count() or 1))
     else:
