@@ -33,18 +33,36 @@ def algorithmic_bytes_per_cell_stage(nprim: int, viscous: bool, rans: bool) -> i
     return 8 * (3 * nprim + G)
 
 
-def profiled_traffic(n: int, nprim: int, viscous: bool, tile) -> int | None:
-    """HBM bytes per stage launch measured by rocprofv3 counters for this workload (profiles/*_traffic.txt), or None."""
+def profiled_counts(n: int, nprim: int, viscous: bool, tile) -> tuple[int | None, int | None]:
+    """(HBM bytes, VALU wavefront instructions) per stage launch, measured by rocprofv3 counters for this workload
+    (profiles/*_traffic.txt), or None where the workload has not been profiled."""
     key = ("k_stage", str(n), str(nprim), str(int(viscous)), "x".join(str(t) for t in tile))
     for f in sorted((REPO / "profiles").glob("*_traffic.txt")):
         for line in f.read_text().splitlines():
             w = line.split()
-            if len(w) == 6 and not line.startswith("#") and tuple(w[:5]) == key:
-                return int(w[5])
-    return None
+            if len(w) >= 6 and not line.startswith("#") and tuple(w[:5]) == key:
+                return int(w[5]), (int(w[6]) if len(w) > 6 else None)
+    return None, None
 
 
-def cpu_baseline(n: int, steps: int, threads: int):
+# FP64 VALU issue: a wave64 instruction takes 4 cycles of one SIMD; MI355X has 256 CUs x 4 SIMDs at 2.4 GHz
+VALU_ISSUE_PER_S = 1024 * 2.4e9 / 4
+
+
+def host_cpu() -> dict:
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "model": model}
+
+
+def oracle_rate(n: int, steps: int, threads: int) -> tuple[float, float]:
+    """cell-updates/s of the CPU oracle on the Taylor-Green case at n^3 over `steps` steps after a warm-up step."""
     sys.path.insert(0, str(REPO / "tests"))
     from mosegpu import cases
     from oracle_bind import OracleCase
@@ -55,9 +73,18 @@ def cpu_baseline(n: int, steps: int, threads: int):
     t0 = time.perf_counter()
     o.step(P, steps)
     dt = time.perf_counter() - t0
-    return {"value": n ** 3 * steps / dt, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+    return n ** 3 * steps / dt, dt
+
+
+def cpu_baseline(n: int, steps: int, threads: int):
+    rate, dt = oracle_rate(n, steps, threads)
+    n1, s1 = 32, 16
+    rate1, dt1 = oracle_rate(n1, s1, 1)
+    return {"value": rate, "unit": "cell-updates/s", "cores": threads, "kind": "port",
             "sample": f"taylor-green {n}^3, {steps} SSP-RK3 steps after 1 warm-up step, CPU oracle "
-                      f"(gcc -O2 -ffp-contract=off, OpenMP {threads} threads), {dt:.1f} s"}
+                      f"(gcc -O2 -ffp-contract=off, OpenMP {threads} threads), {dt:.1f} s",
+            "single_thread": {"value": rate1, "sample": f"taylor-green {n1}^3, {s1} steps, 1 thread, {dt1:.1f} s"},
+            "host": host_cpu()}
 
 
 def chemistry_leg(ncell: int, cpu_cells: int, threads: int, radau_cells: int = 0, radau_cpu: int = 0):
@@ -230,9 +257,12 @@ def main():
     value = cells * args.steps / elapsed
     B = algorithmic_bytes_per_cell_stage(case.nprim, True, False)
     achieved = (B * cells) / (ms_avg * 1e-3) / 1e9 if ms_avg > 0 else None
+    traffic, valu = profiled_counts(args.n, case.nprim, True, chosen) if path == mosegpu.PATH_FUSED else (None, None)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-                "traffic": profiled_traffic(args.n, case.nprim, True, chosen) if path == mosegpu.PATH_FUSED else None,
+                "traffic": traffic,
+                # the share of the launch that FP64 VALU issue alone would take (SQ_INSTS_VALU from the profile)
+                "valu_issue_frac": (valu / VALU_ISSUE_PER_S) / (ms_avg * 1e-3) if valu and ms_avg > 0 else None,
                 "kernel": "k_stage (fused residual + RK stage update)" if path == mosegpu.PATH_FUSED
                           else "face kernels + cell update (one stage)",
                 "kernel_ms_avg": ms_avg, "launches": nlaunch, "bytes_per_cell_stage": B}
