@@ -292,6 +292,8 @@ static int plan_case(const mose_scheme* sc, int nb, const int* dims, int nrec, c
         }
         break;
       }
+      case MOSE_BC_NONE:         // no flux; zero-gradient ghosts
+        break;
       case MOSE_BC_EXTRAPOLATION:
         if (n[dir] == 2) { why = "extrapolation faces need a block extent of 1 or >= 3 along the face normal"; return MOSE_ERR_ARG; }
         break;

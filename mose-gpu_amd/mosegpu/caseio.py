@@ -1,0 +1,365 @@
+"""Case directories in the reference's own input formats (test/<dim>/<case>/{input.ini, INPUT/*}), read into a Case
+that this library runs.
+
+  input.ini      the MOSE_open dialect: [MOSE-Physics], [MOSE-Numerics], [MOSE-Parameters] (IO_Ini_Open.f90:60-320)
+  phase.txt      a header line, then one "name molecular-weight" line per species (Load_ThermoTransport.f90:7-60)
+  thermo.dat     Tecplot POINT, one zone per species: T, cp, h, s, dcp on T = Tmin..Tmax; with Tmin = 1, row 0 is
+                 cp(1) with h = 0 (Load_ThermoTransport.f90:77-101)
+  transport.dat  Tecplot POINT, one zone per species: T, mu, k; row 0 = row 1 (Load_ThermoTransport.f90:144-189)
+  ic.tec         Tecplot BLOCK, one zone per block: the nodal x, y, z and the cell-centred primitives
+  bc.txt         one record per boundary cell-face, 3-digit MOSE_open types (IO_BC_Open.f90:132-399)
+
+Only what this path runs is accepted; anything else is refused by name (local time stepping, prim/prec
+integration, Riemann solvers other than HLLC / AUSM+ / LMRoe, boundary types outside 0/100, 101/201, 301 with zero
+heat flux, 302 and 400, axisymmetric meshes).  Files may be stored gzip-compressed (name + ".gz").
+"""
+from __future__ import annotations
+
+import gzip
+import math
+import re
+import os
+
+import numpy as np
+
+from .cases import Case, fill_ghosts_nearest, make_scheme
+from .core import (BC_CONNECTION, BC_EXTRAPOLATION, BC_NONE, BC_WALL_ADIABATIC, BC_WALL_ISOTHERMAL, LIM, NTAB,
+                   RECORD_DTYPE, RS, Layout, ThermoTables, metrics_build)
+
+RUNIV = 8314.46261815324     # Mod_Convective.f90:30
+
+
+class CaseFormatError(ValueError):
+    """A case file that this path cannot run, or that does not parse."""
+
+
+def _open(path: str):
+    if os.path.exists(path):
+        return open(path, "rt")
+    if os.path.exists(path + ".gz"):
+        return gzip.open(path + ".gz", "rt")
+    raise FileNotFoundError(path)
+
+
+# ------------------------------------------------------------------------------------------------- input.ini
+def read_ini(path: str) -> dict:
+    """{section: {key: value}}; ';' and '#' start comments; keys keep their case, values are stripped."""
+    out: dict = {}
+    sec = None
+    with _open(path) as f:
+        for line in f:
+            s = line.split(";", 1)[0].split("#", 1)[0].strip()
+            if not s:
+                continue
+            if s.startswith("[") and s.endswith("]"):
+                sec = s[1:-1].strip()
+                out.setdefault(sec, {})
+            elif "=" in s and sec is not None:
+                k, v = s.split("=", 1)
+                out[sec][k.strip()] = v.strip()
+    return out
+
+
+def _truth(v: str) -> bool:
+    return v.strip().lower() in ("true", "t", ".true.", "yes", "1")
+
+
+# ------------------------------------------------------------------------------------------------- phase, tables
+def read_phase(path: str):
+    """(names, molecular weights) of phase.txt: the first line is a header."""
+    names, wm = [], []
+    with _open(path) as f:
+        next(f)
+        for line in f:
+            w = line.split()
+            if len(w) >= 2:
+                names.append(w[0])
+                wm.append(float(w[1]))
+    return names, np.array(wm)
+
+
+def read_tecplot_point(path: str) -> list[np.ndarray]:
+    """The zones of a Tecplot POINT file, each as a (rows, nvars) array."""
+    zones: list[list[list[float]]] = []
+    with _open(path) as f:
+        for line in f:
+            s = line.strip()
+            if not s:
+                continue
+            head = s.split(None, 1)[0].upper()
+            if head.startswith("ZONE"):
+                zones.append([])
+                continue
+            try:
+                row = [float(t) for t in s.replace(",", " ").split()]
+            except ValueError:
+                continue                          # TITLE, VARIABLES, "I=..., F=POINT"
+            if zones:
+                zones[-1].append(row)
+    return [np.array(z, dtype=np.float64) for z in zones]
+
+
+def _rows_from_zero(T: np.ndarray, col: np.ndarray, what: str, row0_h_zero: bool = False) -> np.ndarray:
+    """A table column on T = Tmin..Tmax as this library's rows T = 0..NTAB-1.  Tmin must be 1 (row 0 is then
+    filled as FLINT does); rows above Tmax repeat the last row, which a lookup reaches only above Tmax - 1 K."""
+    Tmin = int(round(T[0]))
+    if Tmin != 1 or np.any(np.diff(np.round(T)) != 1):
+        raise CaseFormatError(f"{what}: rows must run T = 1, 2, ... K (got {T[0]} .. {T[-1]})")
+    n = T.size
+    if n > NTAB - 1:
+        raise CaseFormatError(f"{what}: {n} rows; this library's tables end at {NTAB - 1} K")
+    out = np.empty(NTAB)
+    out[1:n + 1] = col
+    out[n + 1:] = col[-1]
+    out[0] = 0.0 if row0_h_zero else col[0]
+    return out
+
+
+def read_tables(input_dir: str, viscous: bool) -> ThermoTables:
+    names, wm = read_phase(os.path.join(input_dir, "phase.txt"))
+    zones = read_tecplot_point(os.path.join(input_dir, "thermo.dat"))
+    if len(zones) != len(names):
+        raise CaseFormatError(f"thermo.dat has {len(zones)} zones for {len(names)} species")
+    cp = np.stack([_rows_from_zero(z[:, 0], z[:, 1], "thermo.dat") for z in zones])
+    h = np.stack([_rows_from_zero(z[:, 0], z[:, 2], "thermo.dat", row0_h_zero=True) for z in zones])
+    mu = k = None
+    if viscous:
+        tz = read_tecplot_point(os.path.join(input_dir, "transport.dat"))
+        if len(tz) != len(names):
+            raise CaseFormatError(f"transport.dat has {len(tz)} zones for {len(names)} species")
+        mu = np.stack([_rows_from_zero(z[:, 0], z[:, 1], "transport.dat") for z in tz])
+        k = np.stack([_rows_from_zero(z[:, 0], z[:, 2], "transport.dat") for z in tz])
+    return ThermoTables(wm, cp, h, mu, k, names=names)
+
+
+# ------------------------------------------------------------------------------------------------- ic.tec
+def _parse_varlocation(header: str, nvar: int) -> list[bool]:
+    """True = cell-centred, from VARLOCATION=([a-b]=NODAL,[c-d]=CELLCENTERED); NODAL where absent."""
+    cc = [False] * nvar
+    up = header.upper()
+    p = up.find("VARLOCATION")
+    if p < 0:
+        return cc
+    body = up[up.find("(", p) + 1:up.find(")", p)]
+    for rng, loc in re.findall(r"\[([\d,\- ]+)\]\s*=\s*(\w+)", body):
+        for piece in rng.split(","):
+            a, _, b = piece.partition("-")
+            lo, hi = int(a), int(b or a)
+            for q in range(lo, hi + 1):
+                cc[q - 1] = loc.strip().startswith("CELL")
+    return cc
+
+
+def _zone_int(header: str, key: str) -> int:
+    up = header.upper().replace(" ", "")
+    p = up.find(key + "=")
+    if p < 0:
+        return 1
+    q = p + len(key) + 1
+    e = q
+    while e < len(up) and up[e].isdigit():
+        e += 1
+    return int(up[q:e])
+
+
+def read_tecplot_block(path: str):
+    """Zones of a Tecplot BLOCK file: [(names, I, J, K, {name: array})]; nodal arrays have shape (K, J, I),
+    cell-centred ones (K-1, J-1, I-1) (i fastest, as Tecplot writes them)."""
+    with _open(path) as f:
+        text = f.read()
+    lines = text.splitlines()
+    names: list[str] = []
+    zones = []
+    q = 0
+    while q < len(lines):
+        s = lines[q].strip()
+        up = s.upper()
+        if up.startswith("VARIABLES"):
+            names = [t for t in s.split("=", 1)[1].replace(",", " ").split()]
+            names = [t.strip('"') for t in names]
+            q += 1
+        elif up.startswith("ZONE"):
+            header = s
+            q += 1
+            # header continuation lines (no leading number)
+            while q < len(lines):
+                t = lines[q].strip()
+                try:
+                    float(t.split()[0])
+                    break
+                except (ValueError, IndexError):
+                    header += " " + t
+                    q += 1
+            I, J, K = _zone_int(header, "I"), _zone_int(header, "J"), _zone_int(header, "K")
+            cc = _parse_varlocation(header, len(names))
+            counts = [((I - 1) * max(J - 1, 1) * max(K - 1, 1)) if c else I * J * K for c in cc]
+            need = sum(counts)
+            vals: list[float] = []
+            while len(vals) < need and q < len(lines):
+                t = lines[q].strip()
+                if t:
+                    vals.extend(float(x) for x in t.split())
+                q += 1
+            if len(vals) < need:
+                raise CaseFormatError(f"{path}: zone with {len(vals)} values, {need} expected")
+            arr = np.array(vals[:need])
+            data, o = {}, 0
+            for nm, c, n in zip(names, cc, counts):
+                shape = (max(K - 1, 1), max(J - 1, 1), I - 1) if c else (K, J, I)
+                data[nm] = arr[o:o + n].reshape(shape)
+                o += n
+            zones.append((names, I, J, K, data))
+        else:
+            q += 1
+    return zones
+
+
+# ------------------------------------------------------------------------------------------------- bc.txt
+# 3-digit MOSE_open types this path runs -> this library's record types (IO_BC_Open.f90:155-211)
+_OPEN_TYPES = {0: BC_NONE, 100: BC_NONE, 101: BC_CONNECTION, 201: BC_CONNECTION, 301: BC_WALL_ADIABATIC,
+               302: BC_WALL_ISOTHERMAL, 400: BC_EXTRAPOLATION}
+
+
+def read_bc_open(path: str) -> np.ndarray:
+    """Records of a MOSE_open bc.txt, in file order, as RECORD_DTYPE with 0-based block ids.  A first line
+    "b i j k f type" is followed by the type's property line: 101/201 "bs is js ks fs d11 d12 d21 d22",
+    301 "qw ks eps", 302 "Tw ks eps"; 0, 100 and 400 have none."""
+    recs = []
+    with _open(path) as f:
+        lines = [ln for ln in (x.strip() for x in f) if ln]
+    q = 0
+    while q < len(lines):
+        w = lines[q].split()
+        q += 1
+        b, i, j, k, fc, typ = (int(x) for x in w[:6])
+        r = np.zeros(1, dtype=RECORD_DTYPE)[0]
+        r["b"] = b - 1; r["i"] = i; r["j"] = j; r["k"] = k; r["f"] = fc
+        r["d11"] = 1; r["d22"] = 1
+        if typ not in _OPEN_TYPES:
+            raise CaseFormatError(f"bc.txt: boundary type {typ} (block {b}, face {fc}) is not on this path")
+        r["typ"] = _OPEN_TYPES[typ]
+        if typ in (101, 201):
+            v = [int(x) for x in lines[q].split()[:9]]
+            q += 1
+            r["bs"] = v[0] - 1; r["is_"] = v[1]; r["js"] = v[2]; r["ks"] = v[3]; r["fs"] = v[4]
+            r["d11"], r["d12"], r["d21"], r["d22"] = v[5:9]
+            r["periodic"] = 1 if typ == 201 else 0      # 201: ghost nodes translated by one period (metrics)
+        elif typ in (301, 302):
+            v = [float(x.rstrip(",")) for x in lines[q].split()[:3]]
+            q += 1
+            if any(x != 0.0 for x in v[1:]):
+                raise CaseFormatError("bc.txt: wall roughness / emissivity extras are not on this path")
+            if typ == 301 and v[0] != 0.0:
+                raise CaseFormatError("bc.txt: a 301 wall with a nonzero heat flux is not on this path")
+            if typ == 302:
+                r["tw"] = v[0]
+        recs.append(r)
+    return np.array(recs, dtype=RECORD_DTYPE)
+
+
+# ------------------------------------------------------------------------------------------------- the case
+def mesh_ndir(dims, nodes0) -> int:
+    """Check_Mesh_Type (Lib_Metrics.f90:139-196) on block 1: 3 if n3 > 1, 1 if n2 = n3 = 1, else 2.  A mesh that
+    the reference takes as axisymmetric (2Daxi, or a 3D wedge) is refused."""
+    n1, n2, n3 = dims
+    nd = nodes0                                      # (n3+1, n2+1, n1+1, 3), node(i, j, k) = nd[k, j, i]
+    ang = lambda p: math.atan2(p[2], p[1])
+    if n3 > 1:
+        t1 = ang(nd[n3, 1, 0]) - ang(nd[0, 1, 0])
+        t2 = ang(nd[n3, n2, 0]) - ang(nd[0, n2, 0])
+        if (t1 - t2) < 1e-5 and t1 != 0.0:
+            raise CaseFormatError("axisymmetric 3D mesh: not on this path")
+        return 3
+    if n2 == 1:
+        return 1
+    t1 = ang(nd[1, 1, 0]) - ang(nd[0, 1, 0])
+    t2 = ang(nd[1, n2, 0]) - ang(nd[0, n2, 0])
+    r0 = math.hypot(nd[0, 1, 0][1], nd[0, 1, 0][2])
+    r1 = math.hypot(nd[1, 1, 0][1], nd[1, 1, 0][2])
+    if (t1 - t2) < 1e-5 and r0 > 1e-10 and abs(r0 - r1) < 1e-5 * r0:
+        raise CaseFormatError("axisymmetric 2D mesh: not on this path")
+    return 2
+
+
+def scheme_from_ini(ini: dict, ndir: int, nsc: int):
+    ph = ini.get("MOSE-Physics", {})
+    nu = ini.get("MOSE-Numerics", {})
+    eq = ph.get("equations", "")
+    if eq not in ("euler", "navier-stokes"):
+        raise CaseFormatError(f"equations = {eq!r}: euler or navier-stokes")
+    viscous = eq == "navier-stokes"
+    turb = ph.get("turbulence", "none") if viscous else "none"
+    if turb not in ("none", "SA"):
+        raise CaseFormatError(f"turbulence = {turb}: only SA is on this path")
+    if not _truth(nu.get("time-accurate", "false")):
+        raise CaseFormatError("time-accurate = false (local time stepping) is not on this path")
+    rk = {"euler": 1, "RK2": 2, "RK3": 3}.get(nu.get("time-scheme", ""))
+    if rk is None:
+        raise CaseFormatError(f"time-scheme = {nu.get('time-scheme')!r}: euler, RK2 or RK3")
+    if nu.get("integration-variables", "cons") != "cons":
+        raise CaseFormatError("integration-variables other than cons are not on this path")
+    recon = nu.get("space-reconstruction", "")
+    if recon not in ("MUSCL", "first-order"):
+        raise CaseFormatError(f"space-reconstruction = {recon!r}: MUSCL or first-order")
+    lim = "first-order" if recon == "first-order" else nu.get("flux-limiter", "").lower()
+    if lim not in LIM:
+        raise CaseFormatError(f"flux-limiter = {lim!r}: one of {sorted(LIM)}")
+    rs = nu.get("riemann-solver", "HLLC")
+    if rs not in ("HLLC", "AUSM+", "LMRoe"):
+        raise CaseFormatError(f"riemann-solver = {rs}: HLLC, AUSM+ or LMRoe on this path")
+    mco = float(nu.get("riemann-options-Mco", "1.0"))
+    if "cfl" not in nu:
+        raise CaseFormatError("[MOSE-Numerics] cfl is required")
+    kw = {}
+    if viscous:
+        for key in ("vnn",):
+            if key not in nu:
+                raise CaseFormatError(f"[MOSE-Numerics] {key} is required by this loader for a viscous case")
+        if "schmidt" not in ph or float(ph["schmidt"]) <= 0.0:
+            raise CaseFormatError("a viscous case needs [MOSE-Physics] schmidt > 0 (multicomponent diffusion is not "
+                                  "on this path)")
+        kw = dict(vnn=float(nu["vnn"]), prandtl=float(ph.get("prandtl", "0")), sc=float(ph["schmidt"]))
+    pimp = 1
+    if turb == "SA":
+        pimp = 1 if _truth(ini.get("MOSE-Turbulence", {}).get("point-implicit", "true")) else 0
+    return make_scheme(nsc=nsc, nrans=1 if turb == "SA" else 0, limiter=lim, riemann=rs, viscous=viscous, n_rk=rk,
+                       ndir=ndir, cfl=float(nu["cfl"]), mco=mco, point_implicit=pimp, **kw)
+
+
+def load_case(case_dir: str, nsteps: int | None = None) -> Case:
+    """The case of a reference test directory (input.ini and INPUT/), ready for DeviceCase and the oracle.
+    nsteps defaults to [MOSE-Parameters] iter-threshold (0 when the case ends on time-threshold)."""
+    ini = read_ini(os.path.join(case_dir, "input.ini"))
+    inp = os.path.join(case_dir, "INPUT")
+    viscous = ini.get("MOSE-Physics", {}).get("equations") == "navier-stokes"
+    tables = read_tables(inp, viscous)
+    zones = read_tecplot_block(os.path.join(inp, "ic.tec"))
+    dims, nodes, fields = [], [], []
+    for names, I, J, K, data in zones:
+        n1, n2, n3 = I - 1, max(J - 1, 1), max(K - 1, 1)
+        dims.append((n1, n2, n3))
+        nd = np.stack([data[names[0]], data[names[1]], data[names[2]]], axis=-1)
+        nodes.append(np.ascontiguousarray(nd))
+        fields.append([data[nm] for nm in names[3:]])
+    ndir = mesh_ndir(dims[0], nodes[0])
+    sch = scheme_from_ini(ini, ndir, tables.nsc)
+    nprim = tables.nsc + 4 + sch.nrans
+    for b, fl in enumerate(fields):
+        if len(fl) != nprim:
+            raise CaseFormatError(f"ic.tec zone {b + 1}: {len(fl)} cell variables, {nprim} expected")
+    layout = Layout(dims)
+    rec = read_bc_open(os.path.join(inp, "bc.txt"))
+    order = np.lexsort((rec["f"], rec["b"]))          # grouped by block, then face; file order inside a group
+    rec = np.ascontiguousarray(rec[order])
+    geom = metrics_build(layout, nodes, rec, ndir=ndir)
+    P = np.zeros((nprim, layout.NC))
+    for b, fl in enumerate(fields):
+        V = layout.interior(P, b)
+        for s in range(nprim):
+            V[s] = fl[s]
+    fill_ghosts_nearest(layout, P)
+    par = ini.get("MOSE-Parameters", {})
+    if nsteps is None:
+        nsteps = int(float(par.get("iter-threshold", "0")))
+    return Case(os.path.basename(os.path.normpath(case_dir)), tables, sch, layout, nodes, rec, geom, P, nsteps,
+                meta={"ini": ini, "ndir": ndir, "time_threshold": float(par.get("time-threshold", "0") or 0)})

@@ -1,0 +1,35 @@
+#!/usr/bin/env python3
+"""Copies the input files of the reference's own test cases that this path runs into fixtures beside this script
+(input.ini and INPUT/*; the large files gzip-compressed, which mosegpu.caseio reads as they are).  Run in a
+container where the reference tree is at /root/reference; the GPU tests read only the copies.
+
+  2D/euler/isentropic-vortex   Euler, HLLC + van Leer, SSP-RK3, CFL 0.5, 64 x 64 x 1, periodic (MOSE 201) in x and y,
+                               no-flux faces (MOSE 0) in the one-cell direction
+"""
+import gzip
+import os
+import shutil
+
+REF = "/root/reference/test"
+HERE = os.path.dirname(os.path.abspath(__file__))
+CASES = {"isentropic-vortex": "2D/euler/isentropic-vortex"}
+COMPRESS = ("bc.txt", "ic.tec", "thermo.dat", "transport.dat")
+
+
+def main():
+    for name, rel in CASES.items():
+        src, dst = os.path.join(REF, rel), os.path.join(HERE, name)
+        os.makedirs(os.path.join(dst, "INPUT"), exist_ok=True)
+        shutil.copyfile(os.path.join(src, "input.ini"), os.path.join(dst, "input.ini"))
+        for f in sorted(os.listdir(os.path.join(src, "INPUT"))):
+            a = os.path.join(src, "INPUT", f)
+            if f in COMPRESS:
+                with open(a, "rb") as fi, gzip.GzipFile(os.path.join(dst, "INPUT", f + ".gz"), "wb", 9, mtime=0) as fo:
+                    shutil.copyfileobj(fi, fo)
+            else:
+                shutil.copyfile(a, os.path.join(dst, "INPUT", f))
+        print(name, sorted(os.listdir(os.path.join(dst, "INPUT"))))
+
+
+if __name__ == "__main__":
+    main()

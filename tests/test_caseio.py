@@ -1,0 +1,130 @@
+"""The reference's own test-case inputs (tests/golden/cases, copied from test/2D/euler/isentropic-vortex by
+tests/golden/cases/make_cases.py) read by mosegpu.caseio, and the oracle run on the case that results.
+
+Pins: what the files state (64 x 64 x 1 cells, periodic MOSE 201 faces in x and y, MOSE 0 faces in the one-cell
+direction, HLLC + van Leer, SSP-RK3 at CFL 0.5, the constant-cp air table); exact conservation of mass and momentum
+on the periodic box (every face flux enters one cell and leaves another); the vortex drifting with the mean flow
+(1, 1); and the refusals of inputs this path does not run.
+"""
+import os
+import shutil
+
+import numpy as np
+import pytest
+
+from mosegpu import BC_CONNECTION, BC_NONE, caseio
+from mosegpu.cases import CP_AIR, R_AIR
+from oracle_bind import OracleCase
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+VORTEX = os.path.join(HERE, "golden", "cases", "isentropic-vortex")
+
+
+@pytest.fixture(scope="module")
+def vortex():
+    return caseio.load_case(VORTEX, nsteps=20)
+
+
+def test_vortex_inputs(vortex):
+    c = vortex
+    assert c.layout.dims == [(64, 64, 1)]
+    assert c.meta["ndir"] == 2
+    typ = c.records["typ"]
+    assert (typ == BC_CONNECTION).sum() == 4 * 64 and (typ == BC_NONE).sum() == 2 * 64 * 64
+    per = c.records[typ == BC_CONNECTION]
+    assert np.all(per["periodic"] == 1) and np.all(per["bs"] == 0)
+    # the donor of a face-1 cell is the face-2 cell of the same row, and the other way round
+    f1 = per[per["f"] == 1]
+    assert np.all(f1["fs"] == 2) and np.all(f1["is_"] == 64) and np.all(f1["js"] == f1["j"])
+    f3 = per[per["f"] == 3]
+    assert np.all(f3["fs"] == 4) and np.all(f3["js"] == 64) and np.all(f3["is_"] == f3["i"])
+    # records grouped by block, then face
+    assert np.all(np.diff(c.records["f"]) >= 0)
+    s = c.scheme
+    assert (s.limiter, s.riemann, s.n_rk, s.viscous, s.nrans, s.cfl) == (2, 1, 3, 0, 0, 0.5)
+    # thermo.dat: constant cp of air from T = 1 K, FLINT's row 0 = cp(1), h = 0
+    assert np.all(c.tables.cp[0] == CP_AIR) and c.tables.h[0, 0] == 0.0 and c.tables.h[0, 2] == 2 * CP_AIR
+    assert c.tables.wm[0] == 28.970418
+    # the box: 10 x 10 in x and y, one cell of 1e-3 in z
+    nd = c.nodes[0]
+    assert nd.shape == (2, 65, 65, 3)
+    assert nd[0, 0, 0, 0] == 0.0 and nd[0, 0, 64, 0] == 10.0 and abs(nd[1, 0, 0, 2] - nd[0, 0, 0, 2] - 1e-3) < 1e-9
+    V = c.layout.interior(c.P, 0)
+    assert V.shape == (5, 1, 64, 64)
+    assert np.all(V[3] == 0.0) and V[0].max() == 1.0 and V[4].max() == 1.0
+
+
+def test_vortex_oracle_conserves_and_advects(vortex):
+    c = vortex
+    vol = c.layout.interior(c.geom.vol[None, :], 0)[0]
+
+    def totals(P):
+        V = c.layout.interior(P, 0)
+        return np.array([np.sum(V[0] * vol), np.sum(V[0] * V[1] * vol), np.sum(V[0] * V[2] * vol)])
+
+    def centre(P):
+        # centroid of the density deficit, unwrapped about the initial centre (5, 5)
+        V = c.layout.interior(P, 0)[0, 0]
+        xc = (np.arange(64) + 0.5) * (10.0 / 64)
+        wgt = (1.0 - V).clip(min=0.0)
+        X, Y = np.meshgrid(xc, xc, indexing="xy")
+        return np.array([np.sum(wgt * X) / wgt.sum(), np.sum(wgt * Y) / wgt.sum()])
+
+    P = c.P.copy()
+    t0, c0 = totals(P), centre(P)
+    dts = OracleCase(c, nthreads=8).step(P, 20)
+    t1, c1 = totals(P), centre(P)
+    assert np.all(np.abs(t1 - t0) <= 1e-13 * np.abs(t0))
+    T = dts.sum()
+    assert np.all(np.abs((c1 - c0) - T) < 0.02 * 10.0 / 64 + 0.05 * T)      # moves with the mean flow (1, 1)
+    assert np.all(c.layout.interior(P, 0)[3] == 0.0)                          # no flux through the MOSE 0 faces
+    # the case is cold in these units: T = p / (rho R) < 1 K, the table's row 0 (as the reference reads it)
+    V = c.layout.interior(P, 0)
+    assert np.max(V[4] / (V[0] * R_AIR)) < 1.0
+
+
+@pytest.mark.parametrize("key,value,msg", [
+    ("time-accurate", "false", "local time stepping"),
+    ("riemann-solver", "SLAU", "SLAU"),
+    ("integration-variables", "prim", "integration-variables"),
+])
+def test_refusals(tmp_path, key, value, msg):
+    d = tmp_path / "case"
+    shutil.copytree(VORTEX, d)
+    ini = (d / "input.ini").read_text().splitlines()
+    out = [(f"{key} = {value}" if ln.split("=")[0].strip() == key else ln) for ln in ini]
+    if not any(ln.split("=")[0].strip() == key for ln in ini):
+        out = [ln + (f"\n{key} = {value}" if ln.strip() == "[MOSE-Numerics]" else "") for ln in out]
+    (d / "input.ini").write_text("\n".join(out) + "\n")
+    with pytest.raises(caseio.CaseFormatError, match=msg):
+        caseio.load_case(str(d))
+
+
+def test_unsupported_boundary_type(tmp_path):
+    d = tmp_path / "bc.txt"
+    d.write_text("       1       1       1       1       3     300\n")
+    with pytest.raises(caseio.CaseFormatError, match="300"):
+        caseio.read_bc_open(str(d))
+
+
+def test_tecplot_block_reader(tmp_path):
+    """A two-zone BLOCK file with nodal and cell-centred variables, several values per line."""
+    p = tmp_path / "ic.tec"
+    z1 = np.arange(3 * 2 * 2, dtype=float)
+    lines = ['VARIABLES = "x" "y" "z" "q"',
+             'ZONE T="a", I=3, J=2, K=2, DATAPACKING=BLOCK, VARLOCATION=([1-3]=NODAL,[4]=CELLCENTERED)']
+    for v in (z1, z1 + 100, z1 + 200):
+        lines.append(" ".join(repr(float(x)) for x in v))
+    lines.append("7.0 8.0")
+    lines.append('ZONE T="b", I=2, J=2, K=2, DATAPACKING=BLOCK, VARLOCATION=([1-3]=NODAL,[4-4]=CELLCENTERED)')
+    for v in range(3):
+        lines.append(" ".join(["1.5"] * 8))
+    lines.append("9.0")
+    p.write_text("\n".join(lines) + "\n")
+    zones = caseio.read_tecplot_block(str(p))
+    assert len(zones) == 2
+    names, I, J, K, data = zones[0]
+    assert (I, J, K) == (3, 2, 2) and names == ["x", "y", "z", "q"]
+    assert data["x"].shape == (2, 2, 3) and data["x"][1, 0, 2] == 8.0 and data["y"][0, 1, 0] == 103.0
+    assert data["q"].shape == (1, 1, 2) and list(data["q"].ravel()) == [7.0, 8.0]
+    assert zones[1][4]["q"].ravel().tolist() == [9.0]
