@@ -151,15 +151,9 @@ def _parse_varlocation(header: str, nvar: int) -> list[bool]:
 
 
 def _zone_int(header: str, key: str) -> int:
-    up = header.upper().replace(" ", "")
-    p = up.find(key + "=")
-    if p < 0:
-        return 1
-    q = p + len(key) + 1
-    e = q
-    while e < len(up) and up[e].isdigit():
-        e += 1
-    return int(up[q:e])
+    """The integer of `key=` in a ZONE header (a whole key: I= does not match inside another word); 1 if absent."""
+    m = re.search(r"(?:^|[,\s])" + key + r"\s*=\s*(\d+)", header, re.IGNORECASE)
+    return int(m.group(1)) if m else 1
 
 
 def read_tecplot_block(path: str):
@@ -276,7 +270,7 @@ def mesh_ndir(dims, nodes0) -> int:
     t2 = ang(nd[1, n2, 0]) - ang(nd[0, n2, 0])
     r0 = math.hypot(nd[0, 1, 0][1], nd[0, 1, 0][2])
     r1 = math.hypot(nd[1, 1, 0][1], nd[1, 1, 0][2])
-    if (t1 - t2) < 1e-5 and r0 > 1e-10 and abs(r0 - r1) < 1e-5 * r0:
+    if (t1 - t2) < 1e-5 and r0 > 1e-10 and abs(r0 - r1) < 1e-5 * r0 and t1 != 0.0:
         raise CaseFormatError("axisymmetric 2D mesh: not on this path")
     return 2
 
