@@ -217,7 +217,7 @@ def main():
                     "plain per-block path (the comparison path)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (the multi-GPU sharding "
+        print(f"bench.py: this benchmark runs on one GPU; --gpus {args.gpus} is not supported (the multi-GPU sharding "
               f"of the residual path is a design note, not code)", file=sys.stderr)
         sys.exit(2)
 

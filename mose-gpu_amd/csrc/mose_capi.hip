@@ -479,7 +479,7 @@ extern "C" int mose_case_create(mose_ctx* ctx, const mose_scheme* sc, int nb, co
   if (sc->nsc != ctx->nsc) return fail(ctx, MOSE_ERR_ARG, "the scheme's nsc differs from the context tables");
   if (sc->viscous && !ctx->viscous_tables) return fail(ctx, MOSE_ERR_ARG, "viscous case needs mu and k tables");
   if (!((sc->nsc == 1 && sc->nrans == 0) || (sc->nsc == 1 && sc->nrans == 1)))
-    return fail(ctx, MOSE_ERR_ARG, "this build instantiates the residual kernels for nsc = 1 (nrans 0 or 1)");
+    return fail(ctx, MOSE_ERR_ARG, "the library instantiates the residual kernels for nsc = 1 (nrans 0 or 1)");
   HIPCHK(ctx, hipSetDevice(ctx->device));
   rc = alloc_case(ctx, *c);
   if (rc != MOSE_OK) { case_free(c.get()); return rc; }
@@ -1155,7 +1155,7 @@ static void chem_free(ChemObj* o) {
 
 extern "C" int mose_chem_create(mose_ctx* ctx, const mose_mechanism* m, int64_t ncell_max, int* chem_id) {
   if (!ctx || !m || !chem_id || ncell_max < 1) return MOSE_ERR_ARG;
-  if (m->ns != 9) return fail(ctx, MOSE_ERR_ARG, "this build instantiates the chemistry kernel for 9 species");
+  if (m->ns != 9) return fail(ctx, MOSE_ERR_ARG, "the library instantiates the chemistry kernel for 9 species");
   if (m->tmax <= m->tmin || m->n_arrh < 0 || m->n_troe < 0 || m->n_lind < 0 || !m->wm || !m->cp || !m->h ||
       !m->ni1 || !m->ni2 || !m->eps || (m->n_arrh > 0 && (!m->kf || !m->kb)) || (m->n_troe > 0 && !m->troe) ||
       (m->n_lind > 0 && !m->lind))
