@@ -1,5 +1,5 @@
 """Case directories in the reference's own input formats (test/<dim>/<case>/{input.ini, INPUT/*}), read into a Case
-that this library runs.
+that this library runs, and its solution written in the reference's output formats (OUTPUT/field.tec, field.bin).
 
   input.ini      the MOSE_open dialect: [MOSE-Physics], [MOSE-Numerics], [MOSE-Parameters] (IO_Ini_Open.f90:60-320)
   phase.txt      a header line, then one "name molecular-weight" line per species (Load_ThermoTransport.f90:7-60)
@@ -357,3 +357,141 @@ def load_case(case_dir: str, nsteps: int | None = None) -> Case:
         nsteps = int(float(par.get("iter-threshold", "0")))
     return Case(os.path.basename(os.path.normpath(case_dir)), tables, sch, layout, nodes, rec, geom, P, nsteps,
                 meta={"ini": ini, "ndir": ndir, "time_threshold": float(par.get("time-threshold", "0") or 0)})
+
+
+# ------------------------------------------------------------------------------------------------- output files
+def fortran_e23_15e3(x: float) -> str:
+    """One value as Fortran's E23.15E3 edit descriptor writes it (ORION's FR8P): a 0.d...d mantissa of 15 digits,
+    a signed 3-digit exponent, right-justified in 23 columns."""
+    x = float(x)
+    if x != x:
+        return "NaN".rjust(23)
+    if x in (math.inf, -math.inf):
+        return ("Infinity" if x > 0 else "-Infinity").rjust(23)
+    if x == 0.0:
+        body, e = "0." + "0" * 15, 0
+    else:
+        s = "%.14e" % abs(x)                       # d.dddddddddddddde+XX, correctly rounded to 15 digits
+        body, e = "0." + s[0] + s[2:16], int(s[17:]) + 1
+    sign = "-" if (x < 0.0 or (x == 0.0 and math.copysign(1.0, x) < 0.0)) else ""
+    return (sign + body + "E" + ("+" if e >= 0 else "-") + "%03d" % abs(e)).rjust(23)
+
+
+def _str_r8p_no_sign(x: float) -> str:
+    """ORION's str(no_sign=.true., n=x) for a real(8): the E23.15E3 string without its first column."""
+    s = fortran_e23_15e3(x)
+    if x > 0.0:
+        s = "+" + s[1:]
+    return s[1:].strip()
+
+
+def _output_names(case: Case) -> list[str]:
+    """Output_Solution_Setup (IO_Solution.f90:346-378): rho(s), u, v, w, p, the RANS variable, then T."""
+    names = [f'"rho({s + 1})"' for s in range(case.scheme.nsc)] + ['"u"', '"v"', '"w"', '"p"']
+    if case.scheme.nrans == 1:
+        names.append('"mi_tilde"')
+    return names + ['"T"']
+
+
+def _cell_temperature(case: Case, V: np.ndarray) -> np.ndarray:
+    """T = p / (rho R) with rho = sum rho_s and R = sum R_s rho_s / rho, summed in species order (write_vtk_tec)."""
+    nsc = case.scheme.nsc
+    rho = V[0].copy()
+    for s in range(1, nsc):
+        rho = rho + V[s]
+    R = np.zeros_like(rho)
+    for s in range(nsc):
+        R = R + (RUNIV / case.tables.wm[s]) * V[s] / rho
+    return V[nsc + 3] / (rho * R)
+
+
+def write_field_tec(case: Case, P: np.ndarray, path: str, time: float, time_accurate: bool = True, iteration: int = 0):
+    """OUTPUT/field.tec as write_vtk_tec + tec_write_structured_multiblock (IO_Solution.f90:507-584,
+    Lib_Tecplot.f90:71-354) write it in ASCII: the variable line, then per block a zone header and, one value per
+    line, the nodal x, y, z and the cell-centred primitives and T.  A steady run writes -iteration as its solution
+    time and marks the zone STRANDID = 0 (Wrap_Postprocess.f90:43-50)."""
+    names = _output_names(case)
+    nvar = len(names)
+    stime = time if time_accurate else float(iteration)
+    lines = [' VARIABLES ="x" "y" "z" ' + " ".join(names)]
+    for b, (n1, n2, n3) in enumerate(case.layout.dims):
+        head = (f" ZONE  T = Block{b + 1}, I={n1 + 1}, J={n2 + 1}, K={n3 + 1}, DATAPACKING=BLOCK"
+                f", VARLOCATION=([1-3]=NODAL,[4-{3 + nvar}]=CELLCENTERED), SOLUTIONTIME={_str_r8p_no_sign(stime)}")
+        if not time_accurate:
+            head += ", STRANDID = 0"
+        lines.append(head)
+        nd = case.nodes[b]
+        for a in range(3):
+            lines.extend(fortran_e23_15e3(v) for v in nd[..., a].reshape(-1))
+        V = case.layout.interior(P, b)
+        for s in range(case.nprim):
+            lines.extend(fortran_e23_15e3(v) for v in V[s].reshape(-1))
+        lines.extend(fortran_e23_15e3(v) for v in _cell_temperature(case, V).reshape(-1))
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def _record(f, arr: np.ndarray):
+    data = np.ascontiguousarray(arr).astype(arr.dtype.newbyteorder(">"), copy=False).tobytes()
+    mark = np.array([len(data)], dtype=">i4").tobytes()
+    f.write(mark + data + mark)
+
+
+def write_native(case: Case, P: np.ndarray, path: str, time: float):
+    """OUTPUT/field.bin as write_native (IO_Solution.f90:450-500): a big-endian Fortran unformatted sequential file
+    (4-byte record markers).  Records: nb; dim(1), dim(2), dim(3) one block per record; the species of each block
+    with s fastest; each of u, v, w, p over the blocks; the RANS variables of each block with s fastest; the time."""
+    dims = case.layout.dims
+    nsc, nrans = case.scheme.nsc, case.scheme.nrans
+    with open(path, "wb") as f:
+        _record(f, np.array([len(dims)], dtype=np.int32))
+        for a in range(3):
+            for d in dims:
+                _record(f, np.array([d[a]], dtype=np.int32))
+        for b in range(len(dims)):
+            V = case.layout.interior(P, b)                       # (nprim, n3, n2, n1)
+            _record(f, np.moveaxis(V[:nsc], 0, -1).reshape(-1))  # s fastest, then i, j, k
+        for s in range(nsc, nsc + 4):
+            for b in range(len(dims)):
+                _record(f, case.layout.interior(P, b)[s].reshape(-1))
+        if nrans > 0:
+            for b in range(len(dims)):
+                V = case.layout.interior(P, b)
+                _record(f, np.moveaxis(V[nsc + 4:], 0, -1).reshape(-1))
+        _record(f, np.array([time], dtype=np.float64))
+
+
+def read_native(case: Case, path: str) -> tuple[np.ndarray, float]:
+    """The state and time of a field.bin written by write_native (or by the reference's write_native) on the case's
+    blocks; the ghost rows are filled from the nearest interior cell."""
+    with open(path, "rb") as f:
+        raw = f.read()
+    recs, q = [], 0
+    while q < len(raw):
+        n = int(np.frombuffer(raw, ">i4", 1, q)[0])
+        recs.append(raw[q + 4:q + 4 + n])
+        q += 8 + n
+    dims = case.layout.dims
+    nb = int(np.frombuffer(recs[0], ">i4")[0])
+    got = [tuple(int(np.frombuffer(recs[1 + a * nb + b], ">i4")[0]) for a in range(3)) for b in range(nb)]
+    if nb != len(dims) or [tuple(d) for d in dims] != got:
+        raise CaseFormatError(f"{path}: blocks {got} do not match the case's {dims}")
+    nsc, nrans = case.scheme.nsc, case.scheme.nrans
+    P = np.zeros((case.nprim, case.layout.NC))
+    r = 1 + 3 * nb
+    for b, (n1, n2, n3) in enumerate(dims):
+        sp = np.frombuffer(recs[r], ">f8").reshape(n3, n2, n1, nsc)
+        case.layout.interior(P, b)[:nsc] = np.moveaxis(sp, -1, 0)
+        r += 1
+    for s in range(nsc, nsc + 4):
+        for b, (n1, n2, n3) in enumerate(dims):
+            case.layout.interior(P, b)[s] = np.frombuffer(recs[r], ">f8").reshape(n3, n2, n1)
+            r += 1
+    if nrans > 0:
+        for b, (n1, n2, n3) in enumerate(dims):
+            tv = np.frombuffer(recs[r], ">f8").reshape(n3, n2, n1, nrans)
+            case.layout.interior(P, b)[nsc + 4:] = np.moveaxis(tv, -1, 0)
+            r += 1
+    time = float(np.frombuffer(recs[r], ">f8")[0])
+    fill_ghosts_nearest(case.layout, P)
+    return P, time

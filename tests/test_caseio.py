@@ -161,3 +161,58 @@ def test_none_faces_conserve_mass():
     assert abs(m1 - m0) <= 1e-13 * m0
     # and the faces are not inert: the state changed
     assert np.max(np.abs(c.layout.interior(P, 0)[1] - c.layout.interior(c.P, 0)[1])) > 0.0
+
+
+def test_e23_15e3_matches_fortran(tmp_path):
+    """The output formatter against Fortran's own E23.15E3 (amdflang), on seeded values over the whole range."""
+    import subprocess
+    fc = shutil.which("amdflang") or ("/opt/rocm/bin/amdflang" if os.path.exists("/opt/rocm/bin/amdflang") else None)
+    if fc is None:
+        pytest.skip("no Fortran compiler")
+    rng = np.random.Generator(np.random.PCG64(5))
+    vals = np.concatenate([rng.uniform(-1, 1, 300) * 10.0 ** rng.integers(-300, 300, 300), rng.uniform(0, 1, 100),
+                           [0.0, 1.0, -1.0, 0.1, 1 / 3, 5e-324, 1.7976931348623157e308, 2.5, 0.05, 9.9999999999999995]])
+    (tmp_path / "v.txt").write_text("\n".join(repr(float(v)) for v in vals) + "\n")
+    (tmp_path / "f.f90").write_text(
+        "program f\n  implicit none\n  real(8) :: v\n  integer :: ios\n  open(10, file='v.txt')\n  do\n"
+        "    read(10, *, iostat=ios) v\n    if (ios /= 0) exit\n    write(*, '(E23.15E3)') v\n  end do\nend program\n")
+    subprocess.run([fc, "-O0", "f.f90", "-o", "f"], cwd=tmp_path, check=True, capture_output=True)
+    out = subprocess.run(["./f"], cwd=tmp_path, check=True, capture_output=True, text=True).stdout.splitlines()
+    assert out == [caseio.fortran_e23_15e3(v) for v in vals]
+
+
+def test_field_tec_layout(tmp_path, vortex):
+    c = vortex
+    P = c.P.copy()
+    p = tmp_path / "field.tec"
+    caseio.write_field_tec(c, P, str(p), time=0.25)
+    lines = p.read_text().splitlines()
+    assert lines[0] == ' VARIABLES ="x" "y" "z" "rho(1)" "u" "v" "w" "p" "T"'
+    assert lines[1] == (" ZONE  T = Block1, I=65, J=65, K=2, DATAPACKING=BLOCK, "
+                        "VARLOCATION=([1-3]=NODAL,[4-9]=CELLCENTERED), SOLUTIONTIME=0.250000000000000E+000")
+    assert len(lines) == 2 + 3 * 65 * 65 * 2 + 6 * 64 * 64
+    assert all(len(ln) == 23 for ln in lines[2:])
+    names, I, J, K, data = caseio.read_tecplot_block(str(p))[0]
+    assert (I, J, K) == (65, 65, 2)
+    V = c.layout.interior(P, 0)
+    for a, nm in enumerate("xyz"):
+        assert np.allclose(data[nm], c.nodes[0][..., a], rtol=1e-14, atol=0)
+    for s, nm in enumerate(["rho(1)", "u", "v", "w", "p"]):
+        assert np.allclose(data[nm], V[s], rtol=1e-14, atol=0)
+    assert np.allclose(data["T"], V[4] / (V[0] * R_AIR), rtol=1e-14, atol=0)
+    caseio.write_field_tec(c, P, str(p), time=0.0, time_accurate=False, iteration=7)
+    assert p.read_text().splitlines()[1].endswith("SOLUTIONTIME=0.700000000000000E+001, STRANDID = 0")
+
+
+def test_native_roundtrip(tmp_path, vortex):
+    c = vortex
+    P = c.P.copy()
+    P[1] += 1e-3 * np.sin(np.arange(P.shape[1]))
+    p = tmp_path / "field.bin"
+    caseio.write_native(c, P, str(p), time=1.5)
+    raw = p.read_bytes()
+    # record 1: nb = 1 between 4-byte big-endian markers
+    assert raw[:12] == (4).to_bytes(4, "big") + (1).to_bytes(4, "big") + (4).to_bytes(4, "big")
+    Q, t = caseio.read_native(c, str(p))
+    assert t == 1.5
+    assert np.array_equal(c.layout.interior(Q, 0), c.layout.interior(P, 0))
