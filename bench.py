@@ -168,6 +168,41 @@ def radau5_leg(M, st, cpu_cells: int, threads: int):
     return res
 
 
+def lattice_leg(steps: int, warmup: int = 2):
+    """The 4 x 4 x 4 lattice of blocks with seeded uneven widths in [24, 96] per axis (cases.channel_lattice, seed 7),
+    one-to-one interfaces, an adiabatic no-slip wall at y = 0, far field on the other outer faces, M 0.3, Re 1e6,
+    Spalart-Allmaras, LMRoe (Mco = 1) + van Albada, SSP-RK3: the fused stage kernel over the tiles of every block
+    against the plain per-block launch path, on the same start state; the two final state hashes must be equal."""
+    import mosegpu
+    from mosegpu import PATH_FUSED, PATH_PLAIN, cases
+    case = cases.channel_lattice(riemann="LMRoe", nsteps=steps)
+    cells = case.interior_cells()
+    out = {"blocks": len(case.layout.dims), "cells": cells, "steps": steps,
+           "workload": "4x4x4 block lattice, widths U{24..96} (seed 7), wall + far field, SA, M 0.3, Re 1e6, "
+                       "LMRoe + van Albada, SSP-RK3"}
+    hashes = {}
+    for name, path in (("fused", PATH_FUSED), ("plain", PATH_PLAIN)):
+        ctx = mosegpu.Context(case.tables)
+        dc = mosegpu.DeviceCase(ctx, case)
+        dc.upload(case.P)
+        chosen = dc.tune(path, (0, 0, 0), timing=False)
+        dc.step(warmup)
+        dc.step(0)
+        t0 = time.perf_counter()
+        dc.step(steps)
+        el = time.perf_counter() - t0
+        hashes[name] = dc.state_hash()
+        out[name] = {"cell_updates_per_s": cells * steps / el, "ms_per_step": 1e3 * el / steps,
+                     "tile": list(chosen) if path == PATH_FUSED else None}
+        dc.destroy()
+        ctx.close()
+    out["fused_over_plain"] = out["fused"]["cell_updates_per_s"] / out["plain"]["cell_updates_per_s"]
+    out["same_state"] = hashes["fused"] == hashes["plain"]
+    print(f"bench: lattice {cells} cells in 64 blocks: fused {out['fused']['ms_per_step']:.2f} ms/step, plain "
+          f"{out['plain']['ms_per_step']:.2f} ms/step, same state {out['same_state']}", file=sys.stderr, flush=True)
+    return out
+
+
 def serving_leg(count: int, plain: bool):
     """A Poisson stream (20 cases/s) of 2D four-quadrant Riemann (n log-uniform in [64, 1024]) and 3D TGV (n
     log-uniform in [48, 192]) cases of 50-500 steps, served with continuous batching; p50/p99 time-to-solution."""
@@ -209,6 +244,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--probe-bw", action="store_true", help="only run the stream-copy probe (1 GiB buffers)")
     ap.add_argument("--serve", type=int, default=16, help="requests of the serving leg (0: skip)")
+    ap.add_argument("--lattice", type=int, default=20, help="timed steps of the multi-block lattice leg (0: skip)")
     ap.add_argument("--chem", type=int, default=262144, help="cells of the chemistry leg (0: skip)")
     ap.add_argument("--chem-cpu", type=int, default=2048, help="cells of the chemistry CPU baseline sample")
     ap.add_argument("--chem-radau", type=int, default=65536, help="cells of the RADAU5 chemistry leg (0: skip)")
@@ -278,6 +314,8 @@ def main():
     }
     dc.destroy()
     ctx.close()
+    if args.lattice > 0:
+        out["lattice"] = lattice_leg(args.lattice)
     if args.serve > 0:
         out["serving"] = serving_leg(args.serve, args.serve_plain)
     if args.chem > 0:
