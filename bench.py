@@ -78,7 +78,7 @@ def oracle_rate(n: int, steps: int, threads: int) -> tuple[float, float]:
 
 def cpu_baseline(n: int, steps: int, threads: int):
     rate, dt = oracle_rate(n, steps, threads)
-    n1, s1 = 32, 16
+    n1, s1 = 40, 32
     rate1, dt1 = oracle_rate(n1, s1, 1)
     return {"value": rate, "unit": "cell-updates/s", "cores": threads, "kind": "port",
             "sample": f"taylor-green {n}^3, {steps} SSP-RK3 steps after 1 warm-up step, CPU oracle "
