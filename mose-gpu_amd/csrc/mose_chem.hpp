@@ -253,9 +253,9 @@ __device__ __forceinline__ double team_rhs(const ChemDev& M, const RxnLds& X, co
   if (T < (double)M.tmin || T >= (double)(M.tmin + M.rows - 1) || T != T) return -1.0;
   const int Ti = static_cast<int>(T);
   const double Tf = T - (double)Ti;
-  double coi[NS];
-#pragma unroll
-  for (int s = 0; s < NS; ++s) coi[s] = L.c[s];
+  // the concentrations are read from the team's LDS copy where they are used: a register copy of all NS of them
+  // stayed live through the reaction loop and pushed the kernel into 27 more spilled VGPRs
+  const double* coi = L.c;
   const int nr = M.na + M.nt + M.nl;
 #pragma unroll 1
   for (int q0 = 0; q0 < nr; q0 += NZ) {
