@@ -127,7 +127,8 @@ __device__ __forceinline__ int tshi(int v, int src) { return __shfl(v, src, 64);
 // (nu'(q, ns+1) > 0) and the third-body efficiencies.
 struct RxnLds {
   unsigned long long o1[NRMAX], o2[NRMAX];
-  int flags[NRMAX];          // bit 0: third body; bit 1: the orders are packed in o1 / o2
+  unsigned int c1[NRMAX], c2[NRMAX];   // the nonzero orders of each side as bytes (species << 4 | order), ascending
+  int flags[NRMAX];          // bit 0: third body; bit 1: the orders are packed in o1 / o2; bit 2: and in c1 / c2
   double eps[NRMAX * SLOT];  // eps(q, s) at [q * SLOT + s]
 };
 
@@ -138,6 +139,7 @@ __device__ __forceinline__ void rxn_build(const ChemDev& M, RxnLds& X) {
   for (int q = threadIdx.x; q < NRMAX; q += blockDim.x) {
     int fl = 0;
     unsigned long long o1 = 0ull, o2 = 0ull;
+    unsigned int c1 = 0u, c2 = 0u;
     if (q < nr) {
       const double* n1 = M.ni1 + (size_t)q * W;
       const double* n2 = M.ni2 + (size_t)q * W;
@@ -153,9 +155,19 @@ __device__ __forceinline__ void rxn_build(const ChemDev& M, RxnLds& X) {
         }
       }
       if (packed) fl |= 2;
+      // at most four species with a nonzero order on each side: the byte lists (SLOT = 16 species fit 4 bits)
+      int k1 = 0, k2 = 0;
+      for (int s = 0; packed && s < NS; ++s) {
+        const int a = (int)((o1 >> (4 * s)) & 15ull), b = (int)((o2 >> (4 * s)) & 15ull);
+        if (a != 0) { if (k1 < 4) c1 |= (unsigned)(s << 4 | a) << (8 * k1); ++k1; }
+        if (b != 0) { if (k2 < 4) c2 |= (unsigned)(s << 4 | b) << (8 * k2); ++k2; }
+      }
+      if (packed && k1 <= 4 && k2 <= 4) fl |= 4;
     }
     X.o1[q] = o1;
     X.o2[q] = o2;
+    X.c1[q] = c1;
+    X.c2[q] = c2;
     X.flags[q] = fl;
   }
   for (int i = threadIdx.x; i < NRMAX * SLOT; i += blockDim.x) {
@@ -182,7 +194,16 @@ __device__ __forceinline__ double net_rate(const ChemDev& M, const RxnLds& X, in
   const double* n1 = M.ni1 + (size_t)q * W;
   const double* n2 = M.ni2 + (size_t)q * W;
   double pf = 1.0, pr = 1.0, rf, rr;
-  if (fl & 2) {
+  if (fl & 4) {
+    // the same products as the loop below, over the nonzero orders only, in the same species order
+    const unsigned c1 = X.c1[q], c2 = X.c2[q];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const unsigned v = (c1 >> (8 * e)) & 255u, w = (c2 >> (8 * e)) & 255u;
+      if (v != 0u) pf = pf * chem_ipow(coi[v >> 4], (int)(v & 15u));
+      if (w != 0u) pr = pr * chem_ipow(coi[w >> 4], (int)(w & 15u));
+    }
+  } else if (fl & 2) {
     const unsigned long long o1 = X.o1[q], o2 = X.o2[q];
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
