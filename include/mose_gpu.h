@@ -203,7 +203,7 @@ int mose_case_state_hash(mose_ctx* ctx, int case_id, uint64_t* h_lo, uint64_t* h
 
 /* Execution path and tile shape of a case.  path: MOSE_PATH_FUSED (default) runs one fused stage kernel per RK
  * stage over the tiles of every block; MOSE_PATH_PLAIN runs the per-block face kernels + cell update (the
- * per-block launch structure of the reference, kept as the comparison path).  A tile holds 256 or 512 cells
+ * per-block launch structure of the reference, kept as the comparison path).  A tile holds 128, 256 or 512 cells
  * (ti * tj * tk); ti = tj = tk = 0 autotunes the tile (each candidate timed on the case itself, the state left
  * unchanged).  timing = 1 records HIP events around
  * every stage kernel of the following step calls (mose_case_timing).  No reference counterpart (launch tuning). */

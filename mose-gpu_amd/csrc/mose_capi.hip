@@ -615,8 +615,8 @@ static size_t stage_lds_(int nprim, bool visc, int TI, int TJ, int TK) {
 }
 
 static int set_tiles(mose_ctx* ctx, Case& c, int TI, int TJ, int TK) {
-  if (TI < 1 || TJ < 1 || TK < 1 || (TI * TJ * TK != 256 && TI * TJ * TK != 512))
-    return fail(ctx, MOSE_ERR_ARG, "a tile holds 256 or 512 cells");
+  if (TI < 1 || TJ < 1 || TK < 1 || (TI * TJ * TK != 128 && TI * TJ * TK != 256 && TI * TJ * TK != 512))
+    return fail(ctx, MOSE_ERR_ARG, "a tile holds 128, 256 or 512 cells");
   if (stage_lds(c, TI, TJ, TK) + kStaticLds > (size_t)ctx->max_lds)
     return fail(ctx, MOSE_ERR_ARG, "tile " + std::to_string(TI) + "x" + std::to_string(TJ) + "x" + std::to_string(TK) +
                                        " needs " + std::to_string(stage_lds(c, TI, TJ, TK)) + " B of LDS, the device allows " +
@@ -663,16 +663,17 @@ static void launch_stage_sh(Case& c, const CaseDev& D, hipStream_t s, const Stag
   }
 }
 
-// Every tile runs as one workgroup of kStageThreads threads: two per cell of a 256-cell tile, one per cell of a
-// 512-cell tile.  The kernel takes 127 VGPRs, so two 256-cell tiles (76 KB of LDS each at nprim 5, viscous) run on
-// a CU at 4 waves per SIMD; with one thread per cell they ran at 2, and the stage took 16 % longer (7.22 against
-// 5.96 ms on the 256^3 Taylor-Green case).  The two preferred tile shapes run kernels with the shape fixed at compile
-// time; the others read it at run time.
-static constexpr int kStageThreads = 512;
+// Every tile of 256 or 512 cells runs as one workgroup of 512 threads: two per cell of a 256-cell tile, one per cell
+// of a 512-cell tile.  The kernel takes 127 VGPRs, so two 256-cell tiles (76 KB of LDS each at nprim 5, viscous) run
+// on a CU at 4 waves per SIMD; with one thread per cell they ran at 2, and the stage took 16 % longer (7.22 against
+// 5.96 ms on the 256^3 Taylor-Green case).  A 128-cell tile runs as 256 threads (two per cell): with RANS (nprim 6,
+// 149 VGPRs) three of its 48 KB workgroups share a CU, where a 256-cell tile's 92 KB leaves one.  The two preferred
+// tile shapes run kernels with the shape fixed at compile time; the others read it at run time.
 template <int NS, int NR>
 static void launch_stage_bs(Case& c, const CaseDev& D, hipStream_t s, const StageArgs& A, unsigned grid, size_t lds) {
-  constexpr int BS = kStageThreads;
-  if (c.TI == 8 && c.TJ == 8 && c.TK == 4) launch_stage_sh<NS, NR, BS, 1>(c, D, s, A, grid, lds);
+  constexpr int BS = 512;
+  if (c.TI * c.TJ * c.TK == 128) launch_stage_sh<NS, NR, 256, 0>(c, D, s, A, grid, lds);
+  else if (c.TI == 8 && c.TJ == 8 && c.TK == 4) launch_stage_sh<NS, NR, BS, 1>(c, D, s, A, grid, lds);
   else if (c.TI == 8 && c.TJ == 8 && c.TK == 8) launch_stage_sh<NS, NR, BS, 2>(c, D, s, A, grid, lds);
   else launch_stage_sh<NS, NR, BS, 0>(c, D, s, A, grid, lds);
 }
@@ -790,11 +791,11 @@ static int run_steps(mose_ctx* ctx, Case& c, int nsteps) {
 
 template <int NS, int NR>
 static int autotune(mose_ctx* ctx, Case& c, mose_tuning* chosen) {
-  // candidates of 256 and 512 cells; each runs the ghost, record and stage kernels of stage 1 (into the scratch
+  // candidates of 128, 256 and 512 cells; each runs the ghost, record and stage kernels of stage 1 (into the scratch
   // buffer PA, the state P0 untouched) a few times, the fastest is kept
   static const int cand[][3] = {{32, 4, 2}, {64, 2, 2}, {16, 4, 4}, {16, 8, 2}, {8, 8, 4}, {64, 4, 1}, {32, 8, 1},
                                 {128, 2, 1}, {256, 1, 1}, {8, 8, 8}, {16, 8, 4}, {16, 4, 8}, {32, 4, 4}, {16, 16, 2},
-                                {32, 16, 1}, {64, 8, 1}};
+                                {32, 16, 1}, {64, 8, 1}, {8, 4, 4}, {16, 4, 2}, {16, 8, 1}};
   hipStream_t s = c.stream;
   hipEvent_t e0, e1;
   HIPCHK(ctx, hipEventCreate(&e0));

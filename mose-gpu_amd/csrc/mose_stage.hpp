@@ -1,6 +1,6 @@
-// The fused stage kernel: one workgroup of 512 threads per tile of TI x TJ x TK cells (256 or 512 cells: two threads
-// per cell of a 256-cell tile) of any block of the case.  The face and half-slope loops spread over all 512
-// threads; the per-cell phases run on the first TI TJ TK.
+// The fused stage kernel: one workgroup per tile of TI x TJ x TK cells of any block of the case: 512 threads for a
+// tile of 256 or 512 cells, 256 threads for a tile of 128 (two threads per cell of the smaller tiles).  The face and
+// half-slope loops spread over all the threads; the per-cell phases run on the first TI TJ TK.
 //
 //   phase A  every interior face of the tile (TI+1)TJTK + TI(TJ+1)TK + TITJ(TK+1): the convective flux into LDS;
 //            the per-cell (rho, 1/rho, T) and the primitives of the tile and its one-cell halo into LDS (viscous

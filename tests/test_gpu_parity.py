@@ -119,7 +119,7 @@ def run_fused(case, nsteps, tile):
     return dc.download(), rep, chosen
 
 
-@pytest.mark.parametrize("tile", [(32, 4, 2), (16, 4, 4), (64, 2, 2), (0, 0, 0), (8, 8, 8), (32, 4, 4)])
+@pytest.mark.parametrize("tile", [(32, 4, 2), (16, 4, 4), (64, 2, 2), (0, 0, 0), (8, 8, 8), (32, 4, 4), (8, 4, 4)])
 def test_fused_taylor_green_parity(gpu_ready, tile):
     case = cases.taylor_green(20, 6, dims=(20, 18, 21))
     case.meta["gamma"] = 1004.5 / (1004.5 - 8314.46261815324 / 28.970418)

@@ -44,7 +44,7 @@ def run_device(case, nsteps, path=mosegpu.PATH_FUSED, tile=(0, 0, 0)):
 
 @pytest.mark.parametrize("wall", [BC_WALL_ADIABATIC, BC_WALL_ISOTHERMAL])
 @pytest.mark.parametrize("path,tile", [(mosegpu.PATH_FUSED, (0, 0, 0)), (mosegpu.PATH_FUSED, (16, 4, 4)),
-                                       (mosegpu.PATH_PLAIN, (0, 0, 0))])
+                                       (mosegpu.PATH_FUSED, (8, 4, 4)), (mosegpu.PATH_PLAIN, (0, 0, 0))])
 def test_channel_lattice_small(gpu_ready, wall, path, tile):
     case = cases.channel_lattice(nb=(2, 2, 2), wmin=4, wmax=9, seed=11, nsteps=5, wall=wall, T_wall=280.0)
     Pg, rep, chosen = run_device(case, 5, path, tile)
