@@ -501,6 +501,8 @@ extern "C" int mose_case_create(mose_ctx* ctx, const mose_scheme* sc, int nb, co
     HIPCHK(ctx, hipMemcpyAsync(c->dff, c->ff.data(), c->ff.size() * sizeof(mose_farfield), hipMemcpyHostToDevice, s));
   HIPCHK(ctx, hipMemcpyAsync(c->dbtyp, c->btyp.data(), c->btyp.size(), hipMemcpyHostToDevice, s));
   HIPCHK(ctx, hipMemsetAsync(c->P0, 0, 3 * (size_t)c->NC * c->nprim * d, s));
+  // the boundary planes start at zero: the record kernel leaves the entries of no-flux (MOSE 0) faces untouched
+  HIPCHK(ctx, hipMemsetAsync(c->binc, 0, 2 * (size_t)std::max<int64_t>(c->NBP, 1) * c->nprim * d, s));
   HIPCHK(ctx, hipStreamSynchronize(s));
   rc = ensure_rpart(ctx, *c, plain_parts(*c));
   if (rc != MOSE_OK) { case_free(c.get()); return rc; }

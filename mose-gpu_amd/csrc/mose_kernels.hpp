@@ -121,6 +121,7 @@ __global__ void __launch_bounds__(128) k_bcrec(CaseDev C) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= C.nrec) return;
   const mose_bc_record r = C.rec[e];
+  if (r.typ == MOSE_BC_NONE) return;   // no flux: its plane entries stay at the zeros of the case's creation
   const mose_blk B = C.blk[r.b];
   const mose_dtab& T = *C.tab;
   const double* P = C.Pin;

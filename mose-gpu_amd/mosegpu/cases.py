@@ -18,7 +18,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .core import (BC_CONNECTION, BC_EXTRAPOLATION, BC_FARFIELD, BC_WALL_ADIABATIC, BC_WALL_ISOTHERMAL,
+from .core import (BC_CONNECTION, BC_EXTRAPOLATION, BC_NONE, BC_FARFIELD, BC_WALL_ADIABATIC, BC_WALL_ISOTHERMAL,
                    FARFIELD_DTYPE, GC, LIM, NTAB, RECORD_DTYPE, RS, Geometry, Layout, Scheme, ThermoTables,
                    metrics_build, records_tangent)
 
@@ -354,13 +354,16 @@ def riemann2d(n: int = 256, nsteps: int = 100, seed: int = 0, cfl: float = 0.5, 
               riemann="HLLC") -> Case:
     """2D Euler four-quadrant Riemann problem on [0, 1]^2 (one cell thick in z, extent 1/n): seeded quadrant states
     rho, p ~ U(0.1, 2), velocity of magnitude <= 1 at a seeded angle, split at (0.5, 0.5); the constant-cp air
-    table, extrapolation (MOSE 400) on every face, SSP-RK3, two computational directions."""
+    table, extrapolation (MOSE 400) on the four in-plane faces and no-flux faces (MOSE 0) in the one-cell direction,
+    as the reference's own 2D cases mark them (test/2D/euler/isentropic-vortex/INPUT/bc.txt), SSP-RK3, two
+    computational directions."""
     rng = np.random.Generator(np.random.PCG64(seed))
     tables = air_tables()
     dims = (n, n, 1)
     layout = Layout([dims])
     nodes = [box_nodes(n, n, 1, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0 / n))]
-    rec = block_records(0, dims, {f: BC_EXTRAPOLATION for f in range(1, 7)})
+    rec = block_records(0, dims, {1: BC_EXTRAPOLATION, 2: BC_EXTRAPOLATION, 3: BC_EXTRAPOLATION,
+                                  4: BC_EXTRAPOLATION, 5: BC_NONE, 6: BC_NONE})
     geom = metrics_build(layout, nodes, rec, ndir=2)
     sch = make_scheme(limiter=limiter, riemann=riemann, viscous=False, ndir=2, cfl=cfl)
     P = np.zeros((5, layout.NC))
