@@ -51,12 +51,15 @@ enum {
   MOSE_ERR_ODE = -8        /* a chemistry integration returned idid < 0 */
 };
 
-/* boundary record types (internal ids of Mod_BC.f90; MOSE types: 0/100 -> 0, 201/1xx -> 1, 400 -> 11, 301 -> 5,
- * 302 -> 6, 407 -> 4).  Type 0 is a face with no boundary condition (IO_BC_Open.f90:149-156): no flux is computed
- * there, and its ghost cells take the zero-gradient fill (Lib_Extrapolation.f90:590-607). */
+/* boundary record types (internal ids of Mod_BC.f90; MOSE types: 0/100 -> 0, 201/1xx -> 1, 300 -> 3, 400 -> 11,
+ * 301 -> 5, 302 -> 6, 407 -> 4).  Type 0 is a face with no boundary condition (IO_BC_Open.f90:149-156): no flux is
+ * computed there, and its ghost cells take the zero-gradient fill (Lib_Extrapolation.f90:590-607).  Type 3 is an
+ * Euler symmetry plane (IO_BC_Open.f90:191-193): BC_Symmetry_Eul and, when viscous, BC_Symmetry_Visc
+ * (Lib_BC.f90:640-861), behind the mirror ghost of Ghost_Symmetry (Lib_Extrapolation.f90:558-587). */
 enum {
   MOSE_BC_NONE = 0,
   MOSE_BC_CONNECTION = 1,
+  MOSE_BC_SYMMETRY = 3,
   MOSE_BC_FARFIELD = 4,
   MOSE_BC_WALL_ADIABATIC = 5,
   MOSE_BC_WALL_ISOTHERMAL = 6,

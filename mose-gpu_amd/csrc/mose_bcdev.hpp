@@ -1,8 +1,9 @@
 // Device twins of the wall and far-field boundary records and of the Spalart-Allmaras cell source.
 //
-//   bc_symmetry      BC_Symmetry_Eul (Lib_BC.f90:640-733): the wall-pressure flux of the no-slip walls (types 5, 6)
-//                    and of a far-field face that closes (type 4 with error = 1)
+//   bc_symmetry      BC_Symmetry_Eul (Lib_BC.f90:640-733): the pressure flux of the symmetry faces (type 3), of the
+//                    no-slip walls (types 5, 6) and of a far-field face that closes (type 4 with error = 1)
 //   bc_wall_visc     BC_Wall_Heat_dev (type 5, Lib_BC.f90:879-982) / BC_Wall_Temperature_dev (type 6, :997-1115)
+//   bc_sym_visc      BC_Symmetry_Visc (type 3, Lib_BC.f90:741-861)
 //   bc_farfield      BC_Inflow_Outflow (Lib_BC.f90:1519-1597) with BC_Outflow (:1168-1243),
 //                    BC_Supersonic_Inflow (:1247-1330) and BC_Subsonic_Inflow (:1335-1512)
 //   sa_source        SA_Source_Blk + sa_body.inc (Mod_Spalart.f90:105-151), after the wall ghost of
@@ -129,6 +130,71 @@ __device__ __forceinline__ void bc_wall_visc(const mose_dtab& T, const double* P
     double Fr = area * mil * (g0 * nx + g1 * ny + g2 * nz) / SA_SIGMA;
     e2[V::nt] = -((double)modfm2 * Fr);
   }
+}
+
+// BC_Symmetry_Visc (Lib_BC.f90:741-861): the viscous stress of a symmetry face.  The velocity gradient takes the cell
+// and its mirror image across the face along the normal, and along each transverse direction the two neighbours and
+// their mirror images (stencil of the Fm cases :783-802).  sb, sc: the row strides of the transverse directions b, c.
+template <int NS, int NR>
+__device__ __forceinline__ void bc_sym_visc(const mose_dtab& T, const double* P, int64_t NC, const double* m,
+                                            const RecCells& rc, int64_t sb, int64_t sc, double nx, double ny,
+                                            double nz, double area, double* e2) {
+  using V = Vars<NS, NR>;
+  const int modfm2 = 1 - 2 * rc.modfm;
+  const int a = rc.dir - 1, b = rc.dir == 1 ? 1 : 0, c = rc.dir == 3 ? 1 : 2;
+  double Pb[V::nprim], M[3][3];
+#pragma unroll
+  for (int s = 0; s < V::nprim; ++s) Pb[s] = P[s * NC + rc.c];
+#pragma unroll
+  for (int x = 0; x < 3; ++x)
+#pragma unroll
+    for (int y = 0; y < 3; ++y) M[x][y] = m[(int64_t)(x + 3 * y) * NC + rc.c];
+  double rho = rho_sum<NS>(Pb), Rg = f_Rtot<NS>(T, Pb);
+  double Tk = Pb[V::np] / (rho * Rg), mil, kl;
+  int Ti = ftrunc(Tk);
+  wilke<NS>(T, Pb, rho, Ti, Tk - Ti, mil, kl);
+  // Vel3, Vel4 (direction b) and Vel7, Vel8 (direction c), then the mirror images Vel5, Vel6, Vel9, Vel10
+  const int64_t nbr[4] = {rc.c - sb, rc.c + sb, rc.c - sc, rc.c + sc};
+  double v1[3], v2[3], vn[4][3], vm[4][3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) v1[q] = Pb[V::nu + q];
+#pragma unroll
+  for (int n = 0; n < 4; ++n)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) vn[n][q] = P[(V::nu + q) * NC + nbr[n]];
+  const double d1 = v1[0] * nx + v1[1] * ny + v1[2] * nz;
+  v2[0] = v1[0] - 2.0 * d1 * nx;
+  v2[1] = v1[1] - 2.0 * d1 * ny;
+  v2[2] = v1[2] - 2.0 * d1 * nz;
+#pragma unroll
+  for (int n = 0; n < 4; ++n) {
+    const double d = vn[n][0] * nx + vn[n][1] * ny + vn[n][2] * nz;
+    vm[n][0] = vn[n][0] - 2.0 * d * nx;
+    vm[n][1] = vn[n][1] - 2.0 * d * ny;
+    vm[n][2] = vn[n][2] - 2.0 * d * nz;
+  }
+  double Gr[3][3], G2[3][3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    Gr[q][a] = (v2[q] - v1[q]) * (double)modfm2;
+    Gr[q][b] = (vn[1][q] - vn[0][q] + vm[1][q] - vm[0][q]) * 0.25;
+    Gr[q][c] = (vn[3][q] - vn[2][q] + vm[3][q] - vm[2][q]) * 0.25;
+  }
+#pragma unroll
+  for (int kk = 0; kk < 3; ++kk)
+#pragma unroll
+    for (int ii = 0; ii < 3; ++ii) G2[ii][kk] = Gr[ii][0] * M[0][kk] + Gr[ii][1] * M[1][kk] + Gr[ii][2] * M[2][kk];
+  double mie = 0.0;
+  if (NR > 0) {   // Spalart_Eddy_Viscosity (Mod_Spalart.f90:382-398)
+    double chi = Pb[V::nt] / mil, c3 = chi * chi * chi;
+    mie = c3 / (c3 + SA_CV1 * SA_CV1 * SA_CV1) * Pb[V::nt];
+  }
+  double St[3];
+  stress_std(G2, nx, ny, nz, mil, mie, St);
+  const double u0 = 0.5 * (v1[0] + v2[0]), u1 = 0.5 * (v1[1] + v2[1]), u2 = 0.5 * (v1[2] + v2[2]);
+#pragma unroll
+  for (int q = 0; q < 3; ++q) e2[V::nu + q] = -((double)modfm2 * area * St[q]);
+  e2[V::np] = -((double)modfm2 * area * (u0 * St[0] + u1 * St[1] + u2 * St[2]));
 }
 
 // the flux of a far-field face from its face state (the tails of BC_Outflow :1218-1242 and BC_Subsonic_Inflow

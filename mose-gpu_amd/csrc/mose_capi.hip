@@ -293,6 +293,7 @@ static int plan_case(const mose_scheme* sc, int nb, const int* dims, int nrec, c
         break;
       }
       case MOSE_BC_NONE:         // no flux; zero-gradient ghosts
+      case MOSE_BC_SYMMETRY:     // mirror ghost of the cell; the record reads the cell and its neighbour along the normal
         break;
       case MOSE_BC_EXTRAPOLATION:
         if (n[dir] == 2) { why = "extrapolation faces need a block extent of 1 or >= 3 along the face normal"; return MOSE_ERR_ARG; }

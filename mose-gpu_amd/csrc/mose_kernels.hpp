@@ -59,6 +59,19 @@ __device__ __forceinline__ int64_t plane_pos(const mose_blk& B, int f, int i, in
 
 __constant__ int c_guide[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
 
+// the unit normal and area of a record's boundary face: fm<dir> of the face at index (cell - modfm) along the face
+// direction (bc_record_body.inc:9-24)
+__device__ __forceinline__ void rec_face(const CaseDev& C, const mose_blk& B, const mose_bc_record& r, double& nx,
+                                         double& ny, double& nz, double& area) {
+  const int dir = (r.f + 1) / 2, modfm = r.f % 2;
+  const double* fm;
+  int64_t f, NF;
+  if (dir == 1) { fm = C.fm1; NF = C.NF1; f = B.f1off + (r.i - modfm) + (int64_t)(B.n1 + 1) * ((r.j - 1) + (int64_t)B.n2 * (r.k - 1)); }
+  else if (dir == 2) { fm = C.fm2; NF = C.NF2; f = B.f2off + (r.i - 1) + (int64_t)B.n1 * ((r.j - modfm) + (int64_t)(B.n2 + 1) * (r.k - 1)); }
+  else { fm = C.fm3; NF = C.NF3; f = B.f3off + (r.i - 1) + (int64_t)B.n1 * ((r.j - 1) + (int64_t)B.n2 * (r.k - modfm)); }
+  nx = fm[f]; ny = fm[NF + f]; nz = fm[2 * NF + f]; area = fm[3 * NF + f];
+}
+
 // ------------------------------------------------------------------------------------------------------------
 // ghosts (Fill_Ghost_Cell_local, Lib_Extrapolation.f90:1370-1524)
 // ------------------------------------------------------------------------------------------------------------
@@ -85,6 +98,26 @@ __global__ void __launch_bounds__(256) k_ghost(CaseDev C) {
 #pragma unroll
       for (int s = 0; s < V::nprim; ++s) P[s * NC + dst] = P[s * NC + src];
     }
+    return;
+  }
+  if (r.typ == MOSE_BC_SYMMETRY) {
+    // Ghost_Symmetry (Lib_Extrapolation.f90:558-587): the first ghost layer only, the cell with its velocity mirrored
+    // in the face.  The second layer is not written; no stencil of the residual reads it behind a symmetry face.
+    double nx, ny, nz, area;
+    rec_face(C, B, r, nx, ny, nz, area);
+    const int64_t dst = c - (int64_t)gstep * st;
+    double u = P[V::nu * NC + c], v = P[V::nv * NC + c], w = P[V::nw * NC + c];
+    double udn = u * nx + v * ny + w * nz;
+    u = u - 2.0 * udn * nx;
+    v = v - 2.0 * udn * ny;
+    w = w - 2.0 * udn * nz;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) P[s * NC + dst] = P[s * NC + c];
+    P[V::nu * NC + dst] = u;
+    P[V::nv * NC + dst] = v;
+    P[V::nw * NC + dst] = w;
+#pragma unroll
+    for (int s = V::np; s < V::nprim; ++s) P[s * NC + dst] = P[s * NC + c];
     return;
   }
   int ext = dir == 0 ? B.n1 : (dir == 1 ? B.n2 : B.n3);
@@ -130,13 +163,8 @@ __global__ void __launch_bounds__(128) k_bcrec(CaseDev C) {
   int dir = (Fm + 1) / 2, modfm = Fm % 2, gstep = 2 * modfm - 1, modfm2 = 1 - 2 * modfm;
   int64_t st = dir == 1 ? 1 : (dir == 2 ? (int64_t)B.pi : (int64_t)B.pi * B.pj);
   int64_t c = row(B, Im, Jm, Km);
-  // face normal and area (the boundary face of fm<dir> at index cell - modfm)
-  const double* fm;
-  int64_t f, NF;
-  if (dir == 1) { fm = C.fm1; NF = C.NF1; f = B.f1off + (Im - modfm) + (int64_t)(B.n1 + 1) * ((Jm - 1) + (int64_t)B.n2 * (Km - 1)); }
-  else if (dir == 2) { fm = C.fm2; NF = C.NF2; f = B.f2off + (Im - 1) + (int64_t)B.n1 * ((Jm - modfm) + (int64_t)(B.n2 + 1) * (Km - 1)); }
-  else { fm = C.fm3; NF = C.NF3; f = B.f3off + (Im - 1) + (int64_t)B.n1 * ((Jm - 1) + (int64_t)B.n2 * (Km - modfm)); }
-  double nx = fm[f], ny = fm[NF + f], nz = fm[2 * NF + f], area = fm[3 * NF + f];
+  double nx, ny, nz, area;
+  rec_face(C, B, r, nx, ny, nz, area);
   double e1[V::nprim], e2[V::nprim];
 #pragma unroll
   for (int s = 0; s < V::nprim; ++s) { e1[s] = 0.0; e2[s] = 0.0; }
@@ -252,6 +280,15 @@ __global__ void __launch_bounds__(128) k_bcrec(CaseDev C) {
       diff_flux_st<NS, NR>(T, C.vc, Gg, Pa, ft, nx, ny, nz, area, Fg);
 #pragma unroll
       for (int s = 0; s < V::nprim; ++s) e2[s] = -(0.5 * (double)modfm2 * (Fl[s] + Fg[s]));
+    }
+  }
+  if (r.typ == MOSE_BC_SYMMETRY) {
+    RecCells rc{c, c + gstep * st, dir, modfm};
+    bc_symmetry<NS, NR>(T, P, NC, C.dl, C.lim, C.rs, C.mco, rc, nx, ny, nz, area, e1);
+    if (C.visc) {
+      // transverse strides in the order of BC_Symmetry_Visc's stencil: (j, k), (i, k), (i, j)
+      const int64_t sj = B.pi, sk = (int64_t)B.pi * B.pj;
+      bc_sym_visc<NS, NR>(T, P, NC, C.m, rc, dir == 1 ? sj : 1, dir == 3 ? sj : sk, nx, ny, nz, area, e2);
     }
   }
   if (r.typ == MOSE_BC_WALL_ADIABATIC || r.typ == MOSE_BC_WALL_ISOTHERMAL || r.typ == MOSE_BC_FARFIELD) {

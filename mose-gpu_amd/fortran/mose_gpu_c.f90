@@ -14,8 +14,8 @@ module mose_gpu_c
   integer(c_int), parameter :: MOSE_OK = 0, MOSE_ERR_ARG = -1, MOSE_ERR_HIP = -2, MOSE_ERR_NOMEM = -3, &
                                MOSE_ERR_MESH = -4, MOSE_ERR_STATE = -5, MOSE_ERR_THERMO = -6, MOSE_ERR_NODEV = -7, &
                                MOSE_ERR_ODE = -8
-  integer(c_int), parameter :: MOSE_BC_NONE = 0, MOSE_BC_CONNECTION = 1, MOSE_BC_FARFIELD = 4, MOSE_BC_WALL_ADIABATIC = 5, &
-                               MOSE_BC_WALL_ISOTHERMAL = 6, MOSE_BC_EXTRAPOLATION = 11
+  integer(c_int), parameter :: MOSE_BC_NONE = 0, MOSE_BC_CONNECTION = 1, MOSE_BC_SYMMETRY = 3, MOSE_BC_FARFIELD = 4, &
+                               MOSE_BC_WALL_ADIABATIC = 5, MOSE_BC_WALL_ISOTHERMAL = 6, MOSE_BC_EXTRAPOLATION = 11
   integer(c_int), parameter :: MOSE_LIM_MINMOD = 1, MOSE_LIM_VANLEER = 2, MOSE_LIM_VANALBADA = 3, MOSE_LIM_MC = 4, &
                                MOSE_LIM_SUPERBEE = 5, MOSE_LIM_FIRST = 6
   integer(c_int), parameter :: MOSE_RS_HLLC = 1, MOSE_RS_AUSMP = 4, MOSE_RS_LMROE = 12

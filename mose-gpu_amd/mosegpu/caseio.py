@@ -23,7 +23,7 @@ import os
 import numpy as np
 
 from .cases import Case, fill_ghosts_nearest, make_scheme
-from .core import (BC_CONNECTION, BC_EXTRAPOLATION, BC_NONE, BC_WALL_ADIABATIC, BC_WALL_ISOTHERMAL, LIM, NTAB,
+from .core import (BC_CONNECTION, BC_EXTRAPOLATION, BC_NONE, BC_SYMMETRY, BC_WALL_ADIABATIC, BC_WALL_ISOTHERMAL, LIM, NTAB,
                    RECORD_DTYPE, RS, Layout, ThermoTables, metrics_build)
 
 RUNIV = 8314.46261815324     # Mod_Convective.f90:30
@@ -210,14 +210,14 @@ def read_tecplot_block(path: str):
 
 # ------------------------------------------------------------------------------------------------- bc.txt
 # 3-digit MOSE_open types this path runs -> this library's record types (IO_BC_Open.f90:155-211)
-_OPEN_TYPES = {0: BC_NONE, 100: BC_NONE, 101: BC_CONNECTION, 201: BC_CONNECTION, 301: BC_WALL_ADIABATIC,
+_OPEN_TYPES = {0: BC_NONE, 100: BC_NONE, 101: BC_CONNECTION, 201: BC_CONNECTION, 300: BC_SYMMETRY, 301: BC_WALL_ADIABATIC,
                302: BC_WALL_ISOTHERMAL, 400: BC_EXTRAPOLATION}
 
 
 def read_bc_open(path: str) -> np.ndarray:
     """Records of a MOSE_open bc.txt, in file order, as RECORD_DTYPE with 0-based block ids.  A first line
     "b i j k f type" is followed by the type's property line: 101/201 "bs is js ks fs d11 d12 d21 d22",
-    301 "qw ks eps", 302 "Tw ks eps"; 0, 100 and 400 have none."""
+    301 "qw ks eps", 302 "Tw ks eps"; 0, 100, 300 and 400 have none."""
     recs = []
     with _open(path) as f:
         lines = [ln for ln in (x.strip() for x in f) if ln]

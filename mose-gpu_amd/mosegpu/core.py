@@ -24,7 +24,7 @@ ERRORS = {
     -5: "MOSE_ERR_STATE", -6: "MOSE_ERR_THERMO", -7: "MOSE_ERR_NODEV", -8: "MOSE_ERR_ODE",
 }
 
-BC_NONE, BC_CONNECTION, BC_FARFIELD, BC_WALL_ADIABATIC, BC_WALL_ISOTHERMAL, BC_EXTRAPOLATION = 0, 1, 4, 5, 6, 11
+BC_NONE, BC_CONNECTION, BC_SYMMETRY, BC_FARFIELD, BC_WALL_ADIABATIC, BC_WALL_ISOTHERMAL, BC_EXTRAPOLATION = 0, 1, 3, 4, 5, 6, 11
 LIM = {"minmod": 1, "vanleer": 2, "vanalbada": 3, "mc": 4, "superbee": 5, "first-order": 6}
 RS = {"HLLC": 1, "AUSM+": 4, "AUSMp": 4, "LMRoe": 12, "Roe": 12}
 

@@ -696,11 +696,13 @@ static int thermo_cache(const o_case *C, const double *P, double *rc, double *Rc
 /* ------------------------------------------------------------------------------------------------ */
 static const int guide[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
 
+static void record_geometry(const o_case *C, const mose_bc_record *rc, double nrm[3], double *area, double dlc4[4]);
+
 void o_fill_ghosts(const o_case *C, double *P) {
   const int64_t NC = C->NC;
   int nprim = C->sc.nprim;
-  /* pass 1 per block: zero-gradient ghosts (Ghost_ZG_Extrapolate :590-607) for 0/4/11,
-   * the wall extrapolation ghost 3x1 - 3x2 + x3 (Ghost_Extrapolate :616-640) for 5/6 */
+  /* pass 1 per block: zero-gradient ghosts (Ghost_ZG_Extrapolate :590-607) for 0/4/11, the mirror ghost of the
+   * symmetry faces (Ghost_Symmetry :558-587: the first layer only, velocity reflected in the face) for 3 */
   for (int e = 0; e < C->nrec; ++e) {
     const mose_bc_record *rc = &C->rec[e];
     const o_block *B = &C->blk[rc->b];
@@ -711,6 +713,19 @@ void o_fill_ghosts(const o_case *C, double *P) {
         if (dir == 0) ig -= gstep * g; else if (dir == 1) jg -= gstep * g; else kg -= gstep * g;
         for (int s = 0; s < nprim; ++s) PV(P, s, ROW(B, ig, jg, kg)) = PV(P, s, ROW(B, rc->i, rc->j, rc->k));
       }
+    } else if (rc->typ == 3) {
+      const o_scheme *S = &C->sc;
+      double nrm[3], area, dlc4[4];
+      record_geometry(C, rc, nrm, &area, dlc4);
+      int ig = rc->i, jg = rc->j, kg = rc->k;
+      if (dir == 0) ig -= gstep; else if (dir == 1) jg -= gstep; else kg -= gstep;
+      int64_t rm = ROW(B, rc->i, rc->j, rc->k), rg = ROW(B, ig, jg, kg);
+      double vel[3] = {PV(P, S->nu, rm), PV(P, S->nv, rm), PV(P, S->nw, rm)};
+      double udotn = vel[0] * nrm[0] + vel[1] * nrm[1] + vel[2] * nrm[2];
+      for (int q = 0; q < 3; ++q) vel[q] = vel[q] - 2.0 * udotn * nrm[q];
+      for (int s = 0; s < S->nsc; ++s) PV(P, s, rg) = PV(P, s, rm);
+      PV(P, S->nu, rg) = vel[0]; PV(P, S->nv, rg) = vel[1]; PV(P, S->nw, rg) = vel[2];
+      for (int s = S->np; s < nprim; ++s) PV(P, s, rg) = PV(P, s, rm);
     }
   }
   /* wall ghosts (first pass, `case default` of the host fill) and the cubic pass of the 400 faces
@@ -1041,6 +1056,73 @@ static void bc_wall_visc(const o_case *C, const double *P, double *R, const mose
     double Fr = area * mil * (g2[0] * nrm[0] + g2[1] * nrm[1] + g2[2] * nrm[2]) / SA_SIGMA;
     PV(R, S->nt, rb) = PV(R, S->nt, rb) - (double)modfm2 * Fr;
   }
+}
+
+/* BC_Symmetry_Visc: Lib_BC.f90:741-861.  Velocity gradient from the cell and its mirror image along the normal, and
+ * along each transverse direction from the two neighbours and their mirror images (the Fm cases :783-802) */
+static void bc_sym_visc(const o_case *C, const double *P, double *R, const mose_bc_record *rc, const double nrm[3],
+                        double area) {
+  const o_scheme *S = &C->sc;
+  const o_tables *t = &C->tab;
+  const int64_t NC = C->NC;
+  const o_block *B = &C->blk[rc->b];
+  int nu = S->nu, np = S->np;
+  int Fm = rc->f, modfm2 = 1 - 2 * (Fm % 2), Im = rc->i, Jm = rc->j, Km = rc->k, a, b, c;
+  int64_t rb = ROW(B, Im, Jm, Km), n3, n4, n7, n8;
+  if (Fm <= 2) {
+    n3 = ROW(B, Im, Jm - 1, Km); n4 = ROW(B, Im, Jm + 1, Km); n7 = ROW(B, Im, Jm, Km - 1); n8 = ROW(B, Im, Jm, Km + 1);
+    a = 0; b = 1; c = 2;
+  } else if (Fm <= 4) {
+    n3 = ROW(B, Im - 1, Jm, Km); n4 = ROW(B, Im + 1, Jm, Km); n7 = ROW(B, Im, Jm, Km - 1); n8 = ROW(B, Im, Jm, Km + 1);
+    a = 1; b = 0; c = 2;
+  } else {
+    n3 = ROW(B, Im - 1, Jm, Km); n4 = ROW(B, Im + 1, Jm, Km); n7 = ROW(B, Im, Jm - 1, Km); n8 = ROW(B, Im, Jm + 1, Km);
+    a = 2; b = 0; c = 1;
+  }
+  double Pb[ONPRIM], M[3][3];
+  for (int s = 0; s < S->nprim; ++s) Pb[s] = PV(P, s, rb);
+  for (int x = 0; x < 3; ++x)
+    for (int y = 0; y < 3; ++y) M[x][y] = C->m[(int64_t)(x + 3 * y) * NC + rb];
+  double rho, Rgas;
+  co_rotot_Rtot(t, Pb, &rho, &Rgas);
+  double T = Pb[np] / (rho * Rgas), mil, kl;
+  int Ti = fint(T);
+  wilke(t, Pb, rho, Ti, T - Ti, &mil, &kl);
+  double V1[3], V2[3], V3[3], V4[3], V5[3], V6[3], V7[3], V8[3], V9[3], V10[3];
+  for (int q = 0; q < 3; ++q) {
+    V1[q] = Pb[nu + q]; V3[q] = PV(P, nu + q, n3); V4[q] = PV(P, nu + q, n4); V7[q] = PV(P, nu + q, n7);
+    V8[q] = PV(P, nu + q, n8);
+  }
+  double d1 = V1[0] * nrm[0] + V1[1] * nrm[1] + V1[2] * nrm[2];
+  double d3 = V3[0] * nrm[0] + V3[1] * nrm[1] + V3[2] * nrm[2];
+  double d4 = V4[0] * nrm[0] + V4[1] * nrm[1] + V4[2] * nrm[2];
+  double d7 = V7[0] * nrm[0] + V7[1] * nrm[1] + V7[2] * nrm[2];
+  double d8 = V8[0] * nrm[0] + V8[1] * nrm[1] + V8[2] * nrm[2];
+  for (int q = 0; q < 3; ++q) {
+    V2[q] = V1[q] - 2.0 * d1 * nrm[q];
+    V5[q] = V3[q] - 2.0 * d3 * nrm[q];
+    V6[q] = V4[q] - 2.0 * d4 * nrm[q];
+    V9[q] = V7[q] - 2.0 * d7 * nrm[q];
+    V10[q] = V8[q] - 2.0 * d8 * nrm[q];
+  }
+  double Gr[3][3], G2[3][3];
+  for (int q = 0; q < 3; ++q) {
+    Gr[q][a] = (V2[q] - V1[q]) * (double)modfm2;
+    Gr[q][b] = (V4[q] - V3[q] + V6[q] - V5[q]) * 0.25;
+    Gr[q][c] = (V8[q] - V7[q] + V10[q] - V9[q]) * 0.25;
+  }
+  for (int kk = 0; kk < 3; ++kk)
+    for (int ii = 0; ii < 3; ++ii) G2[ii][kk] = Gr[ii][0] * M[0][kk] + Gr[ii][1] * M[1][kk] + Gr[ii][2] * M[2][kk];
+  double mie = 0.0;
+  if (S->rans == 2) {   /* Spalart_Eddy_Viscosity (Mod_Spalart.f90:382-398): fv1(chi) rho nu~ */
+    double chi = Pb[S->nt] / mil, c3 = chi * chi * chi;
+    mie = c3 / (c3 + SA_CV1 * SA_CV1 * SA_CV1) * Pb[S->nt];
+  }
+  double St[3], Vel[3];
+  stress_std(G2, nrm, mil, mie, St);
+  for (int q = 0; q < 3; ++q) Vel[q] = 0.5 * (V1[q] + V2[q]);
+  for (int q = 0; q < 3; ++q) PV(R, nu + q, rb) = PV(R, nu + q, rb) - (double)modfm2 * area * St[q];
+  PV(R, np, rb) = PV(R, np, rb) - (double)modfm2 * area * (Vel[0] * St[0] + Vel[1] * St[1] + Vel[2] * St[2]);
 }
 
 /* the face flux of the far-field closures from a face state (the common tails of BC_Outflow, Lib_BC.f90:1218-1242,
@@ -1483,6 +1565,13 @@ void o_residual(const o_case *C, double *P, double *R, double dt) {
       case 11:
         bc_extrapolation(C, P, R, rc2, nrm, area);
         break;
+      case 3: {
+        double bdl, idl;
+        record_half_lengths(C, rc2, &bdl, &idl);
+        bc_symmetry_eul(C, P, R, rc2, nrm, area, bdl, idl);
+        if (S->visc) bc_sym_visc(C, P, R, rc2, nrm, area);
+        break;
+      }
       case 5:
       case 6: {
         double bdl, idl;

@@ -102,9 +102,17 @@ def test_refusals(tmp_path, key, value, msg):
 
 def test_unsupported_boundary_type(tmp_path):
     d = tmp_path / "bc.txt"
-    d.write_text("       1       1       1       1       3     300\n")
-    with pytest.raises(caseio.CaseFormatError, match="300"):
+    d.write_text("       1       1       1       1       3     303\n")
+    with pytest.raises(caseio.CaseFormatError, match="303"):
         caseio.read_bc_open(str(d))
+
+
+def test_symmetry_type(tmp_path):
+    """MOSE 300 (Euler symmetry, no property line) is the internal type 3 (IO_BC_Open.f90:191-193)."""
+    d = tmp_path / "bc.txt"
+    d.write_text("       1       1       1       1       3     300\n       1       2       1       1       3     300\n")
+    r = caseio.read_bc_open(str(d))
+    assert list(r["typ"]) == [3, 3] and list(r["i"]) == [1, 2]
 
 
 def test_tecplot_block_reader(tmp_path):
