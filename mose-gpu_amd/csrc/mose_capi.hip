@@ -451,6 +451,12 @@ static Case* get_case(mose_ctx* ctx, int id) {
   return (c && c->valid) ? c : nullptr;
 }
 
+// The residual kernels are instantiated for (nsc, nrans) = (1, 0), (1, 1) and (4, 0): the species and RANS counts are
+// compile-time counts, as in the reference (one binary per count, src/lib/CMakeLists.txt:64-71).  Four species are
+// the count of the reference's multi-species cases (test/1D/diff1, diff2).  F<NS, NR>(...) of the case's counts:
+#define MOSE_DISPATCH(sc, F, ...) \
+  ((sc).nsc == 4 ? F<4, 0>(__VA_ARGS__) : (sc).nrans == 0 ? F<1, 0>(__VA_ARGS__) : F<1, 1>(__VA_ARGS__))
+
 static size_t stage_lds_(int nprim, bool visc, int TI, int TJ, int TK);
 // static LDS of the stage kernel beside its dynamic part (the workgroup reductions of dt and of the residual norm)
 static constexpr size_t kStaticLds = 2048;
@@ -479,8 +485,9 @@ extern "C" int mose_case_create(mose_ctx* ctx, const mose_scheme* sc, int nb, co
   if (rc != MOSE_OK) return fail(ctx, rc, why);
   if (sc->nsc != ctx->nsc) return fail(ctx, MOSE_ERR_ARG, "the scheme's nsc differs from the context tables");
   if (sc->viscous && !ctx->viscous_tables) return fail(ctx, MOSE_ERR_ARG, "viscous case needs mu and k tables");
-  if (!((sc->nsc == 1 && sc->nrans == 0) || (sc->nsc == 1 && sc->nrans == 1)))
-    return fail(ctx, MOSE_ERR_ARG, "the library instantiates the residual kernels for nsc = 1 (nrans 0 or 1)");
+  if (!((sc->nsc == 1 && sc->nrans == 0) || (sc->nsc == 1 && sc->nrans == 1) || (sc->nsc == 4 && sc->nrans == 0)))
+    return fail(ctx, MOSE_ERR_ARG, "the library instantiates the residual kernels for nsc = 1 (nrans 0 or 1) and nsc = 4 "
+                                   "(nrans 0)");
   HIPCHK(ctx, hipSetDevice(ctx->device));
   rc = alloc_case(ctx, *c);
   if (rc != MOSE_OK) { case_free(c.get()); return rc; }
@@ -852,7 +859,7 @@ extern "C" int mose_case_tune(mose_ctx* ctx, int id, const mose_tuning* t, mose_
   if (chosen) *chosen = *t;
   if (c->path == MOSE_PATH_FUSED) {
     if (t->ti == 0 && t->tj == 0 && t->tk == 0)
-      return c->sc.nrans == 0 ? autotune<1, 0>(ctx, *c, chosen) : autotune<1, 1>(ctx, *c, chosen);
+      return MOSE_DISPATCH(c->sc, autotune, ctx, *c, chosen);
     return set_tiles(ctx, *c, t->ti, t->tj, t->tk);
   }
   return MOSE_OK;
@@ -862,8 +869,7 @@ extern "C" int mose_case_tune(mose_ctx* ctx, int id, const mose_tuning* t, mose_
 // staging; nothing waits.  case_collect waits for the case's stream and fills the report.
 static int case_enqueue(mose_ctx* ctx, Case* c, int nsteps) {
   int rc;
-  if (c->sc.nrans == 0) rc = run_steps<1, 0>(ctx, *c, nsteps);
-  else rc = run_steps<1, 1>(ctx, *c, nsteps);
+  rc = MOSE_DISPATCH(c->sc, run_steps, ctx, *c, nsteps);
   if (rc != MOSE_OK) return rc;
   size_t nh = (size_t)std::min(nsteps, c->hist_cap);
   if (nh > 0)
@@ -993,7 +999,7 @@ extern "C" int mose_case_residual(mose_ctx* ctx, int id, double* R) {
   Case* c = get_case(ctx, id);
   if (!c || !R) return MOSE_ERR_ARG;
   HIPCHK(ctx, hipSetDevice(ctx->device));
-  return c->sc.nrans == 0 ? residual_impl<1, 0>(ctx, *c, R) : residual_impl<1, 1>(ctx, *c, R);
+  return MOSE_DISPATCH(c->sc, residual_impl, ctx, *c, R);
 }
 
 extern "C" int mose_case_dt(mose_ctx* ctx, int id, double* dt) {
@@ -1001,8 +1007,7 @@ extern "C" int mose_case_dt(mose_ctx* ctx, int id, double* dt) {
   if (!c || !dt) return MOSE_ERR_ARG;
   HIPCHK(ctx, hipSetDevice(ctx->device));
   CaseDev D = make_dev(ctx, *c, c->P0, c->PA);
-  if (c->sc.nrans == 0) launch_dt<1, 0>(*c, D, c->stream, -1);
-  else launch_dt<1, 1>(*c, D, c->stream, -1);
+  MOSE_DISPATCH(c->sc, launch_dt, *c, D, c->stream, -1);
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipMemcpyAsync(dt, c->dt, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(ctx, hipStreamSynchronize(c->stream));
@@ -1038,8 +1043,7 @@ extern "C" int mose_case_state_hash(mose_ctx* ctx, int id, uint64_t* lo, uint64_
   if (!c || !lo || !hi) return MOSE_ERR_ARG;
   HIPCHK(ctx, hipSetDevice(ctx->device));
   HIPCHK(ctx, hipMemsetAsync(c->hslot, 0, 2 * sizeof(unsigned long long), c->stream));
-  if (c->sc.nrans == 0) launch_hash<1, 0>(*c, c->stream);
-  else launch_hash<1, 1>(*c, c->stream);
+  MOSE_DISPATCH(c->sc, launch_hash, *c, c->stream);
   HIPCHK(ctx, hipGetLastError());
   unsigned long long h[2];
   HIPCHK(ctx, hipMemcpyAsync(h, c->hslot, sizeof h, hipMemcpyDeviceToHost, c->stream));
