@@ -377,6 +377,56 @@ def fortran_e23_15e3(x: float) -> str:
     return (sign + body + "E" + ("+" if e >= 0 else "-") + "%03d" % abs(e)).rjust(23)
 
 
+def fortran_e20_10(x: float) -> str:
+    """One value as Fortran's E20.10 edit descriptor writes it: a 0.d...d mantissa of 10 digits and an exponent of
+    two digits after "E" (three digits in place of "E" and the sign's column once |exponent| > 99), right-justified
+    in 20 columns."""
+    x = float(x)
+    if x != x:
+        return "NaN".rjust(20)
+    if x in (math.inf, -math.inf):
+        return ("Infinity" if x > 0 else "-Infinity").rjust(20)
+    if x == 0.0:
+        body, e = "0." + "0" * 10, 0
+    else:
+        s = "%.9e" % abs(x)                        # d.ddddddddde+XX, correctly rounded to 10 digits
+        body, e = "0." + s[0] + s[2:11], int(s[12:]) + 1
+    sign = "-" if (x < 0.0 or (x == 0.0 and math.copysign(1.0, x) < 0.0)) else ""
+    ex = ("E" + ("+" if e >= 0 else "-") + "%02d" % abs(e)) if abs(e) <= 99 else (("+" if e >= 0 else "-") + "%03d" % abs(e))
+    return (sign + body + ex).rjust(20)
+
+
+def residual_history_line(iteration: int, time: float, norms) -> str:
+    """One row of OUTPUT/residual-history.dat (Wrap_Setup.f90:140, Wrap_Postprocess.f90:69): the iteration as I8, the
+    time and the nres = nprim - nsc + 1 residual norms as E20.10."""
+    return "%8d" % iteration + fortran_e20_10(time) + "".join(fortran_e20_10(v) for v in norms)
+
+
+def step_with_history(dc, nsteps: int, res_diter: int, path: str, iter0: int = 0, time0: float = 0.0,
+                      newrun: bool = True) -> tuple[int, float]:
+    """Step a resident case (a DeviceCase) nsteps whole steps and write its residual history the way the reference's
+    explicit loop does (Mod_Explicit.f90:324-358, Wrap_Postprocess.f90:66-70): after every step whose iteration
+    number is a multiple of res_diter (the [MOSE-IO] diter-res key, default 1), one row with the iteration, the
+    simulation time and the L2 norms of that step's change (mose_case_residual_norm).  The device steps up to
+    res_diter steps at a time in between.  iter0 and time0 are the iteration and time of the state the case was
+    created with (0 for a new run); newrun = False appends to an existing file (a restart).  Returns the iteration
+    and time reached."""
+    if nsteps < 0 or res_diter < 1:
+        raise ValueError("nsteps >= 0 and res_diter >= 1")
+    rep = dc.step(0)
+    it, t = iter0 + int(rep.steps), time0 + rep.time
+    with open(path, "w" if newrun else "a") as f:
+        left = nsteps
+        while left > 0:
+            k = min(res_diter - it % res_diter, left)
+            rep = dc.step(k)
+            left -= k
+            it, t = iter0 + int(rep.steps), time0 + rep.time
+            if it % res_diter == 0:
+                f.write(residual_history_line(it, t, dc.residual_norm()) + "\n")
+    return it, t
+
+
 def _str_r8p_no_sign(x: float) -> str:
     """ORION's str(no_sign=.true., n=x) for a real(8): the E23.15E3 string without its first column."""
     s = fortran_e23_15e3(x)

@@ -187,6 +187,18 @@ def test_e23_15e3_matches_fortran(tmp_path):
     subprocess.run([fc, "-O0", "f.f90", "-o", "f"], cwd=tmp_path, check=True, capture_output=True)
     out = subprocess.run(["./f"], cwd=tmp_path, check=True, capture_output=True, text=True).stdout.splitlines()
     assert out == [caseio.fortran_e23_15e3(v) for v in vals]
+    # E20.10, the residual history's descriptor: two exponent digits after E, three in its place beyond 99
+    (tmp_path / "f.f90").write_text((tmp_path / "f.f90").read_text().replace("(E23.15E3)", "(E20.10)"))
+    subprocess.run([fc, "-O0", "f.f90", "-o", "f"], cwd=tmp_path, check=True, capture_output=True)
+    out = subprocess.run(["./f"], cwd=tmp_path, check=True, capture_output=True, text=True).stdout.splitlines()
+    assert out == [caseio.fortran_e20_10(v) for v in vals]
+
+
+def test_residual_history_line():
+    """(I8,E20.10,nres E20.10) of Wrap_Setup.f90:140 for nres = 5."""
+    ln = caseio.residual_history_line(1234, 0.5, [1.0, 0.0, 2.5e-3, 1e-120, 3.0])
+    assert ln == ("    1234" + "    0.5000000000E+00" + "    0.1000000000E+01" + "    0.0000000000E+00"
+                  + "    0.2500000000E-02" + "    0.1000000000-119" + "    0.3000000000E+01")
 
 
 def test_field_tec_layout(tmp_path, vortex):
@@ -224,3 +236,24 @@ def test_native_roundtrip(tmp_path, vortex):
     Q, t = caseio.read_native(c, str(p))
     assert t == 1.5
     assert np.array_equal(c.layout.interior(Q, 0), c.layout.interior(P, 0))
+
+
+SHU_OSHER = os.path.join(HERE, "golden", "cases", "shu-osher")
+
+
+def test_shu_osher_inputs_and_shock_speed():
+    """The reference's test/1D/Shu-Osher inputs: 200 cells on [-5, 5], HLLC + van Leer, RK3 at CFL 0.5, extrapolation
+    faces in x.  A Mach 3 shock starts at x = -4 into gas at p = 1, rho = 1 + 0.2 sin 5x; it runs at 3 sqrt(1.4) on
+    average (rho 1 in the mean), so after t ~ 1.8 it stands at -4 + 3 sqrt(1.4) t within two cells."""
+    c = caseio.load_case(SHU_OSHER, 330)
+    s = c.scheme
+    assert c.layout.dims == [(200, 1, 1)] and (s.riemann, s.limiter, s.n_rk, s.cfl, s.ndir) == (1, 2, 3, 0.5, 1)
+    assert sorted(set(c.records["typ"].tolist())) == [BC_NONE, 11]
+    P = c.P.copy()
+    t = OracleCase(c).step(P, 330).sum()
+    W = c.layout.interior(P, 0)[:, 0, 0, :]
+    x = -5.0 + (np.arange(200) + 0.5) * 0.05
+    jump = x[np.max(np.where(np.abs(np.diff(W[0])) > 0.5)[0])] + 0.025
+    assert 1.7 < t < 1.9
+    assert abs(jump - (-4.0 + 3.0 * np.sqrt(1.4) * t)) <= 0.1, (jump, t)
+    assert W[0].min() > 0.7 and W[4].min() > 0.9

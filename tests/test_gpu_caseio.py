@@ -86,3 +86,64 @@ def test_none_faces_viscous_box(gpu_ready, path, tile):
     print(f"MOSE 0 faces, viscous box, path {path} tile {tuple(chosen)} x 5: rel Linf {err:.3e}, "
           f"values differing bitwise {int(np.sum(a != b))} of {a.size}", flush=True)
     assert err <= TOL
+
+
+SHU_OSHER = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cases", "shu-osher")
+
+
+@pytest.mark.parametrize("path,tile", [(mosegpu.PATH_FUSED, (0, 0, 0)), (mosegpu.PATH_FUSED, (128, 1, 1)),
+                                       (mosegpu.PATH_PLAIN, (0, 0, 0))])
+def test_shu_osher_device_matches_oracle(gpu_ready, path, tile):
+    """The reference's test/1D/Shu-Osher inputs, 330 steps (to t ~ 1.77, its threshold being 1.8)."""
+    c = caseio.load_case(SHU_OSHER, 330)
+    ctx = Context(c.tables)
+    dc = DeviceCase(ctx, c)
+    dc.upload(c.P)
+    chosen = dc.tune(path, tile)
+    rep = dc.step(330)
+    Pg = dc.download()
+    dc.destroy()
+    ctx.close()
+    Po = c.P.copy()
+    dts = OracleCase(c).step(Po, 330)
+    assert abs(rep.time - dts.sum()) <= 1e-13 * dts.sum()
+    a, b = c.layout.interior(Pg, 0), c.layout.interior(Po, 0)
+    err = max(np.abs(a[s] - b[s]).max() / max(np.abs(b[s]).max(), 1.0) for s in range(c.nprim))
+    print(f"shu-osher (reference inputs), path {path} tile {chosen} x 330: rel Linf {err:.3e}, values differing "
+          f"bitwise {int(np.sum(a != b))} of {a.size}", flush=True)
+    assert err <= TOL
+
+
+def test_residual_history_file(gpu_ready, tmp_path):
+    """caseio.step_with_history on the Shu-Osher inputs with diter-res = 7: rows at iterations 7, 14, 21, 28, then a
+    restart that appends rows 35 and 42.  Each row's time is the oracle's, and its norms are the L2 norms of the
+    oracle's change over that step (summed in another order, so to 1e-10)."""
+    from test_gpu_parity import oracle_residual_norm
+    c = caseio.load_case(SHU_OSHER, 45)
+    ctx = Context(c.tables)
+    dc = DeviceCase(ctx, c)
+    dc.upload(c.P)
+    dc.tune(mosegpu.PATH_FUSED, (0, 0, 0))
+    path = str(tmp_path / "residual-history.dat")
+    it, t = caseio.step_with_history(dc, 30, 7, path)
+    assert it == 30
+    it, t = caseio.step_with_history(dc, 15, 7, path, newrun=False)
+    dc.destroy()
+    ctx.close()
+    assert it == 45
+    rows = open(path).read().splitlines()
+    assert [int(r[:8]) for r in rows] == [7, 14, 21, 28, 35, 42]
+    assert all(len(r) == 8 + 20 * 6 for r in rows)
+    o = OracleCase(c)
+    P = c.P.copy()
+    time = 0.0
+    for n in range(1, 43):
+        Pold = P.copy()
+        time += o.step(P, 1)[0]
+        if n % 7 == 0:
+            r = rows[n // 7 - 1]
+            vals = [float(r[8 + 20 * q:8 + 20 * (q + 1)].replace("E", "e")) for q in range(6)]
+            assert abs(vals[0] - time) <= 1e-9 * time
+            want = oracle_residual_norm(c, P, Pold)
+            assert np.all(np.abs(np.array(vals[1:]) - want) <= 1e-9 * want.max() + 1e-300), (n, vals, want)
+    print(f"residual history: {len(rows)} rows, last {rows[-1]}", flush=True)
