@@ -68,7 +68,7 @@ enum {
 
 /* limiter ids (Lib_Limiters.f90:21-61) and Riemann solver ids (Lib_Riemann.f90:23-107) */
 enum { MOSE_LIM_MINMOD = 1, MOSE_LIM_VANLEER = 2, MOSE_LIM_VANALBADA = 3, MOSE_LIM_MC = 4, MOSE_LIM_SUPERBEE = 5, MOSE_LIM_FIRST = 6 };
-enum { MOSE_RS_HLLC = 1, MOSE_RS_AUSMP = 4, MOSE_RS_LMROE = 12 };
+enum { MOSE_RS_HLLC = 1, MOSE_RS_HLLE = 2, MOSE_RS_AUSMP = 4, MOSE_RS_LMROE = 12 };
 
 /* one boundary cell-face: bc_bi/bc_bj/bc_bk/bc_bf/bc_typ of Mod_BC.f90, the connection donor of
  * Lib_Extrapolation.f90:1023-1135 (conn_is/js/ks/fs, conn_d11..d22) and the wall / far-field data */

@@ -225,8 +225,9 @@ static int plan_case(const mose_scheme* sc, int nb, const int* dims, int nrec, c
   if (sc->nrans < 0 || sc->nrans > 1) { why = "nrans must be 0 or 1 (Spalart-Allmaras)"; return MOSE_ERR_ARG; }
   if (sc->nrans > 0 && !sc->viscous) { why = "Spalart-Allmaras needs the viscous fluxes"; return MOSE_ERR_ARG; }
   if (sc->limiter < 1 || sc->limiter > 6) { why = "limiter id must be 1..6"; return MOSE_ERR_ARG; }
-  if (!(sc->riemann == MOSE_RS_HLLC || sc->riemann == MOSE_RS_AUSMP || sc->riemann == MOSE_RS_LMROE)) {
-    why = "riemann solver must be HLLC (1), AUSM+ (4) or LMRoe (12)";
+  if (!(sc->riemann == MOSE_RS_HLLC || sc->riemann == MOSE_RS_HLLE || sc->riemann == MOSE_RS_AUSMP ||
+        sc->riemann == MOSE_RS_LMROE)) {
+    why = "riemann solver must be HLLC (1), HLLE (2), AUSM+ (4) or LMRoe (12)";
     return MOSE_ERR_ARG;
   }
   if (sc->n_rk < 1 || sc->n_rk > 3) { why = "n_rk must be 1..3"; return MOSE_ERR_ARG; }
@@ -665,10 +666,12 @@ static void launch_stage_sh(Case& c, const CaseDev& D, hipStream_t s, const Stag
   if (c.sc.viscous) {
     if (rs == MOSE_RS_LMROE) k_stage<NS, NR, true, MOSE_RS_LMROE, BS, SH><<<grid, BS, lds, s>>>(D, A);
     else if (rs == MOSE_RS_AUSMP) k_stage<NS, NR, true, MOSE_RS_AUSMP, BS, SH><<<grid, BS, lds, s>>>(D, A);
+    else if (rs == MOSE_RS_HLLE) k_stage<NS, NR, true, MOSE_RS_HLLE, BS, SH><<<grid, BS, lds, s>>>(D, A);
     else k_stage<NS, NR, true, MOSE_RS_HLLC, BS, SH><<<grid, BS, lds, s>>>(D, A);
   } else {
     if (rs == MOSE_RS_LMROE) k_stage<NS, NR, false, MOSE_RS_LMROE, BS, SH><<<grid, BS, lds, s>>>(D, A);
     else if (rs == MOSE_RS_AUSMP) k_stage<NS, NR, false, MOSE_RS_AUSMP, BS, SH><<<grid, BS, lds, s>>>(D, A);
+    else if (rs == MOSE_RS_HLLE) k_stage<NS, NR, false, MOSE_RS_HLLE, BS, SH><<<grid, BS, lds, s>>>(D, A);
     else k_stage<NS, NR, false, MOSE_RS_HLLC, BS, SH><<<grid, BS, lds, s>>>(D, A);
   }
 }

@@ -213,6 +213,27 @@ struct F5 {
 };
 
 // HLLC (Batten): conv_physics.inc:87-115 / Lib_Riemann_HLL.f90:21-134
+// HLLE (rs 2): conv_physics.inc:116-124 in the interior kernels, riemann_HLLE (Lib_Riemann_HLL.f90:137-198) in the
+// record kernels: the two-wave HLL flux between the Davis/Einfeldt signal speeds of HLLC
+__device__ __forceinline__ F5 hlle(double dlt, double drt, double ul, double vl, double wl, double pl, double al,
+                                   double ur, double vr, double wr, double pr, double ar, double nx, double ny, double nz,
+                                   const Roe& ro) {
+  double unl = ul * nx + vl * ny + wl * nz, unr = ur * nx + vr * ny + wr * nz;
+  double unr_ = ro.u * nx + ro.v * ny + ro.w * nz;
+  double S1 = dmin2(dmin2(0.0, unl - al), unr_ - ro.a);
+  double S4 = dmax2(dmax2(0.0, unr + ar), unr_ + ro.a);
+  double idn = 1.0 / (S4 - S1);
+  double frl = dlt * unl, frr = drt * unr;
+  F5 F;
+  F.r = (S4 * frl - S1 * frr + S1 * S4 * (drt - dlt)) * idn;
+  F.u = (S4 * (frl * ul + pl * nx) - S1 * (frr * ur + pr * nx) + S1 * S4 * (drt * ur - dlt * ul)) * idn;
+  F.v = (S4 * (frl * vl + pl * ny) - S1 * (frr * vr + pr * ny) + S1 * S4 * (drt * vr - dlt * vl)) * idn;
+  F.w = (S4 * (frl * wl + pl * nz) - S1 * (frr * wr + pr * nz) + S1 * S4 * (drt * wr - dlt * wl)) * idn;
+  F.e = (S4 * (frl * ro.h0l) - S1 * (frr * ro.h0r) + S1 * S4 * (drt * (ro.h0r - pr / drt) - dlt * (ro.h0l - pl / dlt))) *
+        idn;
+  return F;
+}
+
 __device__ __forceinline__ F5 hllc(double dlt, double drt, double ul, double vl, double wl, double pl, double al,
                                    double ur, double vr, double wr, double pr, double ar, double nx, double ny, double nz,
                                    const Roe& ro) {
@@ -381,6 +402,7 @@ __device__ __forceinline__ F5 riemann(const mose_dtab& T, int rs_, double mco, b
   }
   if (rs == 4) return ausmp<NS>(T, dl, ul, vl, wl, pl, al, dlt, dr, ur, vr, wr, pr, ar, drt, nx, ny, nz);
   Roe ro = roe_avg<NS>(T, hllc_clamp, dl, dr, dlt, drt, ul, vl, wl, ur, vr, wr, pl, pr, Rl, Rr);
+  if (rs == 2) return hlle(dlt, drt, ul, vl, wl, pl, al, ur, vr, wr, pr, ar, nx, ny, nz, ro);
   return hllc(dlt, drt, ul, vl, wl, pl, al, ur, vr, wr, pr, ar, nx, ny, nz, ro);
 }
 

@@ -26,7 +26,7 @@ ERRORS = {
 
 BC_NONE, BC_CONNECTION, BC_SYMMETRY, BC_FARFIELD, BC_WALL_ADIABATIC, BC_WALL_ISOTHERMAL, BC_EXTRAPOLATION = 0, 1, 3, 4, 5, 6, 11
 LIM = {"minmod": 1, "vanleer": 2, "vanalbada": 3, "mc": 4, "superbee": 5, "first-order": 6}
-RS = {"HLLC": 1, "AUSM+": 4, "AUSMp": 4, "LMRoe": 12, "Roe": 12}
+RS = {"HLLC": 1, "HLLE": 2, "AUSM+": 4, "AUSMp": 4, "LMRoe": 12, "Roe": 12}
 
 # one boundary cell-face (mose_bc_record)
 RECORD_DTYPE = np.dtype(

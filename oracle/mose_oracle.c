@@ -231,6 +231,26 @@ static void roe_averages(const o_tables *t, int clamp, int lo, int hi, const dou
 }
 
 /* HLLC (Batten): conv_physics.inc:87-115 == Lib_Riemann_HLL.f90:21-134 (same statements) */
+/* HLLE (rs 2): conv_physics.inc:116-124 / riemann_HLLE, Lib_Riemann_HLL.f90:137-198 (the signal speeds of HLLC) */
+static void hlle(double dltot, double drtot, double ul, double vl, double wl, double pl, double al, double ur,
+                 double vr, double wr, double pr, double ar, double nx, double ny, double nz, const roe_t *ro,
+                 double F[5]) {
+  double unl = ul * nx + vl * ny + wl * nz;
+  double unr = ur * nx + vr * ny + wr * nz;
+  double un_roe = ro->u_roe * nx + ro->v_roe * ny + ro->w_roe * nz;
+  double h0l = ro->h0l, h0r = ro->h0r;
+  double S1 = min3(0.0, unl - al, un_roe - ro->a_roe);
+  double S4 = max3(0.0, unr + ar, un_roe + ro->a_roe);
+  double inv_denom = 1.0 / (S4 - S1);
+  double FrL = dltot * unl, FrR = drtot * unr;
+  F[0] = (S4 * FrL - S1 * FrR + S1 * S4 * (drtot - dltot)) * inv_denom;
+  F[1] = (S4 * (FrL * ul + pl * nx) - S1 * (FrR * ur + pr * nx) + S1 * S4 * (drtot * ur - dltot * ul)) * inv_denom;
+  F[2] = (S4 * (FrL * vl + pl * ny) - S1 * (FrR * vr + pr * ny) + S1 * S4 * (drtot * vr - dltot * vl)) * inv_denom;
+  F[3] = (S4 * (FrL * wl + pl * nz) - S1 * (FrR * wr + pr * nz) + S1 * S4 * (drtot * wr - dltot * wl)) * inv_denom;
+  F[4] = (S4 * (FrL * h0l) - S1 * (FrR * h0r) + S1 * S4 * (drtot * (h0r - pr / drtot) - dltot * (h0l - pl / dltot))) *
+         inv_denom;
+}
+
 static void hllc(double dltot, double drtot, double ul, double vl, double wl, double pl, double al, double ur,
                  double vr, double wr, double pr, double ar, double nx, double ny, double nz, const roe_t *ro,
                  double F[5]) {
@@ -395,6 +415,9 @@ static void riemann_any(const o_case *C, int rs, int clamp_hllc, const double *d
   if (rs == 1) {
     roe_averages(t, clamp_hllc, 0, ONTMAX - 1, dl, dr, dltot, drtot, ul, vl, wl, ur, vr, wr, pl, pr, Rl, Rr, &ro);
     hllc(dltot, drtot, ul, vl, wl, pl, al, ur, vr, wr, pr, ar, nx, ny, nz, &ro, F);
+  } else if (rs == 2) {
+    roe_averages(t, clamp_hllc, 0, ONTMAX - 1, dl, dr, dltot, drtot, ul, vl, wl, ur, vr, wr, pl, pr, Rl, Rr, &ro);
+    hlle(dltot, drtot, ul, vl, wl, pl, al, ur, vr, wr, pr, ar, nx, ny, nz, &ro, F);
   } else if (rs == 4) {
     ausmp(t, dl, ul, vl, wl, pl, al, dltot, dr, ur, vr, wr, pr, ar, drtot, nx, ny, nz, F);
   } else if (rs == 12) {

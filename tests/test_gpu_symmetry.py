@@ -34,6 +34,17 @@ def test_symmetry_box(gpu_ready, axis, viscous, path, tile):
 
 
 @pytest.mark.parametrize("path,tile", PATHS)
+@pytest.mark.parametrize("viscous", [False, True])
+def test_symmetry_box_hlle(gpu_ready, path, tile, viscous):
+    """HLLE (rs 2) in the interior faces and in BC_Symmetry_Eul's face flux."""
+    case, _ = mirror_pair(1, [12, 10, 9], viscous, riemann="HLLE", seed=51)
+    Pg, rep, chosen = run_device(case, 4, path, tile)
+    Po = case.P.copy()
+    OracleCase(case).step(Po, 4)
+    assert compare(case, Pg, Po, f"symmetry box, HLLE, viscous {viscous}, path {path} tile {chosen} x 4") <= TOL
+
+
+@pytest.mark.parametrize("path,tile", PATHS)
 def test_symmetry_box_sa(gpu_ready, path, tile):
     case, _ = mirror_pair(0, [12, 10, 9], True, nrans=1, walls=1, seed=41)
     Pg, rep, chosen = run_device(case, 4, path, tile)
