@@ -152,11 +152,12 @@ def test_residual_history_file(gpu_ready, tmp_path):
 EIN91 = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cases", "ein91")
 
 
+@pytest.mark.parametrize("name", ["ein91", "noh87"])
 @pytest.mark.parametrize("path,tile", [(mosegpu.PATH_FUSED, (0, 0, 0)), (mosegpu.PATH_FUSED, (128, 1, 1)),
                                        (mosegpu.PATH_PLAIN, (0, 0, 0))])
-def test_ein91_hlle_device_matches_oracle(gpu_ready, path, tile):
-    """The reference's test/1D/Ein91 inputs (HLLE), 100 steps."""
-    c = caseio.load_case(EIN91, 100)
+def test_hlle_cases_device_matches_oracle(gpu_ready, name, path, tile):
+    """The reference's test/1D/Ein91 and Noh87 inputs (HLLE), 100 steps."""
+    c = caseio.load_case(os.path.join(os.path.dirname(EIN91), name), 100)
     ctx = Context(c.tables)
     dc = DeviceCase(ctx, c)
     dc.upload(c.P)
@@ -170,6 +171,6 @@ def test_ein91_hlle_device_matches_oracle(gpu_ready, path, tile):
     assert abs(rep.time - dts.sum()) <= 1e-13 * dts.sum()
     a, b = c.layout.interior(Pg, 0), c.layout.interior(Po, 0)
     err = max(np.abs(a[s] - b[s]).max() / max(np.abs(b[s]).max(), 1.0) for s in range(c.nprim))
-    print(f"ein91 HLLE (reference inputs), path {path} tile {chosen} x 100: rel Linf {err:.3e}, values differing "
+    print(f"{name} HLLE (reference inputs), path {path} tile {chosen} x 100: rel Linf {err:.3e}, values differing "
           f"bitwise {int(np.sum(a != b))} of {a.size}", flush=True)
     assert err <= TOL

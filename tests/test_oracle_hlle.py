@@ -5,6 +5,8 @@
   one is negative (S4 = 0) the right state's.
 - The reference's own test/1D/Ein91 inputs (Einfeldt's 1-2-0-3 double rarefaction, HLLE + van Leer, RK3 at CFL 0.9)
   run to t = 0.15 within an L1 density error of 0.02 of the exact solution.
+- The reference's own test/1D/Noh87 inputs (HLLE, two cold streams colliding): the shocked gas holds the closed-form
+  density (gamma + 1) / (gamma - 1) = 6 between the two shocks, which run outward at (gamma - 1) / 2 |u0|.
 """
 import os
 
@@ -73,3 +75,22 @@ def test_ein91_against_exact():
     t, err, _ = _ein91_error(c)
     assert abs(t - 0.15) < 0.01
     assert err < 0.02, err
+
+
+NOH87 = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cases", "noh87")
+
+
+def test_noh87_shocked_density():
+    c = caseio.load_case(NOH87, 200)
+    P = c.P.copy()
+    t = OracleCase(c).step(P, 200).sum()
+    x = 0.5 * (c.nodes[0][0, 0, 1:, 0] + c.nodes[0][0, 0, :-1, 0])
+    V0 = c.layout.interior(c.P, 0)[:, 0, 0, :]
+    u0 = abs(V0[1, 0])
+    x0 = x[np.argmax(np.diff(V0[1]) != 0) + 1] - 0.5 * (x[1] - x[0])
+    W = c.layout.interior(P, 0)[:, 0, 0, :]
+    xs = (GAM - 1.0) / 2.0 * u0 * t                    # shock distance from the centre
+    band = (np.abs(x - x0) > 0.3 * xs) & (np.abs(x - x0) < 0.8 * xs)
+    assert band.sum() >= 6
+    assert abs(np.median(W[0][band]) - (GAM + 1.0) / (GAM - 1.0)) <= 0.03 * 6.0
+    assert np.all(np.abs(W[0][np.abs(x - x0) > 1.3 * xs + 0.02] - 1.0) < 0.05)   # unshocked gas ahead of the shocks
