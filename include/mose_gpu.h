@@ -285,10 +285,17 @@ int mose_chem_destroy(mose_ctx* ctx, int chem_id);
 int mose_chem_upload(mose_ctx* ctx, int chem_id, int64_t ncell, const double* state);
 int mose_chem_download(mose_ctx* ctx, int chem_id, double* state, double* hr, int32_t* idid, int32_t* nstep);
 /* Every cell: T = p / (rho R), RODAS or RADAU5 over [0, dt] on (rho_1..rho_ns, T), species floored at 1e-20,
- * p = T rho R (Chemistry_Newstate_blk); hr = |T_new - T_old| / T_old.  One persistent launch: each team of 16 lanes
- * takes the next cell from an atomic work queue when its cell is done, so cells of uneven stiffness share the device
- * evenly. */
+ * p = T rho R (Chemistry_Newstate_blk); hr = |T_new - T_old| / T_old.  One persistent launch: each team of ns + 1
+ * lanes takes the next cell from an atomic work queue when its cell is done, so cells of uneven stiffness share the
+ * device evenly.  The controllers' real powers (ERR**0.25, FACCON**0.8, QNEWT**p) and the Troe factor's 10**x are
+ * glibc's pow bit for bit (mose_pow_batch), so the step sequences are those of the reference's libm-linked Fortran. */
 int mose_chem_advance(mose_ctx* ctx, int chem_id, double dt, const mose_ode_opts* opts, mose_chem_report* report);
+
+/* out[i] = x[i]**y[i] on the device with the chemistry kernels' pow (host arrays of n values): the bits of the host
+ * libm pow that the reference's Fortran `**` calls (glibc 2.35 on x86-64 with FMA; hairerRodas.f:805,816,
+ * hairerRadau5.f:918,942,969, Lib_Chemistry_falloff.f90:78).  Exposed so that a host can check the device's
+ * powers against its own libm. */
+int mose_pow_batch(mose_ctx* ctx, int64_t n, const double* x, const double* y, double* out);
 
 #ifdef __cplusplus
 }

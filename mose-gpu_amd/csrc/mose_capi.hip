@@ -1108,6 +1108,33 @@ extern "C" int mose_bandwidth_probe(mose_ctx* ctx, int64_t bytes, int reps, doub
 #include "mose_chem.hpp"
 #include "mose_radau.hpp"
 
+__global__ void k_pow(const double* __restrict__ x, const double* __restrict__ y, double* __restrict__ out, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = mose_pow(x[i], y[i]);
+}
+
+extern "C" int mose_pow_batch(mose_ctx* ctx, int64_t n, const double* x, const double* y, double* out) {
+  if (!ctx || n < 0 || (n > 0 && (!x || !y || !out))) return MOSE_ERR_ARG;
+  if (n == 0) return MOSE_OK;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  double* d = nullptr;
+  const size_t bytes = (size_t)n * sizeof(double);
+  if (hipMalloc(&d, 3 * bytes) != hipSuccess) return fail(ctx, MOSE_ERR_NOMEM, "pow buffers");
+  hipStream_t s = ctx->stream;
+  hipError_t e = hipMemcpyAsync(d, x, bytes, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(d + n, y, bytes, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) {
+    const int64_t blocks = (n + 255) / 256;
+    k_pow<<<(unsigned)(blocks < 65536 ? blocks : 65536), 256, 0, s>>>(d, d + n, d + 2 * n, n);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(out, d + 2 * n, bytes, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(ctx, MOSE_ERR_HIP, hipGetErrorString(e));
+  return MOSE_OK;
+}
+
 namespace {
 constexpr int kChemTeamsPerWg = TeamShape<9>::per_wg;   // the kernels are instantiated for 9 species
 struct ChemObj {

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include "mose_internal.h"
+#include "mose_libm.h"
 
 namespace mose {
 
@@ -61,12 +62,9 @@ __device__ __forceinline__ double chem_ipow(double x, int n) {
 }
 
 
-/* Deterministic natural log and exp for the Troe broadening factor: the same statements on the CPU and the GPU give
- * the same bits (the libm log10 / pow of the two platforms differ in the last ulp, and the adaptive integrator turns
- * an ulp into a different step sequence).  log: x = m 2^e with m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s),
- * s = (m-1)/(m+1), odd series to s^23; exp: y = k ln2 + r, |r| <= ln2/2, Taylor series to r^16 (Horner, with the
- * rounded reciprocals 1/k), times 2^k.
- * Both are within a few ulp of the true values for the arguments met here (Fcent in (0, 1], Pr > 0). */
+/* Deterministic natural log for the Troe broadening factor's log10, the statements of the host side: x = m 2^e with
+ * m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s), s = (m-1)/(m+1), odd series to s^23.  Within a few ulp of the true
+ * value for the arguments met here (Fcent in (0, 1], Pr > 0). */
 __device__ __forceinline__ double dm_log(double x) {
   int e;
   double m = frexp(x, &e);            /* x = m 2^e, m in [0.5, 1) */
@@ -76,28 +74,21 @@ __device__ __forceinline__ double dm_log(double x) {
   double lm = 2.0 * s * (1.0 + t);
   return (double)e * 0.69314718055994528623 + ((double)e * 2.3190468138462996154e-17 + lm);
 }
-__device__ __forceinline__ double dm_exp(double y) {
-  double kf = floor(y * 1.44269504088896340736 + 0.5);
-  double r = (y - kf * 0.69314718055994528623) - kf * 2.3190468138462996154e-17;
-  double t = 1.0;
-  for (int k = 16; k >= 1; --k) t = 1.0 + t * r * (1.0 / (double)k);
-  return ldexp(t, (int)kf);
-}
 
 constexpr double DM_LN10 = 2.30258509299404568402;
 
-// f_F (Lib_Chemistry_falloff.f90:63-80); log10 and 10** through dm_log / dm_exp, the statements of the host side
+// f_F (Lib_Chemistry_falloff.f90:63-80); log10 through dm_log, 10** through mose_pow (glibc's pow, as the reference)
 __device__ __forceinline__ double troe_F(double Pr, double Fcent) {
   double lF = dm_log(Fcent) / DM_LN10, lP = dm_log(Pr) / DM_LN10;
   double cc = -0.4 - 0.67 * lF, nn = 0.75 - 1.27 * lF;
   double f1 = (lP + cc) / (nn - 0.14 * (lP + cc));
-  return dm_exp(lF / (1.0 + f1 * f1) * DM_LN10);
+  return mose_pow(10.0, lF / (1.0 + f1 * f1));
 }
 
 // c**nu of the falloff classes: integer power for integral nu, pow otherwise
 __device__ __forceinline__ double rpow(double c, double nu) {
   double r = rint(nu);
-  return (r == nu && r >= 0.0 && r <= 8.0) ? chem_ipow(c, (int)r) : pow(c, nu);
+  return (r == nu && r >= 0.0 && r <= 8.0) ? chem_ipow(c, (int)r) : mose_pow(c, nu);
 }
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -607,12 +598,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
 #pragma unroll
         for (int i = 0; i < NZ; ++i) err = err + L.q[i];
         err = sqrt(err / NZ);
-        double fac = fmax(fac2, fmin(fac1, sqrt(sqrt(err)) / safe));   // err**0.25, as the host side
+        double fac = fmax(fac2, fmin(fac1, mose_pow(err, 0.25) / safe));   // err**0.25 (libm pow)
         double hnew = h / fac;
         if (err <= 1.0) {
           nacc++;
           if (nacc > 1) {
-            double facgus = (hacc / h) * sqrt(sqrt(err * err / erracc)) / safe;
+            double facgus = (hacc / h) * mose_pow(err * err / erracc, 0.25) / safe;
             facgus = fmax(fac2, fmin(fac1, facgus));
             fac = fmax(fac, facgus);
             hnew = h / fac;

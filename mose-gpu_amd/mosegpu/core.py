@@ -158,6 +158,7 @@ def lib() -> C.CDLL:
     L.mose_chem_upload.argtypes = [vp, ip, C.c_int64, dp]
     L.mose_chem_download.argtypes = [vp, ip, dp, dp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     L.mose_chem_advance.argtypes = [vp, ip, C.c_double, C.POINTER(OdeOpts), C.POINTER(ChemReport)]
+    L.mose_pow_batch.argtypes = [vp, C.c_int64, dp, dp, dp]
     for name in ("mose_ctx_create", "mose_ctx_destroy", "mose_metrics_build", "mose_records_tangent", "mose_case_create",
                  "mose_case_destroy", "mose_case_bytes", "mose_state_upload", "mose_state_download",
                  "mose_case_step", "mose_cases_step", "mose_case_launch", "mose_case_poll", "mose_case_residual",
@@ -300,6 +301,16 @@ class Context:
         v = C.c_double(0.0)
         self.check(lib().mose_bandwidth_probe(self.h, int(nbytes), int(reps), C.byref(v)), "mose_bandwidth_probe")
         return v.value
+
+    def pow_batch(self, x: np.ndarray, y: np.ndarray) -> np.ndarray:
+        """x**y elementwise on the device with the chemistry kernels' pow (the host libm's bits)."""
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.ascontiguousarray(y, np.float64)
+        if x.shape != y.shape:
+            raise ValueError("pow_batch: x and y differ in shape")
+        out = np.empty_like(x)
+        self.check(lib().mose_pow_batch(self.h, x.size, dptr(x), dptr(y), dptr(out)), "mose_pow_batch")
+        return out
 
     def close(self):
         if self.h:

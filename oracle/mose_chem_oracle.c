@@ -21,10 +21,12 @@
  *                  of radau5_dev (Lib_Radau5_dev.f90:199-652); tolerance transform set_radau5_dev_tols (:71-107)
  *   oc_advance     Chemistry_Newstate_blk    src/lib/physics/Lib_Chemistry.f90:33-76
  *
- * Deliberate ulp-level deviations, shared with the device so that both take the same step sequence:
- * x**0.25 of the step controllers as sqrt(sqrt(x)); log10 / 10** of the Troe factor and the real powers of the
- * RADAU5 Newton controller (faccon**0.8, qnewt**(-1/(nit+3-newt))) through dm_log / dm_exp; the falloff
- * concentration powers and theta**(nit-1-newt) as integer powers.  Parity with the reference's own libm is unpinned at the ulp level.
+ * Real powers x**y (the step controllers' err**0.25, faccon**0.8 and qnewt**(-1/(nit+3-newt)), the Troe factor's
+ * 10**x, non-integral rate orders) go through mose_pow (mose-gpu_amd/csrc/mose_libm.h), the device's own source:
+ * glibc's pow restated bit for bit, so oracle, device and the reference's libm-linked Fortran take the same step
+ * sequences.  Integral powers (rate orders, theta**(nit-1-newt)) are repeated squaring, as the Fortran run time.
+ * log10 of the Troe factor is dm_log(x) / ln 10 on both sides: within a few ulp of libm's log10 (parity with the
+ * reference's libm log10 is unpinned at the ulp level; the reference's general-mechanism source is not built here).
  */
 #include <math.h>
 #include <stdint.h>
@@ -32,6 +34,7 @@
 #include <string.h>
 
 #include "../include/mose_gpu.h"
+#include "../mose-gpu_amd/csrc/mose_libm.h"
 
 #define CNS MOSE_MAX_SPECIES
 #define CNZ (MOSE_MAX_SPECIES + 1)
@@ -93,12 +96,9 @@ static double ipow(double x, int n) {
 }
 
 
-/* Deterministic natural log and exp for the Troe broadening factor: the same statements on the CPU and the GPU give
- * the same bits (the libm log10 / pow of the two platforms differ in the last ulp, and the adaptive integrator turns
- * an ulp into a different step sequence).  log: x = m 2^e with m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s),
- * s = (m-1)/(m+1), odd series to s^23; exp: y = k ln2 + r, |r| <= ln2/2, Taylor series to r^16 (Horner, with the
- * rounded reciprocals 1/k), times 2^k.
- * Both are within a few ulp of the true values for the arguments met here (Fcent in (0, 1], Pr > 0). */
+/* Deterministic natural log for the Troe broadening factor's log10: the same statements on the CPU and the GPU give
+ * the same bits.  x = m 2^e with m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s), s = (m-1)/(m+1), odd series to
+ * s^23.  Within a few ulp of the true value for the arguments met here (Fcent in (0, 1], Pr > 0). */
 static inline double dm_log(double x) {
   int e;
   double m = frexp(x, &e);            /* x = m 2^e, m in [0.5, 1) */
@@ -108,29 +108,22 @@ static inline double dm_log(double x) {
   double lm = 2.0 * s * (1.0 + t);
   return (double)e * 0.69314718055994528623 + ((double)e * 2.3190468138462996154e-17 + lm);
 }
-static inline double dm_exp(double y) {
-  double kf = floor(y * 1.44269504088896340736 + 0.5);
-  double r = (y - kf * 0.69314718055994528623) - kf * 2.3190468138462996154e-17;
-  double t = 1.0;
-  for (int k = 16; k >= 1; --k) t = 1.0 + t * r * (1.0 / (double)k);
-  return ldexp(t, (int)kf);
-}
 
 #define DM_LN10 2.30258509299404568402
 
-/* f_F: the Troe broadening factor (Lib_Chemistry_falloff.f90:63-80); log10 and 10** through dm_log / dm_exp */
+/* f_F: the Troe broadening factor (Lib_Chemistry_falloff.f90:63-80); log10 through dm_log, 10** through mose_pow */
 static double troe_F(double Pr, double Fcent) {
   double lF = dm_log(Fcent) / DM_LN10, lP = dm_log(Pr) / DM_LN10;
   double cc = -0.4 - 0.67 * lF, nn = 0.75 - 1.27 * lF;
   double f1 = (lP + cc) / (nn - 0.14 * (lP + cc));
-  return dm_exp(lF / (1.0 + f1 * f1) * DM_LN10);
+  return mose_pow(10.0, lF / (1.0 + f1 * f1));
 }
 
 /* c**nu of the falloff classes: the integer power for integral nu (the run-time real power of the reference equals
  * it up to an ulp), pow otherwise */
 static double rpow(double c, double nu) {
   double r = nearbyint(nu);
-  return (r == nu && r >= 0.0 && r <= 8.0) ? ipow(c, (int)r) : pow(c, nu);
+  return (r == nu && r >= 0.0 && r <= 8.0) ? ipow(c, (int)r) : mose_pow(c, nu);
 }
 
 /* general(): omegadot(s) = sum_r Wm_s (ni2 - ni1) (rate_fwd - rate_rev) */
@@ -278,15 +271,17 @@ void oc_sol(int n, const double *a, double *b, const int *ip) {
 #undef A
 }
 
-/* The real powers of the step controllers.  By default the deterministic forms shared with the device
- * (x**0.25 as sqrt(sqrt(x)), other real powers through dm_log / dm_exp); with oc_set_reference_pow(1) the libm pow
- * of the reference's statements, so that the integrators can be compared bit for bit with the reference's own
- * Fortran (oracle/_ref).  Set it before a run, not during one. */
+/* The real powers of the step controllers (ERR**0.25, FACCON**0.8, QNEWT**(-1/(4+NIT-1-NEWT))).  By default
+ * mose_pow, the device's restatement of glibc's pow; with oc_set_reference_pow(1) the host's own libm pow, the call
+ * the reference's compiled Fortran makes.  The two give the same bits (tests/test_libm.py), so both modes take the
+ * reference's step sequences.  Mode 2 takes x**0.25 as sqrt(sqrt(x)), a planted deviation for the tests that show
+ * the comparison can fail.  Set it before a run, not during one. */
 static int g_ref_pow = 0;
 void oc_set_reference_pow(int on) { g_ref_pow = on; }
-static double pow_q(double x) { return g_ref_pow ? pow(x, 0.25) : sqrt(sqrt(x)); }
-static double dm_pow(double x, double p);
-static double pow_r(double x, double p) { return g_ref_pow ? pow(x, p) : dm_pow(x, p); }
+static double pow_q(double x) {
+  return g_ref_pow == 1 ? pow(x, 0.25) : g_ref_pow == 2 ? sqrt(sqrt(x)) : mose_pow(x, 0.25);
+}
+static double pow_r(double x, double p) { return g_ref_pow == 1 ? pow(x, p) : mose_pow(x, p); }
 
 /* ---------------------------------------------------------------------------------------------------------- */
 /* RODAS (METH = 1), autonomous, numerical full Jacobian, B = identity                                          */
@@ -545,9 +540,6 @@ void oc_solc(int n, const double *ar, const double *ai, double *br, double *bi, 
 /* ---------------------------------------------------------------------------------------------------------- */
 /* RADAU5 (Radau IIA, order 5), autonomous, numerical full Jacobian, B = identity                               */
 /* ---------------------------------------------------------------------------------------------------------- */
-/* x**p for the Newton controller (faccon**0.8, qnewt**(-1/(nit+3-newt))) through dm_log / dm_exp */
-static double dm_pow(double x, double p) { return dm_exp(p * dm_log(x)); }
-
 /* The ITOL = 1 tolerance transform of the RADAU5 driver and FNEWT (hairerRadau5.f:430-505; set_radau5_dev_tols,
  * Lib_Radau5_dev.f90:71-107).  Host libm pow, computed once per call. */
 void oc_radau5_tols(int n, const double *rt, const double *at, double *rtolt, double *atolt, double *fnewt) {
@@ -936,4 +928,9 @@ int oc_advance_m(void *h, int method, int64_t ncell, double *state, double dt, c
 int oc_advance(void *h, int64_t ncell, double *state, double dt, const double *rtol, const double *atol, double *hr,
                int32_t *idid, int32_t *nstep, int nthreads) {
   return oc_advance_m(h, MOSE_ODE_RODAS, ncell, state, dt, rtol, atol, hr, idid, nstep, nthreads);
+}
+
+/* x[i]**y[i] through mose_pow (mode 0) or the host's libm pow (mode 1): the check that the two are the same bits */
+void oc_pow_batch(int mode, int64_t n, const double *x, const double *y, double *out) {
+  for (int64_t i = 0; i < n; ++i) out[i] = mode ? pow(x[i], y[i]) : mose_pow(x[i], y[i]);
 }

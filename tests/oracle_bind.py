@@ -64,6 +64,7 @@ def oracle_lib():
     L.oc_advance_m.argtypes = [vp, C.c_int, C.c_int64, dp, C.c_double, dp, dp, dp, C.POINTER(C.c_int32),
                                C.POINTER(C.c_int32), C.c_int]
     L.oc_set_reference_pow.argtypes = [C.c_int]
+    L.oc_pow_batch.argtypes = [C.c_int, C.c_int64, dp, dp, dp]
     L.oc_advance.restype = C.c_int
     L.oc_advance.argtypes = [vp, C.c_int64, dp, C.c_double, dp, dp, dp, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                              C.c_int]

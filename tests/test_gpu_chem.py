@@ -1,8 +1,10 @@
-"""Parity of the device chemistry (persistent RODAS kernel through the C ABI) with the chemistry oracle.
+"""Parity of the device chemistry (persistent RODAS and RADAU5 kernels through the C ABI) with the chemistry oracle.
 
-The bar for chemistry is a relative L-infinity <= 1e-8 per variable. Adaptive steps may order differently when
-pow/log10 differ in the last ulp between the device and glibc. At the larger batch size the checks are size-free
-invariants: the species mass, and the table internal energy within the tables' resolution.
+The north star's bar for chemistry is a relative L-infinity <= 1e-8 per variable.  The device's controller powers
+are glibc's pow bit for bit (mose_pow: checked here on the device against the host's libm), and the oracle equals
+the reference's compiled Hairer integrators bit for bit (test_oracle_vs_hairer.py), so the device must equal the
+oracle exactly: same states, same idid, same step counts.  At the larger batch size the checks are size-free
+invariants: the species mass.
 """
 import numpy as np
 import pytest
@@ -43,7 +45,8 @@ def test_chem_parity_with_oracle(gpu_ready):
     print(f"chemistry 384 cells: rel Linf {err:.3e}, identical step counts {same_steps:.3f}, "
           f"steps {rep.steps_total}, kernel {rep.ms:.1f} ms", flush=True)
     assert err <= TOL
-    assert np.allclose(hr, hr_o, rtol=1e-6, atol=1e-12)
+    assert np.array_equal(dev, ref) and np.array_equal(nstep, nstep_o)
+    assert np.array_equal(hr, hr_o)
 
 
 def test_chem_invariants_large_batch(gpu_ready):
@@ -73,7 +76,8 @@ def test_radau5_parity_with_oracle(gpu_ready):
     print(f"radau5 384 cells: rel Linf {err:.3e}, identical step counts {same_steps:.3f}, failed {int((~ok).sum())}, "
           f"kernel {rep.ms:.1f} ms", flush=True)
     assert err <= TOL
-    assert np.allclose(hr[ok], hr_o[ok], rtol=1e-6, atol=1e-12)
+    assert np.array_equal(dev, ref) and np.array_equal(nstep, nstep_o)
+    assert np.array_equal(hr[ok], hr_o[ok])
 
 
 def run_device_allow_fail(state, dt, rtol=1e-8, atol=1e-12, method=2):
@@ -120,3 +124,22 @@ def test_chem_argument_errors(gpu_ready):
         dc.advance(-1e-6, chem.ode_opts())         # dt must be positive
     dc.destroy()
     ctx.close()
+
+
+@pytest.mark.parametrize("kind", ["controllers", "troe", "general", "bits", "negative_integer_y", "subnormal_x",
+                                  "near_over_underflow", "near_one"])
+def test_device_pow_is_the_host_pow(gpu_ready, kind):
+    """The device's pow (mose_pow, compiled for gfx950) against the host's libm pow on 1e6 arguments of each class:
+    the same bits."""
+    from oracle_bind import _d, oracle_lib
+    from test_libm import KINDS, arguments, same_bits
+    rng = np.random.default_rng(1000 + KINDS.index(kind))
+    x, y = arguments(kind, 1_000_000, rng)
+    ctx = Context(chem_tables())
+    dev = ctx.pow_batch(x, y)
+    ctx.close()
+    host = np.empty_like(x)
+    oracle_lib().oc_pow_batch(1, x.size, _d(x), _d(y), _d(host))
+    bad = ~same_bits(dev, host)
+    print(f"device pow, {kind}: {int(bad.sum())} of {x.size} differ from the host libm", flush=True)
+    assert bad.sum() == 0, (x[bad][:4], y[bad][:4], dev[bad][:4], host[bad][:4])
