@@ -320,6 +320,26 @@ bool connect_metrics(const BlockGeo& Bm, const BlockGeo& Bs, const mose_bc_recor
   return ok;
 }
 
+// BC_Symmetry_Metrics (Lib_Metrics.f90:1044-1102): ghost g of a symmetry face takes the metric, the lengths and the
+// volume of its mirror image, the interior cell g - 1 away from the face.
+void symmetry_metrics(const BlockGeo& B, int Im, int Jm, int Km, int Fm, const Out& o) {
+  for (int g = 1; g <= GC; ++g) {
+    int gi = Im, gj = Jm, gk = Km, si = Im, sj = Jm, sk = Km;
+    switch (Fm) {
+      case 1: gi = Im - g; si = Im + g - 1; break;
+      case 2: gi = Im + g; si = Im - g + 1; break;
+      case 3: gj = Jm - g; sj = Jm + g - 1; break;
+      case 4: gj = Jm + g; sj = Jm - g + 1; break;
+      case 5: gk = Km - g; sk = Km + g - 1; break;
+      default: gk = Km + g; sk = Km - g + 1; break;
+    }
+    const int64_t rg = B.row(gi, gj, gk), rs = B.row(si, sj, sk);
+    for (int q = 0; q < 9; ++q) o.m[q * o.NC + rg] = o.m[q * o.NC + rs];
+    for (int h = 0; h < 3; ++h) o.dl[h * o.NC + rg] = o.dl[h * o.NC + rs];
+    o.vol[rg] = o.vol[rs];
+  }
+}
+
 }  // namespace
 
 extern "C" int mose_metrics_build(int nb, const int* dims, const double* nodes, int nrec, const mose_bc_record* rec,
@@ -363,6 +383,9 @@ extern "C" int mose_metrics_build(int nb, const int* dims, const double* nodes, 
     if (r.typ == MOSE_BC_CONNECTION) {
       if (r.bs < 0 || r.bs >= nb) return MOSE_ERR_ARG;
       ok &= connect_metrics(B[r.b], B[r.bs], r, o);
+    } else if (r.typ == MOSE_BC_SYMMETRY) {
+      if (r.f < 1 || r.f > 6) return MOSE_ERR_ARG;
+      symmetry_metrics(B[r.b], r.i, r.j, r.k, r.f, o);
     } else {
       ok &= extrapolate_metrics(B[r.b], ndir, r.i, r.j, r.k, r.f, o);
     }

@@ -317,16 +317,17 @@ def channel_lattice(nb=(4, 4, 4), wmin: int = 24, wmax: int = 96, seed: int = 7,
 
 def box_case(dims, faces, extent=(1.0, 1.0, 1.0), nrans: int = 0, viscous: bool = True, mu_of_T=None,
              prandtl: float = 0.72, farfield=None, T_wall: float = 300.0, ndir: int = 3, nsteps: int = 1,
-             **scheme) -> Case:
+             nodes=None, **scheme) -> Case:
     """One Cartesian block [0, extent] with a boundary type per face: faces[f] (f = 1..6) is a type id, or "periodic"
     for a connection to the opposite face (both faces of the pair must say so).  farfield: one mose_farfield row
-    (FARFIELD_DTYPE) used by every type-4 face.  The state is left zero (set it with layout.interior)."""
+    (FARFIELD_DTYPE) used by every type-4 face.  nodes: the block's nodes (shape (n3+1, n2+1, n1+1, 3)) in place of
+    the uniform Cartesian ones.  The state is left zero (set it with layout.interior)."""
     n1, n2, n3 = dims
     nprim = 5 + nrans
     tables = air_tables(mu_of_T if mu_of_T is not None else (sutherland(1.8e-5, 300.0) if viscous else None),
                         prandtl=prandtl)
     layout = Layout([dims])
-    nodes = [box_nodes(n1, n2, n3, (0.0, 0.0, 0.0), extent)]
+    nodes = [box_nodes(n1, n2, n3, (0.0, 0.0, 0.0), extent) if nodes is None else np.ascontiguousarray(nodes)]
     opp = {1: (2, lambda i, j, k: (np.full_like(i, n1), j, k)), 2: (1, lambda i, j, k: (np.full_like(i, 1), j, k)),
            3: (4, lambda i, j, k: (i, np.full_like(j, n2), k)), 4: (3, lambda i, j, k: (i, np.full_like(j, 1), k)),
            5: (6, lambda i, j, k: (i, j, np.full_like(k, n3))), 6: (5, lambda i, j, k: (i, j, np.full_like(k, 1)))}

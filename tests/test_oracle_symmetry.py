@@ -31,8 +31,23 @@ def _faces(axis, sym, walls):
     return f
 
 
-def mirror_pair(axis, half_dims, viscous, nrans=0, walls=None, riemann="HLLC", limiter="vanleer", seed=5):
-    """The half box with symmetry faces on `axis` and the full periodic box, both at the same mirror-symmetric state."""
+def stretched_nodes(dims, ext, axis, half_n):
+    """Nodes of a box whose spacing along `axis` grows geometrically (ratio 1.35) from x = 0 over the first half_n
+    cells and mirrors about x = L after them (a full box of 2 half_n cells), uniform along the other axes."""
+    n1, n2, n3 = dims
+    w = 1.35 ** np.arange(half_n)
+    xh = np.concatenate([[0.0], np.cumsum(w)]) / w.sum() * ext[axis]
+    x = xh if dims[axis] == half_n else np.concatenate([xh, 2 * ext[axis] - xh[-2::-1]])
+    axes = [np.arange(n + 1, dtype=np.float64) / n * e for n, e in zip((n1, n2, n3), ext)]
+    axes[axis] = x
+    Z, Y, X = np.meshgrid(axes[2], axes[1], axes[0], indexing="ij")
+    return np.stack([X, Y, Z], axis=-1)
+
+
+def mirror_pair(axis, half_dims, viscous, nrans=0, walls=None, riemann="HLLC", limiter="vanleer", seed=5,
+                stretch=False):
+    """The half box with symmetry faces on `axis` and the full periodic box, both at the same mirror-symmetric state.
+    stretch: the cells grow along `axis` away from x = 0 and the full box is the half box and its mirror image."""
     full_dims = list(half_dims)
     full_dims[axis] *= 2
     ext = [1.0, 0.8, 0.6]
@@ -40,8 +55,12 @@ def mirror_pair(axis, half_dims, viscous, nrans=0, walls=None, riemann="HLLC", l
     full_ext[axis] *= 2
     kw = dict(nrans=nrans, viscous=viscous, riemann=riemann, limiter=limiter, ndir=3,
               mu_of_T=cases.sutherland(1.8e-3, 300.0) if viscous else None)
-    half = cases.box_case(tuple(half_dims), _faces(axis, True, walls), extent=tuple(ext), **kw)
-    full = cases.box_case(tuple(full_dims), _faces(axis, False, walls), extent=tuple(full_ext), **kw)
+    nh = nf = None
+    if stretch:
+        nh = stretched_nodes(half_dims, ext, axis, half_dims[axis])
+        nf = stretched_nodes(full_dims, ext, axis, half_dims[axis])
+    half = cases.box_case(tuple(half_dims), _faces(axis, True, walls), extent=tuple(ext), nodes=nh, **kw)
+    full = cases.box_case(tuple(full_dims), _faces(axis, False, walls), extent=tuple(full_ext), nodes=nf, **kw)
     rng = np.random.Generator(np.random.PCG64(seed))
     n3, n2, n1 = half_dims[2], half_dims[1], half_dims[0]
     shape = (n3, n2, n1)
@@ -121,3 +140,34 @@ def test_symmetry_mistakes_are_seen():
     cases.fill_ghosts_nearest(half.layout, half.P)
     err = _compare(half, full, 0, 3)
     assert np.max(err) > 1e-6, err
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+@pytest.mark.parametrize("viscous", [False, True])
+def test_symmetry_face_is_a_mirror_on_a_stretched_mesh(axis, viscous):
+    """The mirror check on cells that grow by 1.35 from face to face along the mirrored axis: the ghosts behind a
+    symmetry face carry the metric, lengths and volume of their mirror images (BC_Symmetry_Metrics,
+    Lib_Metrics.f90:1044-1102), as the full box's cells across the mirror plane do.  Extrapolated ghost lengths would
+    change the limited slopes of the first faces."""
+    half, full = mirror_pair(axis, [6, 5, 4], viscous, riemann="HLLC" if viscous else "LMRoe", stretch=True)
+    err = _compare(half, full, axis, 3)
+    assert np.max(err) <= 1e-12, err
+
+
+def test_symmetry_ghost_metrics_are_mirror_images():
+    """Ghost g behind each symmetry face holds interior cell g - 1's lengths, metric and volume (g = 1, 2), not the
+    cubic node extrapolation of the other boundary types."""
+    half, _ = mirror_pair(0, [6, 5, 4], False, stretch=True)
+    lay, g = half.layout, half.geom
+    dl = lay.block_view(g.dl.reshape(3, -1), 0)            # [3, k, j, i] padded
+    m = lay.block_view(g.m.reshape(9, -1), 0)
+    vol = lay.block_view(g.vol.reshape(1, -1), 0)
+    n1 = half.layout.dims[0][0]
+    for q in (1, 2):                            # ghost q before the first cell and after the last
+        lo, src_lo = GC - q, GC + q - 1
+        hi, src_hi = GC + n1 - 1 + q, GC + n1 - q
+        for arr in (dl, m, vol):
+            assert np.array_equal(arr[:, GC:-GC, GC:-GC, lo], arr[:, GC:-GC, GC:-GC, src_lo])
+            assert np.array_equal(arr[:, GC:-GC, GC:-GC, hi], arr[:, GC:-GC, GC:-GC, src_hi])
+    # the stretched spacing is not what an extrapolation gives: the first ghost is as long as the first cell
+    assert dl[0, GC, GC, GC - 1] != dl[0, GC, GC, GC + 1]
