@@ -291,6 +291,23 @@ int mose_chem_download(mose_ctx* ctx, int chem_id, double* state, double* hr, in
  * glibc's pow bit for bit (mose_pow_batch), so the step sequences are those of the reference's libm-linked Fortran. */
 int mose_chem_advance(mose_ctx* ctx, int chem_id, double dt, const mose_ode_opts* opts, mose_chem_report* report);
 
+/* The largest cell count of one chemistry batch: the work queue hands out 32-bit cell indices. */
+#define MOSE_CHEM_CELLS_MAX 2147483647LL
+
+/* mose_chem_advance on cells that the caller holds in device memory, allocating nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  state = [ns + 1][stride] doubles (species densities, then pressure;
+ * cell c at column c, 1 <= ncell <= stride), hr = [ncell] doubles, idid and nstep = [ncell] int32, all device
+ * memory of the context's GPU (host pointers are refused); chem_id supplies the mechanism (mose_chem_create) and
+ * its batch buffer is not touched.  Same kernels, same results as mose_chem_advance on the same cells.
+ * The work queue and the statistics live in a 64-byte file-scope device array of the library, one per GPU, held
+ * until the process ends and cleared by each call: calls that overlap on one GPU race.  ncell above
+ * MOSE_CHEM_CELLS_MAX is refused (MOSE_ERR_ARG).  The call does not call hipMalloc, hipFree or any allocating
+ * routine; it waits for its own launch (the report's counters). */
+int mose_chem_advance_cells(mose_ctx* ctx, int chem_id, int64_t ncell, int64_t stride, double* state, double* hr,
+                            int32_t* idid, int32_t* nstep, double dt, const mose_ode_opts* opts,
+                            mose_chem_report* report);
+
 /* out[i] = x[i]**y[i] on the device with the chemistry kernels' pow (host arrays of n values): the bits of the host
  * libm pow that the reference's Fortran `**` calls (glibc 2.35 on x86-64 with FMA; hairerRodas.f:805,816,
  * hairerRadau5.f:918,942,969, Lib_Chemistry_falloff.f90:78).  Exposed so that a host can check the device's

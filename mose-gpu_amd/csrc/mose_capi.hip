@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -109,6 +110,21 @@ struct mose_ctx {
     }                                                                                 \
   } while (0)
 
+// The scratch of mose_chem_advance_cells, the entry point that allocates nothing: the chemistry kernels' work queue
+// and batch statistics ([0] the next cell, [1..6] ChemArgs::stats).  Each GPU the library runs on holds its own copy
+// of these 64 bytes until the process ends.  On the host the array's name is not a device address: mose_ctx_create
+// looks up the address once per device (hipGetSymbolAddress, with that device current) into g_scratch, and every use
+// goes through g_scratch.  Calls of mose_chem_advance_cells that overlap on one GPU share it and race.
+constexpr int kScratchWords = 8;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned long long scratch[kScratchWords];
+static std::mutex g_scratch_mu;
+static unsigned long long* g_scratch[kMaxDevices] = {};
+
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0ull;
+}
+
 static int fail(mose_ctx* ctx, int code, const std::string& msg) {
   if (ctx) ctx->err = msg;
   return code;
@@ -147,6 +163,15 @@ extern "C" int mose_ctx_create(int device, const mose_tables* t, mose_ctx** out)
   ctx->device = device;
   ctx->nsc = t->nsc;
   HIPCHK(ctx.get(), hipSetDevice(device));
+  if (device < kMaxDevices) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[device]) {
+      unsigned long long* p = nullptr;
+      HIPCHK(ctx.get(), hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch)));
+      if (!p) return fail(ctx.get(), MOSE_ERR_HIP, "hipGetSymbolAddress(scratch) gave no address");
+      g_scratch[device] = p;
+    }
+  }
   HIPCHK(ctx.get(), hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   {
     int v = 0;
@@ -1146,7 +1171,7 @@ struct ChemObj {
   int* nstep = nullptr;
   unsigned long long* ctr = nullptr;   // [0] queue, [1..6] stats
   int64_t cap = 0, n = 0;
-  int grid = 0;
+  int grid = 0, cus = 0;
   hipEvent_t ev[2] = {nullptr, nullptr};
   bool valid = false;
 };
@@ -1193,6 +1218,7 @@ static void chem_free(ChemObj* o) {
 
 extern "C" int mose_chem_create(mose_ctx* ctx, const mose_mechanism* m, int64_t ncell_max, int* chem_id) {
   if (!ctx || !m || !chem_id || ncell_max < 1) return MOSE_ERR_ARG;
+  if (ncell_max > MOSE_CHEM_CELLS_MAX) return fail(ctx, MOSE_ERR_ARG, "mose_chem_create: at most 2^31 - 1 cells");
   if (m->ns != 9) return fail(ctx, MOSE_ERR_ARG, "the library instantiates the chemistry kernel for 9 species");
   if (m->tmax <= m->tmin || m->n_arrh < 0 || m->n_troe < 0 || m->n_lind < 0 || !m->wm || !m->cp || !m->h ||
       !m->ni1 || !m->ni2 || !m->eps || (m->n_arrh > 0 && (!m->kf || !m->kb)) || (m->n_troe > 0 && !m->troe) ||
@@ -1227,6 +1253,7 @@ extern "C" int mose_chem_create(mose_ctx* ctx, const mose_mechanism* m, int64_t 
   // persistent grid of 256-lane workgroups (24 teams of 10 lanes, one cell per team at a time): enough to give
   // every SIMD several waves
   o->grid = (int)std::min<int64_t>((ncell_max + kChemTeamsPerWg - 1) / kChemTeamsPerWg, (int64_t)cus * 8);
+  o->cus = cus;
   o->cap = ncell_max;
   const int nz = ns + 1;
   size_t sbytes = (size_t)nz * ncell_max * sizeof(double) + (size_t)ncell_max * (sizeof(double) + 2 * sizeof(int)) + 1024;
@@ -1279,43 +1306,48 @@ extern "C" int mose_chem_download(mose_ctx* ctx, int id, double* state, double* 
   return MOSE_OK;
 }
 
-extern "C" int mose_chem_advance(mose_ctx* ctx, int id, double dt, const mose_ode_opts* opts, mose_chem_report* rep) {
-  ChemObj* o = get_chem(id);
-  if (!ctx || !o || !opts || !(dt > 0.0)) return fail(ctx, MOSE_ERR_ARG, "mose_chem_advance: bad arguments");
+// The checks of an advance's options; on success A holds dt, the tolerances (RADAU5: transformed) and the constants.
+static int chem_opts(mose_ctx* ctx, const ChemObj* o, double dt, const mose_ode_opts* opts, ChemArgs& A,
+                     const char* who) {
+  if (!opts || !(dt > 0.0)) return fail(ctx, MOSE_ERR_ARG, std::string(who) + ": bad arguments");
   if (opts->method != MOSE_ODE_RODAS && opts->method != MOSE_ODE_RADAU5)
-    return fail(ctx, MOSE_ERR_ARG, "mose_chem_advance: method must be MOSE_ODE_RODAS or MOSE_ODE_RADAU5");
-  if (o->n < 1) return fail(ctx, MOSE_ERR_ARG, "mose_chem_advance: no cells uploaded");
+    return fail(ctx, MOSE_ERR_ARG, std::string(who) + ": method must be MOSE_ODE_RODAS or MOSE_ODE_RADAU5");
   const int nz = o->M.ns + 1;
   for (int q = 0; q < nz; ++q)
-    if (!(opts->rtol[q] > 0.0) || !(opts->atol[q] > 0.0)) return fail(ctx, MOSE_ERR_ARG, "mose_chem_advance: tolerances must be > 0");
-  HIPCHK(ctx, hipSetDevice(ctx->device));
-  hipStream_t s = ctx->stream;
-  ChemArgs A{};
-  A.state = o->state; A.hr = o->hr; A.idid = o->idid; A.nstep = o->nstep;
-  A.queue = o->ctr; A.stats = o->ctr + 1; A.ncell = o->n; A.dt = dt;
+    if (!(opts->rtol[q] > 0.0) || !(opts->atol[q] > 0.0))
+      return fail(ctx, MOSE_ERR_ARG, std::string(who) + ": tolerances must be > 0");
+  A.dt = dt;
   for (int q = 0; q < nz; ++q) { A.rtol[q] = opts->rtol[q]; A.atol[q] = opts->atol[q]; }
   if (opts->method == MOSE_ODE_RADAU5) radau5_host_setup(nz, opts->rtol, opts->atol, A.rtol, A.atol, A.rk);
-  A.stride = o->cap;
-  HIPCHK(ctx, hipMemsetAsync(o->ctr, 0, 8 * sizeof(unsigned long long), s));
-  int grid = (int)std::min<int64_t>(o->grid, (o->n + kChemTeamsPerWg - 1) / kChemTeamsPerWg);
+  return MOSE_OK;
+}
+
+// One integration of A.ncell cells with the counters at ctr ([0] queue, [1..6] stats, cleared here by a kernel):
+// the persistent launch, then the statistics into the report.  Allocates nothing.
+static int chem_run(mose_ctx* ctx, ChemObj* o, ChemArgs& A, unsigned long long* ctr, int method,
+                    mose_chem_report* rep) {
+  hipStream_t s = ctx->stream;
+  A.queue = ctr; A.stats = ctr + 1;
+  k_zero_u64<<<1, 64, 0, s>>>(ctr, kScratchWords);
+  int grid = (int)std::min<int64_t>((int64_t)o->cus * 8, (A.ncell + kChemTeamsPerWg - 1) / kChemTeamsPerWg);
   HIPCHK(ctx, hipEventRecord(o->ev[0], s));
-  if (opts->method == MOSE_ODE_RADAU5) k_radau<9><<<grid, 256, 0, s>>>(o->M, A);
+  if (method == MOSE_ODE_RADAU5) k_radau<9><<<grid, 256, 0, s>>>(o->M, A);
   else k_chem<9><<<grid, 256, 0, s>>>(o->M, A);
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipEventRecord(o->ev[1], s));
   unsigned long long st[7];
-  HIPCHK(ctx, hipMemcpyAsync(st, o->ctr, sizeof st, hipMemcpyDeviceToHost, s));
+  HIPCHK(ctx, hipMemcpyAsync(st, ctr, sizeof st, hipMemcpyDeviceToHost, s));
   HIPCHK(ctx, hipStreamSynchronize(s));
   int64_t first_failed = st[6] ? (int64_t)(~0ull - st[6]) : -1;
   int32_t first_idid = 0;
   if (first_failed >= 0) {
-    HIPCHK(ctx, hipMemcpyAsync(&first_idid, o->idid + first_failed, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipMemcpyAsync(&first_idid, A.idid + first_failed, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     HIPCHK(ctx, hipStreamSynchronize(s));
   }
   float ms = 0.f;
   HIPCHK(ctx, hipEventElapsedTime(&ms, o->ev[0], o->ev[1]));
   if (rep) {
-    rep->cells = o->n;
+    rep->cells = A.ncell;
     rep->steps_total = (int64_t)st[1];
     rep->rejected_total = (int64_t)st[2];
     rep->rhs_total = (int64_t)st[3];
@@ -1329,4 +1361,55 @@ extern "C" int mose_chem_advance(mose_ctx* ctx, int id, double dt, const mose_od
     return fail(ctx, MOSE_ERR_ODE, std::to_string(st[4]) + " cells failed (idid < 0), the first is cell " +
                                        std::to_string(first_failed) + " with idid " + std::to_string(first_idid));
   return MOSE_OK;
+}
+
+extern "C" int mose_chem_advance(mose_ctx* ctx, int id, double dt, const mose_ode_opts* opts, mose_chem_report* rep) {
+  ChemObj* o = get_chem(id);
+  if (!ctx || !o) return fail(ctx, MOSE_ERR_ARG, "mose_chem_advance: bad arguments");
+  if (o->n < 1) return fail(ctx, MOSE_ERR_ARG, "mose_chem_advance: no cells uploaded");
+  ChemArgs A{};
+  int rc = chem_opts(ctx, o, dt, opts, A, "mose_chem_advance");
+  if (rc != MOSE_OK) return rc;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  A.state = o->state; A.hr = o->hr; A.idid = o->idid; A.nstep = o->nstep;
+  A.ncell = o->n; A.stride = o->cap;
+  return chem_run(ctx, o, A, o->ctr, opts->method, rep);
+}
+
+static bool is_device_ptr(const void* p, int device) {
+  hipPointerAttribute_t a{};
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return a.type == hipMemoryTypeDevice && a.device == device;
+}
+
+extern "C" int mose_chem_advance_cells(mose_ctx* ctx, int id, int64_t ncell, int64_t stride, double* state,
+                                       double* hr, int32_t* idid, int32_t* nstep, double dt,
+                                       const mose_ode_opts* opts, mose_chem_report* rep) {
+  ChemObj* o = get_chem(id);
+  if (!ctx || !o) return fail(ctx, MOSE_ERR_ARG, "mose_chem_advance_cells: bad arguments");
+  if (ncell < 1 || ncell > MOSE_CHEM_CELLS_MAX)
+    return fail(ctx, MOSE_ERR_ARG, "mose_chem_advance_cells: ncell must be in [1, 2^31 - 1], got " +
+                                       std::to_string(ncell));
+  if (stride < ncell) return fail(ctx, MOSE_ERR_ARG, "mose_chem_advance_cells: stride < ncell");
+  if (ctx->device >= kMaxDevices) return fail(ctx, MOSE_ERR_ARG, "mose_chem_advance_cells: device index too large");
+  ChemArgs A{};
+  int rc = chem_opts(ctx, o, dt, opts, A, "mose_chem_advance_cells");
+  if (rc != MOSE_OK) return rc;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  if (!is_device_ptr(state, ctx->device) || !is_device_ptr(hr, ctx->device) || !is_device_ptr(idid, ctx->device) ||
+      !is_device_ptr(nstep, ctx->device))
+    return fail(ctx, MOSE_ERR_ARG, "mose_chem_advance_cells: state, hr, idid and nstep must be device memory of the "
+                                   "context's GPU");
+  unsigned long long* ctr = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    ctr = g_scratch[ctx->device];
+  }
+  if (!ctr) return fail(ctx, MOSE_ERR_HIP, "mose_chem_advance_cells: no scratch address for this device");
+  A.state = state; A.hr = hr; A.idid = idid; A.nstep = nstep;
+  A.ncell = ncell; A.stride = stride;
+  return chem_run(ctx, o, A, ctr, opts->method, rep);
 }

@@ -159,13 +159,16 @@ def lib() -> C.CDLL:
     L.mose_chem_download.argtypes = [vp, ip, dp, dp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     L.mose_chem_advance.argtypes = [vp, ip, C.c_double, C.POINTER(OdeOpts), C.POINTER(ChemReport)]
     L.mose_pow_batch.argtypes = [vp, C.c_int64, dp, dp, dp]
+    L.mose_chem_advance_cells.argtypes = [vp, ip, C.c_int64, C.c_int64, vp, vp, vp, vp, C.c_double,
+                                          C.POINTER(OdeOpts), C.POINTER(ChemReport)]
     for name in ("mose_ctx_create", "mose_ctx_destroy", "mose_metrics_build", "mose_records_tangent", "mose_case_create",
                  "mose_case_destroy", "mose_case_bytes", "mose_state_upload", "mose_state_download",
                  "mose_case_step", "mose_cases_step", "mose_case_launch", "mose_case_poll", "mose_case_residual",
                  "mose_case_dt",
                  "mose_case_residual_norm", "mose_case_state_hash", "mose_case_timing", "mose_device_count",
                  "mose_case_tune", "mose_bandwidth_probe", "mose_ctx_info", "mose_chem_create", "mose_chem_destroy",
-                 "mose_chem_upload", "mose_chem_download", "mose_chem_advance"):
+                 "mose_chem_upload", "mose_chem_download", "mose_chem_advance", "mose_pow_batch",
+                 "mose_chem_advance_cells"):
         getattr(L, name).restype = C.c_int
     _lib_handle = L
     return L
@@ -430,6 +433,18 @@ class DeviceChem:
         self.last_report = rep
         self.ctx.check(lib().mose_chem_advance(self.ctx.h, self.id, dt, C.byref(opts), C.byref(rep)),
                        "mose_chem_advance")
+        return rep
+
+    def advance_cells(self, ncell: int, stride: int, state: int, hr: int, idid: int, nstep: int, dt: float,
+                      opts) -> ChemReport:
+        """mose_chem_advance_cells: the same integration on cells the caller holds in device memory (the integer
+        device addresses of state [ns + 1][stride], hr, idid and nstep), with this object's mechanism; the call
+        allocates nothing."""
+        rep = ChemReport()
+        self.last_report = rep
+        self.ctx.check(lib().mose_chem_advance_cells(self.ctx.h, self.id, int(ncell), int(stride), state, hr, idid,
+                                                     nstep, dt, C.byref(opts), C.byref(rep)),
+                       "mose_chem_advance_cells")
         return rep
 
     def download(self):
