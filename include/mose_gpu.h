@@ -148,6 +148,12 @@ int mose_ctx_info(mose_ctx* ctx, int64_t* hbm_total, int64_t* hbm_free, int* cu_
 int mose_metrics_build(int nb, const int* dims, const double* nodes, int nrec, const mose_bc_record* rec, int ndir,
                        double* dl, double* m, double* vol, double* yn, double* fm1, double* fm2, double* fm3);
 
+/* The wall distance yn[NC] of every interior cell (Compute_Yn, Lib_Metrics.f90:513-744): the distance from the cell
+ * centre (the mean of its eight nodes) to the nearest centroid of a wall face (records of types 5 and 6; huge when
+ * there is none), then the first ghost layer of every connection face from its donor cell (Yn_Connection,
+ * Lib_Metrics.f90:747-763).  Other rows are left as they are.  Host only. */
+int mose_wall_distance(int nb, const int* dims, const double* nodes, int nrec, const mose_bc_record* rec, double* yn);
+
 /* The unit tangent of every far-field record (type 4) from the nodes (Setup_BC_Device, Mod_BC.f90:228-240).
  * Host only; writes rec[e].tvec. */
 int mose_records_tangent(int nb, const int* dims, const double* nodes, int nrec, mose_bc_record* rec);
@@ -206,11 +212,14 @@ int mose_case_state_hash(mose_ctx* ctx, int case_id, uint64_t* h_lo, uint64_t* h
 
 /* Execution path and tile shape of a case.  path: MOSE_PATH_FUSED (default) runs one fused stage kernel per RK
  * stage over the tiles of every block; MOSE_PATH_PLAIN runs the per-block face kernels + cell update (the
- * per-block launch structure of the reference, kept as the comparison path).  A tile holds 128, 256 or 512 cells
+ * per-block launch structure of the reference, kept as the comparison path); MOSE_PATH_FAST is the fused path in
+ * its performance formulation: the same discretisation with fused multiply-adds, reciprocals and a shorter form of
+ * the Roe (LMRoe) flux, within rounding of the other two (relative 1e-12 on the parity cases), not bit for bit.
+ * A tile holds 128, 256 or 512 cells
  * (ti * tj * tk); ti = tj = tk = 0 autotunes the tile (each candidate timed on the case itself, the state left
  * unchanged).  timing = 1 records HIP events around
  * every stage kernel of the following step calls (mose_case_timing).  No reference counterpart (launch tuning). */
-enum { MOSE_PATH_FUSED = 0, MOSE_PATH_PLAIN = 1 };
+enum { MOSE_PATH_FUSED = 0, MOSE_PATH_PLAIN = 1, MOSE_PATH_FAST = 2 };
 typedef struct {
   int32_t path;
   int32_t ti, tj, tk;
@@ -307,6 +316,19 @@ int mose_chem_advance(mose_ctx* ctx, int chem_id, double dt, const mose_ode_opts
 int mose_chem_advance_cells(mose_ctx* ctx, int chem_id, int64_t ncell, int64_t stride, double* state, double* hr,
                             int32_t* idid, int32_t* nstep, double dt, const mose_ode_opts* opts,
                             mose_chem_report* report);
+
+/* Chemistry on a flow case (nsc = the mechanism's ns, no RANS): its steps become Strang-split steps of
+ * Explicit_Step (Mod_Explicit.f90:155-294, set_StrangSplitting Lib_Strang.f90:10-22): half a transport step
+ * (strangcoeff 1/2), Chemistry_Newstate_blk (Lib_Chemistry.f90:33-76) over the step's whole dt on the case's resident
+ * state, the other half.  The mechanism is chem_id's (mose_chem_create), the integrator and tolerances opts'.
+ * chem_id < 0 detaches: plain transport steps again.  Cells whose integration ends with idid < 0 make the step call
+ * return MOSE_ERR_ODE (mose_case_chem_report gives the counts). */
+int mose_case_chem_attach(mose_ctx* ctx, int case_id, int chem_id, const mose_ode_opts* opts);
+/* Chemistry_Newstate_blk alone, in place on the interior cells of the case's resident state, over dt (host value):
+ * the reference's per-block call (Lib_Chemistry.f90:54-74) on every block, one persistent launch. */
+int mose_case_chem_advance(mose_ctx* ctx, int case_id, double dt, mose_chem_report* report);
+/* the chemistry counts of the last mose_case_step / mose_cases_step / poll of a case with chemistry attached */
+int mose_case_chem_report(mose_ctx* ctx, int case_id, mose_chem_report* report);
 
 /* out[i] = x[i]**y[i] on the device with the chemistry kernels' pow (host arrays of n values): the bits of the host
  * libm pow that the reference's Fortran `**` calls (glibc 2.35 on x86-64 with FMA; hairerRodas.f:805,816,

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -18,7 +19,9 @@
 
 #include "mose_gpu.h"
 #include "mose_internal.h"
+#include "mose_chem.hpp"
 #include "mose_kernels.hpp"
+#include "mose_radau.hpp"
 #include "mose_stage.hpp"
 
 using namespace mose;
@@ -74,11 +77,22 @@ struct Case {
   void* stage_host = nullptr;
   int pending = 0;
   bool launched = false;          // a mose_case_launch awaits its mose_case_poll
-  // one fused RK step captured as a hipGraph (kernels only), replayed per step
+  // one fused RK step captured as a hipGraph (kernels only), replayed per step; off with mose_tuning.no_graph, or
+  // for every case of the process with MOSE_NO_GRAPH=1 in the environment (read at case creation): a profiler that
+  // intercepts graph launches then sees plain kernel launches
   bool use_graph = true;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int* histpos = nullptr;     // device: next slot of the dt history
+  // chemistry of Strang-split steps (mose_case_chem_attach): n_strang = 2, strang = 1/2 with a mechanism attached
+  int n_strang = 1;
+  double strang = 1.0;
+  int chem_id = -1;               // the mechanism's chemistry object
+  int chem_method = 0;
+  ChemArgs chem{};                // tolerances, constants, rows, outputs (state and dt set per launch)
+  long long* chem_rows = nullptr; // device: the arena row of each interior cell (one allocation with the outputs)
+  unsigned long long* chem_ctr = nullptr;   // device: [0] queue of a launch, [8..13] statistics of a step call
+  unsigned long long* stage_chem = nullptr; // pinned: the statistics of the last step call
   // timing
   bool timing = false;
   std::vector<hipEvent_t> tev;
@@ -221,6 +235,10 @@ static void case_free(Case* c) {
   c->rpart_cap = 0;
   if (c->dtiles) (void)hipFree(c->dtiles);
   c->dtiles = nullptr;
+  if (c->chem_rows) (void)hipFree(c->chem_rows);
+  c->chem_rows = nullptr;
+  if (c->stage_chem) (void)hipHostFree(c->stage_chem);
+  c->stage_chem = nullptr;
   for (auto& e : c->tev)
     if (e) (void)hipEventDestroy(e);
   c->tev.clear();
@@ -477,11 +495,13 @@ static Case* get_case(mose_ctx* ctx, int id) {
   return (c && c->valid) ? c : nullptr;
 }
 
-// The residual kernels are instantiated for (nsc, nrans) = (1, 0), (1, 1) and (4, 0): the species and RANS counts are
-// compile-time counts, as in the reference (one binary per count, src/lib/CMakeLists.txt:64-71).  Four species are
-// the count of the reference's multi-species cases (test/1D/diff1, diff2).  F<NS, NR>(...) of the case's counts:
-#define MOSE_DISPATCH(sc, F, ...) \
-  ((sc).nsc == 4 ? F<4, 0>(__VA_ARGS__) : (sc).nrans == 0 ? F<1, 0>(__VA_ARGS__) : F<1, 1>(__VA_ARGS__))
+// The residual kernels are instantiated for (nsc, nrans) = (1, 0), (1, 1), (4, 0) and (9, 0): the species and RANS
+// counts are compile-time counts, as in the reference (one binary per count, src/lib/CMakeLists.txt:64-71).  Four
+// species are the count of the reference's multi-species cases (test/1D/diff1, diff2), nine that of the hydrogen-air
+// mechanism of the reacting cases.  F<NS, NR>(...) of the case's counts:
+#define MOSE_DISPATCH(sc, F, ...)                                                                     \
+  ((sc).nsc == 9 ? F<9, 0>(__VA_ARGS__) : (sc).nsc == 4 ? F<4, 0>(__VA_ARGS__)                          \
+                                        : (sc).nrans == 0 ? F<1, 0>(__VA_ARGS__) : F<1, 1>(__VA_ARGS__))
 
 static size_t stage_lds_(int nprim, bool visc, int TI, int TJ, int TK);
 // static LDS of the stage kernel beside its dynamic part (the workgroup reductions of dt and of the residual norm)
@@ -511,9 +531,10 @@ extern "C" int mose_case_create(mose_ctx* ctx, const mose_scheme* sc, int nb, co
   if (rc != MOSE_OK) return fail(ctx, rc, why);
   if (sc->nsc != ctx->nsc) return fail(ctx, MOSE_ERR_ARG, "the scheme's nsc differs from the context tables");
   if (sc->viscous && !ctx->viscous_tables) return fail(ctx, MOSE_ERR_ARG, "viscous case needs mu and k tables");
-  if (!((sc->nsc == 1 && sc->nrans == 0) || (sc->nsc == 1 && sc->nrans == 1) || (sc->nsc == 4 && sc->nrans == 0)))
-    return fail(ctx, MOSE_ERR_ARG, "the library instantiates the residual kernels for nsc = 1 (nrans 0 or 1) and nsc = 4 "
-                                   "(nrans 0)");
+  if (!((sc->nsc == 1 && sc->nrans == 0) || (sc->nsc == 1 && sc->nrans == 1) || (sc->nsc == 4 && sc->nrans == 0) ||
+        (sc->nsc == 9 && sc->nrans == 0)))
+    return fail(ctx, MOSE_ERR_ARG, "the library instantiates the residual kernels for nsc = 1 (nrans 0 or 1), nsc = 4 "
+                                   "and nsc = 9 (nrans 0)");
   HIPCHK(ctx, hipSetDevice(ctx->device));
   rc = alloc_case(ctx, *c);
   if (rc != MOSE_OK) { case_free(c.get()); return rc; }
@@ -541,6 +562,10 @@ extern "C" int mose_case_create(mose_ctx* ctx, const mose_scheme* sc, int nb, co
   rc = ensure_rpart(ctx, *c, plain_parts(*c));
   if (rc != MOSE_OK) { case_free(c.get()); return rc; }
   c->valid = true;
+  {
+    const char* e = std::getenv("MOSE_NO_GRAPH");
+    if (e && e[0] == '1') c->use_graph = false;
+  }
   choose_default_tile(ctx, *c);
   ctx->cases.push_back(std::move(c));
   *case_id = (int)ctx->cases.size() - 1;
@@ -592,7 +617,7 @@ static CaseDev make_dev(mose_ctx* ctx, Case& c, const double* Pin, double* Pout)
   D.pimp = c.sc.point_implicit;
   D.vc.sc = c.sc.sc; D.vc.sct = c.sc.sct; D.vc.prt = c.sc.prt; D.vc.prl = c.sc.prandtl;
   D.tab = ctx->dtab;
-  D.dt = c.dt; D.dtmin = c.dtmin; D.bad = c.bad; D.first_bad = c.first_bad;
+  D.dt = c.dt; D.dtmin = c.dtmin; D.bad = c.bad; D.first_bad = c.first_bad; D.strang = c.strang;
   D.cfl = c.sc.cfl; D.vnn = c.sc.vnn; D.ndir = c.sc.ndir;
   return D;
 }
@@ -685,37 +710,12 @@ static int set_tiles(mose_ctx* ctx, Case& c, int TI, int TJ, int TK) {
   return ensure_rpart(ctx, c, c.ntiles);
 }
 
-template <int NS, int NR, int BS, int SH>
-static void launch_stage_sh(Case& c, const CaseDev& D, hipStream_t s, const StageArgs& A, unsigned grid, size_t lds) {
-  const int rs = c.sc.riemann;
-  if (c.sc.viscous) {
-    if (rs == MOSE_RS_LMROE) k_stage<NS, NR, true, MOSE_RS_LMROE, BS, SH><<<grid, BS, lds, s>>>(D, A);
-    else if (rs == MOSE_RS_AUSMP) k_stage<NS, NR, true, MOSE_RS_AUSMP, BS, SH><<<grid, BS, lds, s>>>(D, A);
-    else if (rs == MOSE_RS_HLLE) k_stage<NS, NR, true, MOSE_RS_HLLE, BS, SH><<<grid, BS, lds, s>>>(D, A);
-    else k_stage<NS, NR, true, MOSE_RS_HLLC, BS, SH><<<grid, BS, lds, s>>>(D, A);
-  } else {
-    if (rs == MOSE_RS_LMROE) k_stage<NS, NR, false, MOSE_RS_LMROE, BS, SH><<<grid, BS, lds, s>>>(D, A);
-    else if (rs == MOSE_RS_AUSMP) k_stage<NS, NR, false, MOSE_RS_AUSMP, BS, SH><<<grid, BS, lds, s>>>(D, A);
-    else if (rs == MOSE_RS_HLLE) k_stage<NS, NR, false, MOSE_RS_HLLE, BS, SH><<<grid, BS, lds, s>>>(D, A);
-    else k_stage<NS, NR, false, MOSE_RS_HLLC, BS, SH><<<grid, BS, lds, s>>>(D, A);
-  }
-}
-
 // Every tile of 256 or 512 cells runs as one workgroup of 512 threads: two per cell of a 256-cell tile, one per cell
 // of a 512-cell tile.  The kernel takes 127 VGPRs, so two 256-cell tiles (76 KB of LDS each at nprim 5, viscous) run
 // on a CU at 4 waves per SIMD; with one thread per cell they ran at 2, and the stage took 16 % longer (7.22 against
 // 5.96 ms on the 256^3 Taylor-Green case).  A 128-cell tile runs as 256 threads (two per cell): with RANS (nprim 6,
 // 149 VGPRs) three of its 48 KB workgroups share a CU, where a 256-cell tile's 92 KB leaves one.  The two preferred
-// tile shapes run kernels with the shape fixed at compile time; the others read it at run time.
-template <int NS, int NR>
-static void launch_stage_bs(Case& c, const CaseDev& D, hipStream_t s, const StageArgs& A, unsigned grid, size_t lds) {
-  constexpr int BS = 512;
-  if (c.TI * c.TJ * c.TK == 128) launch_stage_sh<NS, NR, 256, 0>(c, D, s, A, grid, lds);
-  else if (c.TI == 8 && c.TJ == 8 && c.TK == 4) launch_stage_sh<NS, NR, BS, 1>(c, D, s, A, grid, lds);
-  else if (c.TI == 8 && c.TJ == 8 && c.TK == 8) launch_stage_sh<NS, NR, BS, 2>(c, D, s, A, grid, lds);
-  else launch_stage_sh<NS, NR, BS, 0>(c, D, s, A, grid, lds);
-}
-
+// tile shapes run kernels with the shape fixed at compile time; the others read it at run time (launch_stage_tiles).
 template <int NS, int NR>
 static void launch_stage(Case& c, const CaseDev& D, hipStream_t s, double rkc, bool last) {
   StageArgs A;
@@ -727,7 +727,8 @@ static void launch_stage(Case& c, const CaseDev& D, hipStream_t s, double rkc, b
   A.last = last ? 1 : 0; A.rkc = rkc; A.dtnext = c.dtnext; A.rpart = c.rpart;
   size_t lds = stage_lds(c, c.TI, c.TJ, c.TK);
   unsigned grid = (unsigned)(((c.ntiles + 7) / 8) * 8);
-  launch_stage_bs<NS, NR>(c, D, s, A, grid, lds);
+  if (c.path == MOSE_PATH_FAST) launch_stage_fast<NS, NR>(D, A, c.sc.riemann, c.sc.viscous != 0, grid, lds, s);
+  else launch_stage_tiles<NS, NR, false>(D, A, c.sc.riemann, c.sc.viscous != 0, grid, lds, s);
 }
 
 // records the step's dt in the history at the device-side position (the same kernel serves eager and graph steps)
@@ -740,12 +741,14 @@ __global__ void k_step_begin(const double* dt, double* hist, int* pos, int cap) 
 }
 
 
+static int case_chem_launch(mose_ctx* ctx, Case& c, hipStream_t s);
+
 template <int NS, int NR>
 static int run_steps(mose_ctx* ctx, Case& c, int nsteps) {
   hipStream_t s = c.stream;
   static const double rk[3][3] = {{1.0, 0.0, 0.0}, {1.0, 0.5, 0.0}, {1.0, 0.25, 2.0 / 3.0}};   // Lib_RK.f90:6-9
   int nrk = c.sc.n_rk;
-  bool fused = (c.path == MOSE_PATH_FUSED);
+  bool fused = (c.path == MOSE_PATH_FUSED || c.path == MOSE_PATH_FAST);
   if (fused && c.ntiles == 0) {
     int rc = set_tiles(ctx, c, c.TI, c.TJ, c.TK);
     if (rc != MOSE_OK) return rc;
@@ -753,13 +756,15 @@ static int run_steps(mose_ctx* ctx, Case& c, int nsteps) {
   if (fused && stage_lds(c, c.TI, c.TJ, c.TK) + kStaticLds > (size_t)ctx->max_lds) return fail(ctx, MOSE_ERR_ARG, "tile does not fit");
   HIPCHK(ctx, hipMemsetAsync(c.bad, 0, 2 * sizeof(unsigned), s));
   HIPCHK(ctx, hipMemsetAsync(c.first_bad, 0xff, sizeof(unsigned long long), s));
-  size_t nev = c.timing ? (size_t)2 * nsteps * nrk : 0;
+  const int nstr = c.n_strang;
+  size_t nev = c.timing ? (size_t)2 * nsteps * nrk * nstr : 0;
   while (c.tev.size() < nev) {
     hipEvent_t e;
     HIPCHK(ctx, hipEventCreate(&e));
     c.tev.push_back(e);
   }
   HIPCHK(ctx, hipMemsetAsync(c.histpos, 0, sizeof(int), s));
+  if (nstr == 2) k_zero_u64<<<1, 64, 0, s>>>(c.chem_ctr + 8, 8);   // the chemistry statistics of this call
   size_t iev = 0;
   for (int n = 0; n < nsteps; ++n) {
     CaseDev D0 = make_dev(ctx, c, c.P0, c.PA);
@@ -774,35 +779,44 @@ static int run_steps(mose_ctx* ctx, Case& c, int nsteps) {
     }
     if (graph) HIPCHK(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     k_step_begin<<<1, 64, 0, s>>>(c.dt, c.dthist, c.histpos, c.hist_cap);
-    for (int irk = 1; irk <= nrk; ++irk) {
-      double* in = irk == 1 ? c.P0 : (irk == 2 ? c.PA : c.PB);
-      double* out = irk == nrk ? c.P0 : (irk == 1 ? c.PA : c.PB);
-      CaseDev D = make_dev(ctx, c, in, out);
-      double rkc = rk[nrk - 1][irk - 1];
-      if (fused) {
-        int nrec = (int)c.rec.size();
-        if (nrec > 0) {
-          k_ghost<NS, NR><<<nblocks(nrec, 256), 256, 0, s>>>(D);
-          k_bcrec<NS, NR><<<nblocks(nrec, 128), 128, 0, s>>>(D);
+    // the strang loop of Explicit_Step (Mod_Explicit.f90:155-294): one substage without chemistry; with it, half a
+    // transport step (strangcoeff 1/2 in the update), Chemistry_Newstate_blk over the whole dt on the resident state,
+    // the other half.  Each substage starts from the state it finds (Copy_State: P0 is the base of its stages).
+    for (int istr = 1; istr <= nstr; ++istr) {
+      for (int irk = 1; irk <= nrk; ++irk) {
+        double* in = irk == 1 ? c.P0 : (irk == 2 ? c.PA : c.PB);
+        double* out = irk == nrk ? c.P0 : (irk == 1 ? c.PA : c.PB);
+        CaseDev D = make_dev(ctx, c, in, out);
+        double rkc = rk[nrk - 1][irk - 1];
+        const bool last = irk == nrk && istr == nstr;   // the step's last stage: next dt and the norm partials
+        if (fused) {
+          int nrec = (int)c.rec.size();
+          if (nrec > 0) {
+            k_ghost<NS, NR><<<nblocks(nrec, 256), 256, 0, s>>>(D);
+            k_bcrec<NS, NR><<<nblocks(nrec, 128), 128, 0, s>>>(D);
+          }
+          if (last) k_dt_init<<<1, 64, 0, s>>>(c.dtnext);
+          if (c.timing) HIPCHK(ctx, hipEventRecord(c.tev[iev++], s));
+          launch_stage<NS, NR>(c, D, s, rkc, last);
+          if (last) c.nparts_last = c.ntiles;
+          if (c.timing) HIPCHK(ctx, hipEventRecord(c.tev[iev++], s));
+          if (last) k_dt_final<<<1, 64, 0, s>>>(c.dtnext, c.dt, nullptr, 0);
+        } else {
+          launch_flux<NS, NR>(c, D, s);
+          if (c.timing) HIPCHK(ctx, hipEventRecord(c.tev[iev++], s));
+          int64_t part0 = 0;
+          for (int b = 0; b < c.nb; ++b) {
+            int64_t ncell = (int64_t)c.blk[b].n1 * c.blk[b].n2 * c.blk[b].n3;
+            k_update<NS, NR><<<nblocks(ncell, 256), 256, 0, s>>>(D, b, rkc, last ? c.rpart : nullptr, part0);
+            part0 += nblocks(ncell, 256);
+          }
+          if (last) c.nparts_last = part0;
+          if (c.timing) HIPCHK(ctx, hipEventRecord(c.tev[iev++], s));
         }
-        bool last = irk == nrk;
-        if (last) k_dt_init<<<1, 64, 0, s>>>(c.dtnext);
-        if (c.timing) HIPCHK(ctx, hipEventRecord(c.tev[iev++], s));
-        launch_stage<NS, NR>(c, D, s, rkc, last);
-        if (last) c.nparts_last = c.ntiles;
-        if (c.timing) HIPCHK(ctx, hipEventRecord(c.tev[iev++], s));
-        if (last) k_dt_final<<<1, 64, 0, s>>>(c.dtnext, c.dt, nullptr, 0);
-      } else {
-        launch_flux<NS, NR>(c, D, s);
-        if (c.timing) HIPCHK(ctx, hipEventRecord(c.tev[iev++], s));
-        int64_t part0 = 0;
-        for (int b = 0; b < c.nb; ++b) {
-          int64_t ncell = (int64_t)c.blk[b].n1 * c.blk[b].n2 * c.blk[b].n3;
-          k_update<NS, NR><<<nblocks(ncell, 256), 256, 0, s>>>(D, b, rkc, irk == nrk ? c.rpart : nullptr, part0);
-          part0 += nblocks(ncell, 256);
-        }
-        if (irk == nrk) c.nparts_last = part0;
-        if (c.timing) HIPCHK(ctx, hipEventRecord(c.tev[iev++], s));
+      }
+      if (istr == 1 && nstr == 2) {
+        int rc = case_chem_launch(ctx, c, s);
+        if (rc != MOSE_OK) return rc;
       }
     }
     if (graph) {
@@ -877,15 +891,19 @@ static int autotune(mose_ctx* ctx, Case& c, mose_tuning* chosen) {
 extern "C" int mose_case_tune(mose_ctx* ctx, int id, const mose_tuning* t, mose_tuning* chosen) {
   Case* c = get_case(ctx, id);
   if (!c || !t) return MOSE_ERR_ARG;
-  if (t->path != MOSE_PATH_FUSED && t->path != MOSE_PATH_PLAIN) return fail(ctx, MOSE_ERR_ARG, "unknown path");
+  if (t->path != MOSE_PATH_FUSED && t->path != MOSE_PATH_PLAIN && t->path != MOSE_PATH_FAST)
+    return fail(ctx, MOSE_ERR_ARG, "unknown path");
   HIPCHK(ctx, hipSetDevice(ctx->device));
   c->path = t->path;
   c->timing = t->timing != 0;
-  c->use_graph = t->no_graph == 0;
+  {
+    const char* e = std::getenv("MOSE_NO_GRAPH");
+    c->use_graph = t->no_graph == 0 && !(e && e[0] == '1');
+  }
   c->dt_valid = false;
   drop_graph(*c);
   if (chosen) *chosen = *t;
-  if (c->path == MOSE_PATH_FUSED) {
+  if (c->path == MOSE_PATH_FUSED || c->path == MOSE_PATH_FAST) {
     if (t->ti == 0 && t->tj == 0 && t->tk == 0)
       return MOSE_DISPATCH(c->sc, autotune, ctx, *c, chosen);
     return set_tiles(ctx, *c, t->ti, t->tj, t->tk);
@@ -904,6 +922,9 @@ static int case_enqueue(mose_ctx* ctx, Case* c, int nsteps) {
     HIPCHK(ctx, hipMemcpyAsync(c->stage_hist, c->dthist, nh * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(ctx, hipMemcpyAsync(c->stage_bad, c->bad, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(ctx, hipMemcpyAsync(c->stage_fb, c->first_bad, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+  if (c->n_strang == 2)
+    HIPCHK(ctx, hipMemcpyAsync(c->stage_chem, c->chem_ctr + 8, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                               c->stream));
   c->pending = nsteps;
   return MOSE_OK;
 }
@@ -919,6 +940,8 @@ static int case_collect(mose_ctx* ctx, Case* c, mose_report* rep) {
   unsigned bad[2] = {c->stage_bad[0], c->stage_bad[1]};
   unsigned long long fb = *c->stage_fb;
   int status = MOSE_OK;
+  const bool chem_failed = c->n_strang == 2 && nsteps > 0 && c->stage_chem[3] > 0;
+  if (chem_failed) status = MOSE_ERR_ODE;
   if (bad[1] > 0) status = MOSE_ERR_THERMO;
   if (bad[0] > 0) status = MOSE_ERR_STATE;
   if (rep) {
@@ -932,7 +955,11 @@ static int case_collect(mose_ctx* ctx, Case* c, mose_report* rep) {
     rep->first_bad_i = fb == ~0ull ? -1 : (int)(fb & 0xffff);
     rep->status = status;
   }
-  if (status != MOSE_OK) {
+  if (status == MOSE_ERR_ODE) {
+    const unsigned long long* st = c->stage_chem;
+    ctx->err = std::to_string(st[3]) + " chemistry integrations failed (idid < 0) in the last " +
+               std::to_string(nsteps) + " steps, the first at interior cell " + std::to_string(~0ull - st[5]);
+  } else if (status != MOSE_OK) {
     char msg[256];
     snprintf(msg, sizeof msg, "%s in %u cells, first at block %d (i,j,k) = (%d,%d,%d)",
              status == MOSE_ERR_STATE ? "p < 0 or non-finite state" : "T outside (1, 9990) K", bad[0] + bad[1],
@@ -1130,9 +1157,6 @@ extern "C" int mose_bandwidth_probe(mose_ctx* ctx, int64_t bytes, int reps, doub
 // ----------------------------------------------------------------------------------------------------------------
 // stiff chemistry: batches of independent cells (Chemistry_Newstate_blk with the host RODAS of the general path)
 // ----------------------------------------------------------------------------------------------------------------
-#include "mose_chem.hpp"
-#include "mose_radau.hpp"
-
 __global__ void k_pow(const double* __restrict__ x, const double* __restrict__ y, double* __restrict__ out, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     out[i] = mose_pow(x[i], y[i]);
@@ -1372,7 +1396,7 @@ extern "C" int mose_chem_advance(mose_ctx* ctx, int id, double dt, const mose_od
   if (rc != MOSE_OK) return rc;
   HIPCHK(ctx, hipSetDevice(ctx->device));
   A.state = o->state; A.hr = o->hr; A.idid = o->idid; A.nstep = o->nstep;
-  A.ncell = o->n; A.stride = o->cap;
+  A.ncell = o->n; A.stride = o->cap; A.prow = o->M.ns;
   return chem_run(ctx, o, A, o->ctr, opts->method, rep);
 }
 
@@ -1410,6 +1434,136 @@ extern "C" int mose_chem_advance_cells(mose_ctx* ctx, int id, int64_t ncell, int
   }
   if (!ctr) return fail(ctx, MOSE_ERR_HIP, "mose_chem_advance_cells: no scratch address for this device");
   A.state = state; A.hr = hr; A.idid = idid; A.nstep = nstep;
-  A.ncell = ncell; A.stride = stride;
+  A.ncell = ncell; A.stride = stride; A.prow = o->M.ns;
   return chem_run(ctx, o, A, ctr, opts->method, rep);
+}
+
+// ----------------------------------------------------------------------------------------------------------------
+// chemistry on a case's resident state: Chemistry_Newstate_blk (src/lib/physics/Lib_Chemistry.f90:33-76) on the
+// interior cells of every block, in place on the var-last arena, and the Strang-split step around it
+// ----------------------------------------------------------------------------------------------------------------
+static int case_chem_launch(mose_ctx* ctx, Case& c, hipStream_t s) {
+  ChemObj* o = get_chem(c.chem_id);
+  if (!o) return fail(ctx, MOSE_ERR_ARG, "the case's chemistry object was destroyed");
+  ChemArgs A = c.chem;
+  A.state = c.P0; A.stride = c.NC; A.dtp = c.dt;
+  A.queue = c.chem_ctr; A.stats = c.chem_ctr + 8;
+  k_zero_u64<<<1, 64, 0, s>>>(c.chem_ctr, 1);
+  const int grid = (int)std::min<int64_t>((int64_t)o->cus * 8, (A.ncell + kChemTeamsPerWg - 1) / kChemTeamsPerWg);
+  if (c.chem_method == MOSE_ODE_RADAU5) k_radau<9><<<grid, 256, 0, s>>>(o->M, A);
+  else k_chem<9><<<grid, 256, 0, s>>>(o->M, A);
+  HIPCHK(ctx, hipGetLastError());
+  return MOSE_OK;
+}
+
+extern "C" int mose_case_chem_attach(mose_ctx* ctx, int id, int chem_id, const mose_ode_opts* opts) {
+  Case* c = get_case(ctx, id);
+  if (!c) return fail(ctx, MOSE_ERR_ARG, "mose_case_chem_attach: bad case id");
+  if (c->launched || c->pending) return fail(ctx, MOSE_ERR_ARG, "mose_case_chem_attach: a launch of the case awaits its poll");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  HIPCHK(ctx, hipStreamSynchronize(c->stream));
+  if (chem_id < 0) {
+    drop_graph(*c);
+    c->n_strang = 1; c->strang = 1.0; c->chem_id = -1;
+    return MOSE_OK;
+  }
+  ChemObj* o = get_chem(chem_id);
+  if (!o) return fail(ctx, MOSE_ERR_ARG, "mose_case_chem_attach: no such chemistry object");
+  if (c->sc.nsc != o->M.ns || c->sc.nrans != 0)
+    return fail(ctx, MOSE_ERR_ARG, "mose_case_chem_attach: the case's species must be the mechanism's (and no RANS)");
+  if (c->ninterior > MOSE_CHEM_CELLS_MAX) return fail(ctx, MOSE_ERR_ARG, "mose_case_chem_attach: too many cells");
+  ChemArgs A{};
+  int rc = chem_opts(ctx, o, 1.0, opts, A, "mose_case_chem_attach");
+  if (rc != MOSE_OK) return rc;
+  const int64_t n = c->ninterior;
+  if (!c->chem_rows) {
+    // one allocation: rows [n] (int64), hr [n], idid [n], nstep [n], counters [16]
+    const size_t bytes = (size_t)n * (8 + 8 + 4 + 4) + 256 + 16 * sizeof(unsigned long long);
+    HIPCHK(ctx, hipMalloc(&c->chem_rows, bytes));
+    HIPCHK(ctx, hipHostMalloc(&c->stage_chem, 8 * sizeof(unsigned long long), hipHostMallocDefault));
+    std::vector<long long> rows;
+    rows.reserve((size_t)n);
+    for (const mose_blk& B : c->blk)
+      for (int k = 1; k <= B.n3; ++k)
+        for (int j = 1; j <= B.n2; ++j)
+          for (int i = 1; i <= B.n1; ++i)
+            rows.push_back(B.boff + (i + MOSE_GC - 1) + (long long)(j + MOSE_GC - 1) * B.pi +
+                           (long long)(k + MOSE_GC - 1) * B.pi * B.pj);
+    HIPCHK(ctx, hipMemcpy(c->chem_rows, rows.data(), rows.size() * sizeof(long long), hipMemcpyHostToDevice));
+  }
+  double* hr = (double*)(c->chem_rows + n);
+  int32_t* idid = (int32_t*)(hr + n);
+  int32_t* nstep = idid + n;
+  c->chem_ctr = (unsigned long long*)(((uintptr_t)(nstep + n) + 255) & ~(uintptr_t)255);
+  A.rows = c->chem_rows; A.prow = c->sc.nsc + 3; A.hr = hr; A.idid = idid; A.nstep = nstep; A.ncell = n;
+  c->chem = A;
+  c->chem_method = opts->method;
+  c->chem_id = chem_id;
+  c->n_strang = 2;
+  c->strang = 0.5;
+  drop_graph(*c);
+  return MOSE_OK;
+}
+
+static void chem_report_fill(const Case& c, const unsigned long long* st, int32_t first_idid, mose_chem_report* rep) {
+  rep->cells = c.ninterior;
+  rep->steps_total = (int64_t)st[0];
+  rep->rejected_total = (int64_t)st[1];
+  rep->rhs_total = (int64_t)st[2];
+  rep->failed = (int64_t)st[3];
+  rep->steps_max = (int32_t)st[4];
+  rep->first_failed_cell = st[5] ? (int64_t)(~0ull - st[5]) : -1;
+  rep->first_failed_idid = first_idid;
+  rep->ms = 0.0;
+}
+
+extern "C" int mose_case_chem_report(mose_ctx* ctx, int id, mose_chem_report* rep) {
+  Case* c = get_case(ctx, id);
+  if (!c || !rep) return fail(ctx, MOSE_ERR_ARG, "mose_case_chem_report: bad arguments");
+  if (c->n_strang != 2) return fail(ctx, MOSE_ERR_ARG, "mose_case_chem_report: no chemistry attached");
+  if (c->launched || c->pending) return fail(ctx, MOSE_ERR_ARG, "mose_case_chem_report: a launch awaits its poll");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  int32_t first_idid = 0;
+  const unsigned long long* st = c->stage_chem;
+  if (st[5]) {
+    HIPCHK(ctx, hipMemcpy(&first_idid, c->chem.idid + (~0ull - st[5]), sizeof(int32_t), hipMemcpyDeviceToHost));
+  }
+  chem_report_fill(*c, st, first_idid, rep);
+  return MOSE_OK;
+}
+
+extern "C" int mose_case_chem_advance(mose_ctx* ctx, int id, double dt, mose_chem_report* rep) {
+  Case* c = get_case(ctx, id);
+  if (!c || !(dt > 0.0)) return fail(ctx, MOSE_ERR_ARG, "mose_case_chem_advance: bad arguments");
+  if (c->n_strang != 2) return fail(ctx, MOSE_ERR_ARG, "mose_case_chem_advance: no chemistry attached");
+  if (c->launched || c->pending) return fail(ctx, MOSE_ERR_ARG, "mose_case_chem_advance: a launch awaits its poll");
+  ChemObj* o = get_chem(c->chem_id);
+  if (!o) return fail(ctx, MOSE_ERR_ARG, "mose_case_chem_advance: the case's chemistry object was destroyed");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = c->stream;
+  ChemArgs A = c->chem;
+  A.state = c->P0; A.stride = c->NC; A.dtp = nullptr; A.dt = dt;
+  A.queue = c->chem_ctr; A.stats = c->chem_ctr + 8;
+  k_zero_u64<<<1, 64, 0, s>>>(c->chem_ctr, 16);
+  const int grid = (int)std::min<int64_t>((int64_t)o->cus * 8, (A.ncell + kChemTeamsPerWg - 1) / kChemTeamsPerWg);
+  HIPCHK(ctx, hipEventRecord(c->ev[1], s));
+  if (c->chem_method == MOSE_ODE_RADAU5) k_radau<9><<<grid, 256, 0, s>>>(o->M, A);
+  else k_chem<9><<<grid, 256, 0, s>>>(o->M, A);
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipEventRecord(c->ev[2], s));
+  HIPCHK(ctx, hipMemcpyAsync(c->stage_chem, c->chem_ctr + 8, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+  HIPCHK(ctx, hipStreamSynchronize(s));
+  const unsigned long long* st = c->stage_chem;
+  int32_t first_idid = 0;
+  if (st[5]) HIPCHK(ctx, hipMemcpy(&first_idid, A.idid + (~0ull - st[5]), sizeof(int32_t), hipMemcpyDeviceToHost));
+  float ms = 0.f;
+  HIPCHK(ctx, hipEventElapsedTime(&ms, c->ev[1], c->ev[2]));
+  if (rep) {
+    chem_report_fill(*c, st, first_idid, rep);
+    rep->ms = ms;
+  }
+  if (st[3] > 0)
+    return fail(ctx, MOSE_ERR_ODE, std::to_string(st[3]) + " cells failed (idid < 0), the first is interior cell " +
+                                       std::to_string(~0ull - st[5]) + " with idid " + std::to_string(first_idid));
+  return MOSE_OK;
 }

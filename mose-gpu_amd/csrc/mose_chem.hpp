@@ -27,7 +27,10 @@ struct ChemDev {
 };
 
 struct ChemArgs {
-  double* state;            // [NS+1][ncell]: species densities, p
+  double* state;            // rows of `stride` doubles: species densities in rows 0..NS-1, p in row prow
+  const long long* rows;    // the column of cell c in state (a flow case's interior rows), or null: column c
+  int prow;                 // the row of p: NS for a batch, NS + 3 on a flow case's arena
+  const double* dtp;        // the case's dt on the device (a Strang step reacts over the step's dt), or null: dt
   double* hr;               // [ncell]
   int* idid;                // [ncell]
   int* nstep;               // [ncell]
@@ -426,8 +429,9 @@ __device__ __forceinline__ void chem_setup(const ChemDev& M, double* cw, RxnLds&
 template <int NS>
 __device__ __forceinline__ double cell_load(const ChemDev& M, const ChemArgs& A, int cell, int base, int r, int rc,
                                             double& rho0, double& Tb) {
-  const double q = A.state[(long long)rc * A.stride + cell];
-  const double p = A.state[(long long)NS * A.stride + cell];
+  const long long col = A.rows ? A.rows[cell] : (long long)cell;
+  const double q = A.state[(long long)(rc < NS ? rc : A.prow) * A.stride + col];
+  const double p = A.state[(long long)A.prow * A.stride + col];
   double rho = 0.0, Rm = 0.0;
 #pragma unroll
   for (int s = 0; s < NS; ++s) rho = rho + tsh(q, base + s);
@@ -450,9 +454,10 @@ __device__ __forceinline__ void cell_store(const ChemDev& M, const ChemArgs& A, 
   for (int s = 0; s < NS; ++s) rs = rs + tsh(yf, base + s);
 #pragma unroll
   for (int s = 0; s < NS; ++s) R2 = R2 + M.Ri[s] * tsh(yf, base + s) / rs;
-  if (r < NS) A.state[(long long)r * A.stride + cell] = yf;
+  const long long col = A.rows ? A.rows[cell] : (long long)cell;
+  if (r < NS) A.state[(long long)r * A.stride + col] = yf;
   if (r == NS) {
-    A.state[(long long)NS * A.stride + cell] = T * rho0 * R2;
+    A.state[(long long)A.prow * A.stride + col] = T * rho0 * R2;
     A.hr[cell] = fabs(T - Tb) / Tb;
     A.idid[cell] = idid;
     A.nstep[cell] = nst;
@@ -480,6 +485,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
   using R = RodasC;
   const double uround = 1.1e-19, safe = 0.9, fac1 = 5.0, fac2 = 1.0 / 6.0;
   const int nmax = 100000;
+  const double dtend = A.dtp ? *A.dtp : A.dt;   // XEND of every cell
 
   __shared__ double cw[NRMAX * SLOT];
   __shared__ TeamLds tl[TSh::per_wg];
@@ -513,7 +519,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     if (0.1 * fabs(h) <= fabs(x) * uround) { idid = -3; return true; }
     if (last) { h = hopt; idid = 1; return true; }
     hopt = h;
-    if ((x + h * 1.0001 - A.dt) * posneg >= 0.0) { h = A.dt - x; last = true; }
+    if ((x + h * 1.0001 - dtend) * posneg >= 0.0) { h = dtend - x; last = true; }
     zin = y;
     ev = EV_F0;
     return false;
@@ -535,9 +541,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
       y = cell_load<NS>(M, A, cell, base, r, rc, rho0, Tb);
       // ROSCOR initial preparations (x = 0, xend = dt, H = 0 -> 1e-6)
       x = 0.0;
-      double hmax = A.dt - x;
-      posneg = copysign(1.0, A.dt - x);
-      hmaxn = fmin(fabs(hmax), fabs(A.dt - x));
+      double hmax = dtend - x;
+      posneg = copysign(1.0, dtend - x);
+      hmaxn = fmin(fabs(hmax), fabs(dtend - x));
       h = 0.0;
       if (fabs(h) <= 10.0 * uround) h = 1.0e-6;
       h = fmin(fabs(h), hmaxn);

@@ -41,6 +41,7 @@ struct CaseDev {
   ViscConst vc;
   const mose_dtab* tab;
   double* dt;            // dt of the current step (device scalar)
+  double strang;         // strangcoeff of the update: 1, or 1/2 in a Strang-split step (Lib_Strang.f90:10-22)
   unsigned long long* dtmin;   // min reduction slot (positive doubles as bits)
   unsigned int* bad;     // [0] count of failing cells, [1] thermo guard count
   unsigned long long* first_bad;
@@ -503,7 +504,7 @@ __device__ __forceinline__ void block_sum_store(const double* v, double* out) {
 }
 
 // sums of the per-workgroup partials in index order (one thread per component)
-__global__ void k_rnorm_final(const double* part, int64_t nparts, int nres, double* out) {
+static __global__ void k_rnorm_final(const double* part, int64_t nparts, int nres, double* out) {
   int q = threadIdx.x;
   if (blockIdx.x != 0 || q >= nres) return;
   double t = 0.0;
@@ -619,7 +620,7 @@ __device__ __forceinline__ void update_one(const CaseDev& C, const mose_blk& B, 
     Tseed = dmin2(dmax2(Tseed, 1.0), 9990.0);
   }
   if (NR > 0) cell_sa<NS, NR>(C, B, i, j, k, c0, rho, cell_mil<NS>(T, p, rho, Tseed), res);
-  update_cell<NS, NR>(T, p, po, res, C.vol[c0], *C.dt, rkc, Tseed, out);
+  update_cell<NS, NR>(T, p, po, res, C.vol[c0], *C.dt, C.strang, rkc, Tseed, out);
   bool ok = out[V::np] >= 0.0;
 #pragma unroll
   for (int s = 0; s < V::nprim; ++s) ok = ok && isfinite(out[s]);
@@ -731,10 +732,10 @@ __global__ void __launch_bounds__(256) k_dt(CaseDev C, int b) {
 }
 
 // dt <- min(1e5, reduction) (Explicit_Step sets dtglobal = 1d5 before Compute_dt, Mod_Explicit.f90:124)
-__global__ void k_dt_init(unsigned long long* slot) {
+static __global__ void k_dt_init(unsigned long long* slot) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *slot = (unsigned long long)__double_as_longlong(1e5);
 }
-__global__ void k_dt_final(const unsigned long long* slot, double* dt, double* hist, int n) {
+static __global__ void k_dt_final(const unsigned long long* slot, double* dt, double* hist, int n) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     double v = __longlong_as_double((long long)*slot);
     *dt = v;

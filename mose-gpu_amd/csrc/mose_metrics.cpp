@@ -8,6 +8,8 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <limits>
+#include <algorithm>
 #include <vector>
 
 #include "mose_gpu.h"
@@ -391,6 +393,127 @@ extern "C" int mose_metrics_build(int nb, const int* dims, const double* nodes, 
     }
   }
   return ok ? MOSE_OK : MOSE_ERR_MESH;
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// Wall distance: Compute_Yn (Lib_Metrics.f90:513-744) and Yn_Connection (:747-763)
+// ------------------------------------------------------------------------------------------------------------
+namespace {
+
+// A 3-d tree over the wall-face centroids: median splits along x, y, z in turn; the nearest-point query visits the
+// far side of a split only while the squared distance to the splitting plane is below the best squared distance,
+// so it returns the minimum of dx^2 + dy^2 + dz^2 over all points, the value a scan of every point gives.
+struct WallTree {
+  std::vector<double> x, y, z;
+  void build(int lo, int hi, int axis) {   // [lo, hi)
+    if (hi - lo <= 1) return;
+    const int mid = (lo + hi) / 2;
+    std::vector<int> idx((size_t)(hi - lo));
+    for (int q = lo; q < hi; ++q) idx[(size_t)(q - lo)] = q;
+    const std::vector<double>& key = axis == 0 ? x : (axis == 1 ? y : z);
+    std::nth_element(idx.begin(), idx.begin() + (mid - lo), idx.end(),
+                     [&](int a, int b) { return key[(size_t)a] < key[(size_t)b]; });
+    std::vector<double> tx, ty, tz;
+    tx.reserve(idx.size()); ty.reserve(idx.size()); tz.reserve(idx.size());
+    for (int q : idx) { tx.push_back(x[(size_t)q]); ty.push_back(y[(size_t)q]); tz.push_back(z[(size_t)q]); }
+    for (int q = lo; q < hi; ++q) {
+      x[(size_t)q] = tx[(size_t)(q - lo)]; y[(size_t)q] = ty[(size_t)(q - lo)]; z[(size_t)q] = tz[(size_t)(q - lo)];
+    }
+    build(lo, mid, (axis + 1) % 3);
+    build(mid + 1, hi, (axis + 1) % 3);
+  }
+  void nearest(int lo, int hi, int axis, double qx, double qy, double qz, double& best2) const {
+    if (lo >= hi) return;
+    const int mid = (lo + hi) / 2;
+    const double dx = x[(size_t)mid] - qx, dy = y[(size_t)mid] - qy, dz = z[(size_t)mid] - qz;
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    if (d2 < best2) best2 = d2;
+    const double diff = axis == 0 ? qx - x[(size_t)mid] : (axis == 1 ? qy - y[(size_t)mid] : qz - z[(size_t)mid]);
+    const int nxt = (axis + 1) % 3;
+    if (diff < 0.0) {
+      nearest(lo, mid, nxt, qx, qy, qz, best2);
+      if (diff * diff < best2) nearest(mid + 1, hi, nxt, qx, qy, qz, best2);
+    } else {
+      nearest(mid + 1, hi, nxt, qx, qy, qz, best2);
+      if (diff * diff < best2) nearest(lo, mid, nxt, qx, qy, qz, best2);
+    }
+  }
+};
+
+}  // namespace
+
+extern "C" int mose_wall_distance(int nb, const int* dims, const double* nodes, int nrec, const mose_bc_record* rec,
+                                  double* yn) {
+  if (nb <= 0 || !dims || !nodes || !yn || (nrec > 0 && !rec)) return MOSE_ERR_ARG;
+  std::vector<BlockGeo> B(nb);
+  int64_t NC = 0, NN = 0;
+  for (int b = 0; b < nb; ++b) {
+    int n1 = dims[3 * b], n2 = dims[3 * b + 1], n3 = dims[3 * b + 2];
+    if (n1 < 1 || n2 < 1 || n3 < 1) return MOSE_ERR_ARG;
+    B[b] = BlockGeo{n1, n2, n3, NC, 0, 0, 0, n1 + 2 * GC, n2 + 2 * GC, nodes + NN};
+    NC += (int64_t)(n1 + 2 * GC) * (n2 + 2 * GC) * (n3 + 2 * GC);
+    NN += 3 * (int64_t)(n1 + 1) * (n2 + 1) * (n3 + 1);
+  }
+  // the centroids of the wall faces (Is_Wall: the no-slip types 5 and 6 of this path), 0.25 (n1 + n2 + n3 + n4)
+  WallTree t;
+  for (int e = 0; e < nrec; ++e) {
+    const mose_bc_record& r = rec[e];
+    if (r.typ != MOSE_BC_WALL_ADIABATIC && r.typ != MOSE_BC_WALL_ISOTHERMAL) continue;
+    if (r.b < 0 || r.b >= nb || r.f < 1 || r.f > 6) return MOSE_ERR_ARG;
+    const BlockGeo& G = B[r.b];
+    const int Fm = r.f, m2 = Fm % 2;
+    int c[4][3];
+    if (Fm <= 2) {
+      const int i = r.i - m2;
+      int q[4][3] = {{i, r.j - 1, r.k - 1}, {i, r.j, r.k - 1}, {i, r.j - 1, r.k}, {i, r.j, r.k}};
+      std::memcpy(c, q, sizeof c);
+    } else if (Fm <= 4) {
+      const int j = r.j - m2;
+      int q[4][3] = {{r.i - 1, j, r.k - 1}, {r.i, j, r.k - 1}, {r.i - 1, j, r.k}, {r.i, j, r.k}};
+      std::memcpy(c, q, sizeof c);
+    } else {
+      const int k = r.k - m2;
+      int q[4][3] = {{r.i - 1, r.j - 1, k}, {r.i, r.j - 1, k}, {r.i - 1, r.j, k}, {r.i, r.j, k}};
+      std::memcpy(c, q, sizeof c);
+    }
+    V3 n[4];
+    for (int q = 0; q < 4; ++q) n[q] = G.nd(c[q][0], c[q][1], c[q][2]);
+    t.x.push_back(0.25 * (n[0].c[0] + n[1].c[0] + n[2].c[0] + n[3].c[0]));
+    t.y.push_back(0.25 * (n[0].c[1] + n[1].c[1] + n[2].c[1] + n[3].c[1]));
+    t.z.push_back(0.25 * (n[0].c[2] + n[1].c[2] + n[2].c[2] + n[3].c[2]));
+  }
+  const int nw = (int)t.x.size();
+  t.build(0, nw, 0);
+  // every interior cell: the nearest centroid to the cell centre 1/8 (sum of the eight nodes, in Wall_Distance_Blk's
+  // order); no wall: huge (kdtree_nearest on an empty tree)
+  for (int b = 0; b < nb; ++b) {
+    const BlockGeo& G = B[b];
+    for (int k = 1; k <= G.n3; ++k)
+      for (int j = 1; j <= G.n2; ++j)
+        for (int i = 1; i <= G.n1; ++i) {
+          const V3 q[8] = {G.nd(i, j, k), G.nd(i, j - 1, k), G.nd(i, j - 1, k - 1), G.nd(i, j, k - 1),
+                           G.nd(i - 1, j, k), G.nd(i - 1, j - 1, k), G.nd(i - 1, j - 1, k - 1), G.nd(i - 1, j, k - 1)};
+          double ctr[3];
+          for (int a = 0; a < 3; ++a) {
+            double sum = q[0].c[a];
+            for (int v = 1; v < 8; ++v) sum = sum + q[v].c[a];
+            ctr[a] = 1.0 / 8.0 * sum;
+          }
+          double best2 = std::numeric_limits<double>::max();
+          if (nw > 0) t.nearest(0, nw, 0, ctr[0], ctr[1], ctr[2], best2);
+          yn[G.row(i, j, k)] = nw > 0 ? std::sqrt(best2) : std::numeric_limits<double>::max();
+        }
+  }
+  // Yn_Connection: the first ghost layer of a connection face takes the donor cell's distance
+  for (int e = 0; e < nrec; ++e) {
+    const mose_bc_record& r = rec[e];
+    if (r.typ != MOSE_BC_CONNECTION) continue;
+    if (r.b < 0 || r.b >= nb || r.bs < 0 || r.bs >= nb || r.f < 1 || r.f > 6) return MOSE_ERR_ARG;
+    static const int guide[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+    const int* gd = guide[r.f - 1];
+    yn[B[r.b].row(r.i - gd[0], r.j - gd[1], r.k - gd[2])] = yn[B[r.bs].row(r.is, r.js, r.ks)];
+  }
+  return MOSE_OK;
 }
 
 // The unit tangent of each far-field face: the diagonal node(Fi,Fj,Fk) - node(one back along both transverse

@@ -607,7 +607,7 @@ __device__ __forceinline__ void diff_flux_st(const mose_dtab& T, const ViscConst
 }
 
 // ---------------------------------------------------------------------------------------------------------
-// conservative update of one cell: newstate_cons_body.inc:5-89 (rkc = RKcoeff, strang = 1)
+// conservative update of one cell: newstate_cons_body.inc:5-89 (rkc = RKcoeff, strang = strangcoeff)
 // returns false when the new state fails State_Check_blk (p <= 0 or non-finite)
 // ---------------------------------------------------------------------------------------------------------
 template <int NS, int NR>
@@ -634,14 +634,14 @@ __device__ __forceinline__ void prim2cons(const mose_dtab& T, const double* q, d
 
 template <int NS, int NR>
 __device__ __forceinline__ void update_cell(const mose_dtab& T, const double* p, const double* po, const double* res,
-                                            double vol, double dt, double rkc, double Tseed, double* out) {
+                                            double vol, double dt, double strang, double rkc, double Tseed, double* out) {
   using V = Vars<NS, NR>;
   double cold[V::nprim], c[V::nprim];
   prim2cons<NS, NR>(T, po, cold);
   prim2cons<NS, NR>(T, p, c);
 #pragma unroll
   for (int s = 0; s < V::nprim; ++s) {
-    double r = -res[s] / vol * dt * 1.0;
+    double r = -res[s] / vol * dt * strang;
     c[s] = cold[s] + (c[s] - cold[s] + r) * rkc;
   }
   double rho = rho_sum<NS>(c), Rg = f_Rtot<NS>(T, c);

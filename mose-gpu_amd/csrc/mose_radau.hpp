@@ -189,6 +189,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1))) k
   using T = RadauT;
   const double uround = 1.1e-19, safe = 0.9, thet = 0.001, quot1 = 1.0, quot2 = 1.2, facl = 5.0, facr = 1.0 / 8.0;
   const int nmax = 100000, nit = 7;
+  const double dtend = A.dtp ? *A.dtp : A.dt;   // XEND of every cell
   const double c1 = A.rk[0], c2 = A.rk[1], c1m1 = A.rk[2], c2m1 = A.rk[3], c1mc2 = A.rk[4];
   const double dd1 = A.rk[5], dd2 = A.rk[6], dd3 = A.rk[7], u1 = A.rk[8], alph = A.rk[9], beta = A.rk[10];
   const double fnewt = A.rk[11];
@@ -239,15 +240,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1))) k
       cell = c;
       y = cell_load<NS>(M, A, cell, base, r, rc, rho0, Tb);
       x = 0.0;
-      const double hmax = A.dt - x;
-      posneg = copysign(1.0, A.dt - x);
-      hmaxn = fmin(fabs(hmax), fabs(A.dt - x));
+      const double hmax = dtend - x;
+      posneg = copysign(1.0, dtend - x);
+      hmaxn = fmin(fabs(hmax), fabs(dtend - x));
       h = 1.0e-6;
       h = fmin(fabs(h), hmaxn);
       h = copysign(h, posneg);
       hold = h;
       reject = false; first = true; last = false;
-      if ((x + h * 1.0001 - A.dt) * posneg >= 0.0) { h = A.dt - x; last = true; }
+      if ((x + h * 1.0001 - dtend) * posneg >= 0.0) { h = dtend - x; last = true; }
       hopt = h; faccon = 1.0; nsing = 0; ier = 0; newt = 0; caljac = false; zzero = false;
       theta = 0.0; thqold = 0.0; dynold = 0.0; hacc = 0.0; erracc = 0.0;
       nstep = 0; naccpt = 0; nrejct = 0; nfcn = 0; idid = 0;
@@ -472,8 +473,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1))) k
           if (reject) hnew = posneg * fmin(fabs(hnew), fabs(h));
           reject = false;
           int nxt;
-          if ((x + hnew / quot1 - A.dt) * posneg >= 0.0) {
-            h = A.dt - x;
+          if ((x + hnew / quot1 - dtend) * posneg >= 0.0) {
+            h = dtend - x;
             last = true;
             nxt = theta <= thet ? LB_L20 : LB_L10;
           } else {

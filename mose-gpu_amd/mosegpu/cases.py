@@ -20,7 +20,7 @@ import numpy as np
 
 from .core import (BC_CONNECTION, BC_EXTRAPOLATION, BC_NONE, BC_FARFIELD, BC_WALL_ADIABATIC, BC_WALL_ISOTHERMAL,
                    FARFIELD_DTYPE, GC, LIM, NTAB, RECORD_DTYPE, RS, Geometry, Layout, Scheme, ThermoTables,
-                   metrics_build, records_tangent)
+                   metrics_build, records_tangent, wall_distance)
 
 RUNIV = 8314.46261815324
 WM_AIR = 28.970418          # test/1D/Sod79/INPUT/phase.txt
@@ -308,7 +308,9 @@ def channel_lattice(nb=(4, 4, 4), wmin: int = 24, wmax: int = 96, seed: int = 7,
         V[3] = 1e-3 * U * pert[2]
         V[4] = p_inf
         V[5] = mu_inf * rng.uniform(1.0, 5.0, size=yc.shape)
-        layout.interior(geom.yn, bix)[...] = yc
+    # the wall distance as the reference computes it (Compute_Yn: the nearest wall-face centroid; here the cell
+    # centre's height above the wall up to rounding)
+    wall_distance(layout, nodes, rec, geom.yn)
     fill_ghosts_nearest(layout, P)
     return Case("channel_lattice", tables, sch, layout, nodes, rec, geom, P, nsteps,
                 meta={"U": U, "rho_inf": rho_inf, "mu_inf": mu_inf, "widths": (wx, wy, wz), "T0": T0, "p0": p0},
@@ -382,3 +384,63 @@ def riemann2d(n: int = 256, nsteps: int = 100, seed: int = 0, cfl: float = 0.5, 
         V[0, 0][msk], V[1, 0][msk], V[2, 0][msk], V[4, 0][msk] = q[k][0], q[k][1], q[k][2], q[k][3]
     fill_ghosts_nearest(layout, P)
     return Case("riemann2d", tables, sch, layout, nodes, rec, geom, P, nsteps, meta={"states": q, "seed": seed})
+
+
+def h2_air_tables(viscous: bool = False, prandtl: float = 0.72) -> ThermoTables:
+    """The flow tables of the nine hydrogen-air species of mosegpu.chem (cp, h per Kelvin on 0..10000 K, as FLINT's
+    tables), and for viscous cases a power-law viscosity per species, mu_i = mu0_i (T / 300 K)^0.7 with mu0 of the
+    order of the gases' 300 K values, conductivity from the Prandtl number (the transport rows are fixtures like the
+    mechanism: plausible, not a validated model)."""
+    from . import chem
+    m = chem.h2_air_mechanism()
+    mu = k = None
+    if viscous:
+        mu0 = np.array([8.9e-6, 2.06e-5, 1.0e-5, 9.0e-6, 1.9e-5, 1.9e-5, 2.0e-5, 1.5e-5, 1.78e-5])
+        T = np.maximum(np.arange(NTAB, dtype=np.float64), 1.0)
+        mu = np.ascontiguousarray(mu0[:, None] * (T[None, :] / 300.0) ** 0.7)
+        k = np.ascontiguousarray(mu * m.cp / prandtl)
+    return ThermoTables(m.wm, m.cp, m.h, mu, k, names=m.names)
+
+
+def reacting_box(dims=(12, 10, 8), nsteps: int = 4, seed: int = 11, viscous: bool = False, phi: float = 1.0,
+                 T_range=(1100.0, 1500.0), riemann="HLLC", limiter="minmod", cfl: float = 0.5) -> Case:
+    """A periodic box of hydrogen-air (nine species, 1 cm on a side) with a seeded temperature field in T_range, a
+    trace of H, O and OH, a swirling velocity field of 50 m/s and a 2 % pressure wave about 1 atm: the transport
+    and the chemistry of a Strang-split step both act in every cell."""
+    from . import chem
+    tables = h2_air_tables(viscous)
+    n1, n2, n3 = dims
+    ext = (0.01, 0.01 * n2 / n1, 0.01 * n3 / n1)
+    c = box_case(dims, {f: "periodic" for f in range(1, 7)}, extent=ext, viscous=viscous, ndir=3, nsteps=nsteps,
+                 riemann=riemann, limiter=limiter, cfl=cfl, prandtl=0.72 if viscous else 0.0)
+    c.tables = tables
+    c.scheme.nsc = 9
+    c.P = np.zeros((13, c.layout.NC))
+    c.name = "reacting_box"
+    rng = np.random.Generator(np.random.PCG64(seed))
+    x = (np.arange(n1) + 0.5) / n1
+    y = (np.arange(n2) + 0.5) / n2
+    z = (np.arange(n3) + 0.5) / n3
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    a = rng.uniform(0.0, 2 * np.pi, 3)
+    T = T_range[0] + (T_range[1] - T_range[0]) * 0.5 * (1 + np.sin(2 * np.pi * X + a[0]) * np.cos(2 * np.pi * Y + a[1]))
+    p = 101325.0 * (1.0 + 0.02 * np.sin(2 * np.pi * Z + a[2]))
+    S = chem.SPECIES
+    Xm = np.zeros((9,) + X.shape)
+    tot = 2 * phi + 1 + 3.76
+    Xm[S.index("H2")] = 2 * phi / tot
+    Xm[S.index("O2")] = 1 / tot
+    Xm[S.index("N2")] = 3.76 / tot
+    for sp in ("H", "O", "OH"):
+        Xm[S.index(sp)] = 1e-4
+    Xm /= Xm.sum(axis=0, keepdims=True)
+    V = c.layout.interior(c.P, 0)
+    for q in range(9):
+        V[q] = Xm[q] * chem.WM[q] * p / (chem.RU * T)
+    V[9] = 50.0 * np.sin(2 * np.pi * Y)
+    V[10] = 30.0 * np.cos(2 * np.pi * X)
+    V[11] = 10.0 * np.sin(2 * np.pi * Z)
+    V[12] = p
+    fill_ghosts_nearest(c.layout, c.P)
+    c.meta.update({"phi": phi, "T_range": T_range})
+    return c

@@ -70,7 +70,7 @@ class Tables(C.Structure):
                 ("mu", C.POINTER(C.c_double)), ("k", C.POINTER(C.c_double))]
 
 
-PATH_FUSED, PATH_PLAIN = 0, 1
+PATH_FUSED, PATH_PLAIN, PATH_FAST = 0, 1, 2
 
 
 class Tuning(C.Structure):
@@ -135,6 +135,7 @@ def lib() -> C.CDLL:
     L.mose_ctx_destroy.argtypes = [vp]
     L.mose_metrics_build.argtypes = [ip, C.POINTER(C.c_int), dp, ip, C.c_void_p, ip, dp, dp, dp, dp, dp, dp, dp]
     L.mose_records_tangent.argtypes = [ip, C.POINTER(C.c_int), dp, ip, C.c_void_p]
+    L.mose_wall_distance.argtypes = [ip, C.POINTER(C.c_int), dp, ip, C.c_void_p, dp]
     L.mose_case_create.argtypes = [vp, C.POINTER(Scheme), ip, C.POINTER(C.c_int), dp, dp, dp, dp, dp, dp, dp, ip,
                                    C.c_void_p, ip, C.c_void_p, C.POINTER(C.c_int)]
     L.mose_case_destroy.argtypes = [vp, ip]
@@ -161,6 +162,9 @@ def lib() -> C.CDLL:
     L.mose_pow_batch.argtypes = [vp, C.c_int64, dp, dp, dp]
     L.mose_chem_advance_cells.argtypes = [vp, ip, C.c_int64, C.c_int64, vp, vp, vp, vp, C.c_double,
                                           C.POINTER(OdeOpts), C.POINTER(ChemReport)]
+    L.mose_case_chem_attach.argtypes = [vp, ip, ip, C.POINTER(OdeOpts)]
+    L.mose_case_chem_advance.argtypes = [vp, ip, C.c_double, C.POINTER(ChemReport)]
+    L.mose_case_chem_report.argtypes = [vp, ip, C.POINTER(ChemReport)]
     for name in ("mose_ctx_create", "mose_ctx_destroy", "mose_metrics_build", "mose_records_tangent", "mose_case_create",
                  "mose_case_destroy", "mose_case_bytes", "mose_state_upload", "mose_state_download",
                  "mose_case_step", "mose_cases_step", "mose_case_launch", "mose_case_poll", "mose_case_residual",
@@ -168,7 +172,8 @@ def lib() -> C.CDLL:
                  "mose_case_residual_norm", "mose_case_state_hash", "mose_case_timing", "mose_device_count",
                  "mose_case_tune", "mose_bandwidth_probe", "mose_ctx_info", "mose_chem_create", "mose_chem_destroy",
                  "mose_chem_upload", "mose_chem_download", "mose_chem_advance", "mose_pow_batch",
-                 "mose_chem_advance_cells"):
+                 "mose_chem_advance_cells", "mose_case_chem_attach", "mose_case_chem_advance",
+                 "mose_case_chem_report", "mose_wall_distance"):
         getattr(L, name).restype = C.c_int
     _lib_handle = L
     return L
@@ -244,6 +249,17 @@ def metrics_build(layout: Layout, nodes: list[np.ndarray], records: np.ndarray, 
     if rc != MOSE_OK:
         raise MoseError(rc, "mose_metrics_build failed (degenerate cell or bad record)")
     return Geometry(layout, dl, m, vol, yn, fm1, fm2, fm3)
+
+
+def wall_distance(layout: Layout, nodes: list[np.ndarray], records: np.ndarray, yn: np.ndarray) -> None:
+    """Compute_Yn + Yn_Connection (Lib_Metrics.f90:513-763) through mose_wall_distance, into yn [NC] in place."""
+    flat = np.ascontiguousarray(np.concatenate([np.ascontiguousarray(n, dtype=np.float64).reshape(-1) for n in nodes]))
+    rec = np.ascontiguousarray(records, dtype=RECORD_DTYPE)
+    assert yn.dtype == np.float64 and yn.flags.c_contiguous and yn.size == layout.NC
+    rc = lib().mose_wall_distance(layout.nb, iptr(layout.dims_arr), dptr(flat), len(rec),
+                                  rec.ctypes.data_as(C.c_void_p), dptr(yn))
+    if rc != MOSE_OK:
+        raise MoseError(rc, "mose_wall_distance failed")
 
 
 def records_tangent(layout: Layout, nodes: list[np.ndarray], records: np.ndarray) -> None:
@@ -356,6 +372,17 @@ class DeviceCase:
         self.ctx.check(lib().mose_case_step(self.ctx.h, self.id, nsteps, C.byref(rep)), "mose_case_step")
         return rep
 
+    def run_until(self, t_end: float, max_steps: int = 10 ** 7) -> Report:
+        """Steps while the case's time is below t_end: the reference's time-threshold loop (`do while time <
+        timemax`, without the optional dt clip of Clip_dt_To_Timemax, Mod_dt.f90:283-296), so the last step may end
+        past t_end.  One step per call: the time of each step is the device's dt of that step."""
+        rep = self.step(0)
+        n = 0
+        while rep.time < t_end and n < max_steps:
+            rep = self.step(1)
+            n += 1
+        return rep
+
     def launch(self, nsteps: int):
         """Queue nsteps steps and return at once (mose_case_launch); poll() until it reports them done."""
         self.ctx.check(lib().mose_case_launch(self.ctx.h, self.id, nsteps), "mose_case_launch")
@@ -402,6 +429,23 @@ class DeviceCase:
         a, t, n = C.c_double(), C.c_double(), C.c_int64()
         self.ctx.check(lib().mose_case_timing(self.ctx.h, self.id, C.byref(a), C.byref(t), C.byref(n)), "timing")
         return a.value, t.value, n.value
+
+    def chem_attach(self, chem: "DeviceChem | None", opts=None):
+        """mose_case_chem_attach: Strang-split steps with chem's mechanism (None: plain transport steps)."""
+        rc = lib().mose_case_chem_attach(self.ctx.h, self.id, -1 if chem is None else chem.id,
+                                         None if opts is None else C.byref(opts))
+        self.ctx.check(rc, "mose_case_chem_attach")
+
+    def chem_advance(self, dt: float) -> "ChemReport":
+        """mose_case_chem_advance: Chemistry_Newstate_blk on the resident state over dt."""
+        rep = ChemReport()
+        self.ctx.check(lib().mose_case_chem_advance(self.ctx.h, self.id, dt, C.byref(rep)), "mose_case_chem_advance")
+        return rep
+
+    def chem_report(self) -> "ChemReport":
+        rep = ChemReport()
+        self.ctx.check(lib().mose_case_chem_report(self.ctx.h, self.id, C.byref(rep)), "mose_case_chem_report")
+        return rep
 
     def destroy(self):
         if self.id >= 0:

@@ -1677,7 +1677,7 @@ void o_newstate(const o_case *C, double *P, const double *PO, const double *R, d
   const o_scheme *S = &C->sc;
   const o_tables *t = &C->tab;
   const int64_t NC = C->NC;
-  const double strang = 1.0;
+  const double strang = C->strang;   /* strangcoeff: 1, or 1/2 in a Strang-split step (Lib_Strang.f90:10-22) */
   int nsc = S->nsc, np = S->np, nu = S->nu, nw = S->nw, nprim = S->nprim;
   int tmin_ = 0, tmax_ = t->tmax;
   for (int b = 0; b < C->nb; ++b) {
@@ -1763,20 +1763,25 @@ static double rk_coeff(int n_rk, int irk) {
   return c[n_rk - 1][irk - 1];
 }
 
-/* Explicit_Step (Mod_Explicit.f90:52-397), time accurate, no Strang split:
- * dt from the state at the start of the step, PO <- P, then n_RK stages of
- * ghosts -> residual -> conservative update. */
+/* Explicit_Step (Mod_Explicit.f90:52-397), time accurate: dt from the state at the start of the step, then the
+ * strang loop (:155-294): per substage PO <- P (Copy_State) and n_RK stages of ghosts -> residual -> conservative
+ * update with res = -R/vol dt strangcoeff; with chemistry (n_strang = 2, strangcoeff = 1/2) the first substage is
+ * followed by Chemistry_Newstate_blk over the whole dt on the resident state (:282-294). */
 int o_step(const o_case *C, double *P, double *PO, double *R, int nsteps, double *dt_out) {
   const int64_t NC = C->NC;
   int nprim = C->sc.nprim;
+  const int n_strang = C->n_strang > 0 ? C->n_strang : 1;
   for (int n = 0; n < nsteps; ++n) {
     double dt = o_compute_dt(C, P);
     if (dt_out) dt_out[n] = dt;
-    memcpy(PO, P, sizeof(double) * NC * nprim);
-    for (int irk = 1; irk <= C->sc.n_rk; ++irk) {
-      o_fill_ghosts(C, P);
-      o_residual(C, P, R, dt);
-      o_newstate(C, P, PO, R, dt, rk_coeff(C->sc.n_rk, irk));
+    for (int i_strang = 1; i_strang <= n_strang; ++i_strang) {
+      memcpy(PO, P, sizeof(double) * NC * nprim);
+      for (int irk = 1; irk <= C->sc.n_rk; ++irk) {
+        o_fill_ghosts(C, P);
+        o_residual(C, P, R, dt);
+        o_newstate(C, P, PO, R, dt, rk_coeff(C->sc.n_rk, irk));
+      }
+      if (i_strang + n_strang == 3 && C->react) C->react(C, P, dt, C->react_ctx);
     }
   }
   return 0;

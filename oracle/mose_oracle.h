@@ -66,7 +66,11 @@ typedef struct {
 
 
 
-typedef struct {
+struct o_case_s;
+/* the chemistry of a Strang-split step: react the interior cells of P in place over dt (Chemistry_Newstate_blk) */
+typedef void (*o_react_fn)(const struct o_case_s *C, double *P, double dt, void *ctx);
+
+typedef struct o_case_s {
   o_scheme sc;
   o_tables tab;
   int nb;
@@ -78,6 +82,12 @@ typedef struct {
   int nff;
   const mose_farfield *ff;      /* far-field table of the type-4 records (rec->ff) */
   int nthreads;
+  /* Strang splitting (Lib_Strang.f90:10-22): n_strang = 1, strang = 1 without chemistry; n_strang = 2, strang = 1/2
+   * and react != NULL with it */
+  int n_strang;
+  double strang;
+  o_react_fn react;
+  void *react_ctx;
 } o_case;
 
 #ifdef __cplusplus

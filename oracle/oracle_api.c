@@ -21,12 +21,24 @@ void o_riemann(const o_case *C, int rs, double bet, const double *dl, double ul,
                double drtot, double nx, double ny, double nz, double F[5]);
 double o_sound_speed(const o_case *C, const double *rhoi, double p);
 
+/* the chemistry oracle (mose_chem_oracle.c) */
+int oc_advance_m(void *h, int method, int64_t ncell, double *state, double dt, const double *rtol, const double *atol,
+                 double *hr, int32_t *idid, int32_t *nstep, int nthreads);
+
+typedef struct {
+  void *mech;                          /* oc_create handle */
+  int method;                          /* MOSE_ODE_RODAS / MOSE_ODE_RADAU5 */
+  double rtol[ONSMAX + 1], atol[ONSMAX + 1];
+  int64_t failed;                      /* cells that ended with idid < 0, summed over the calls */
+} oa_chem;
+
 typedef struct {
   o_case c;
   o_block *blk;
   mose_bc_record *rec;
   mose_farfield *ff;
   double *tabs;
+  oa_chem chem;
 } oa_t;
 
 static void tables_init(o_tables *t, const mose_tables *tb, double *store) {
@@ -86,7 +98,65 @@ void *oa_create(const mose_scheme *sc, const mose_tables *tb, int nb, const int 
   if (nff > 0) memcpy(a->ff, ff, sizeof(mose_farfield) * (size_t)nff);
   C->nff = nff; C->ff = a->ff;
   C->nthreads = nthreads;
+  C->n_strang = 1;
+  C->strang = 1.0;
   return a;
+}
+
+/* Chemistry_Newstate_blk (src/lib/physics/Lib_Chemistry.f90:33-76) over every block: the interior cells' species
+ * densities and pressure gathered into [ns + 1][n], advanced over dt by the chemistry oracle, scattered back. */
+static void oa_react(const o_case *C, double *P, double dt, void *ctx) {
+  oa_chem *ch = (oa_chem *)ctx;
+  const o_scheme *S = &C->sc;
+  int64_t n = 0;
+  for (int b = 0; b < C->nb; ++b) n += (int64_t)C->blk[b].n1 * C->blk[b].n2 * C->blk[b].n3;
+  const int nz = S->nsc + 1;
+  double *st = (double *)malloc(sizeof(double) * (size_t)nz * (size_t)n);
+  int64_t *rows = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
+  int64_t q = 0;
+  for (int b = 0; b < C->nb; ++b) {
+    const o_block *B = &C->blk[b];
+    for (int k = 1; k <= B->n3; ++k)
+      for (int j = 1; j <= B->n2; ++j)
+        for (int i = 1; i <= B->n1; ++i)
+          rows[q++] = B->boff + (i + OGC - 1) + (int64_t)(j + OGC - 1) * B->pi + (int64_t)(k + OGC - 1) * B->pi * B->pj;
+  }
+  for (int64_t c = 0; c < n; ++c) {
+    for (int s = 0; s < S->nsc; ++s) st[(int64_t)s * n + c] = P[(int64_t)s * C->NC + rows[c]];
+    st[(int64_t)S->nsc * n + c] = P[(int64_t)S->np * C->NC + rows[c]];
+  }
+  ch->failed += oc_advance_m(ch->mech, ch->method, n, st, dt, ch->rtol, ch->atol, NULL, NULL, NULL, C->nthreads);
+  for (int64_t c = 0; c < n; ++c) {
+    for (int s = 0; s < S->nsc; ++s) P[(int64_t)s * C->NC + rows[c]] = st[(int64_t)s * n + c];
+    P[(int64_t)S->np * C->NC + rows[c]] = st[(int64_t)S->nsc * n + c];
+  }
+  free(st);
+  free(rows);
+}
+
+/* Chemistry on (mech != NULL): Strang-split steps, half transport, chemistry over dt, half transport
+ * (set_StrangSplitting, Lib_Strang.f90:10-22); mech = NULL turns it off. */
+void oa_set_chemistry(void *h, void *mech, int method, const double *rtol, const double *atol) {
+  oa_t *a = (oa_t *)h;
+  o_case *C = &a->c;
+  if (!mech) {
+    C->n_strang = 1; C->strang = 1.0; C->react = NULL; C->react_ctx = NULL;
+    return;
+  }
+  a->chem.mech = mech;
+  a->chem.method = method;
+  for (int q = 0; q <= C->sc.nsc; ++q) { a->chem.rtol[q] = rtol[q]; a->chem.atol[q] = atol[q]; }
+  a->chem.failed = 0;
+  C->n_strang = 2; C->strang = 0.5; C->react = oa_react; C->react_ctx = &a->chem;
+}
+
+/* the chemistry oracle's failed cells (idid < 0) since oa_set_chemistry */
+int64_t oa_chem_failed(void *h) { return ((oa_t *)h)->chem.failed; }
+
+/* Chemistry_Newstate_blk alone on the interior of P (the chemistry set by oa_set_chemistry) */
+void oa_react_only(void *h, double *P, double dt) {
+  oa_t *a = (oa_t *)h;
+  oa_react(&a->c, P, dt, &a->chem);
 }
 
 void oa_destroy(void *h) {

@@ -38,6 +38,10 @@ def oracle_lib():
     L.oa_limiter.argtypes = [ip, C.c_double, C.c_double]
     L.oa_conv_face.argtypes = [vp, dp, dp, dp, dp, dp, dp, C.c_double, dp]
     L.oa_riemann.argtypes = [vp, ip, dp, dp, dp, dp]
+    L.oa_set_chemistry.argtypes = [vp, vp, ip, dp, dp]
+    L.oa_chem_failed.restype = C.c_int64
+    L.oa_chem_failed.argtypes = [vp]
+    L.oa_react_only.argtypes = [vp, dp, C.c_double]
     L.oa_sound_speed.restype = C.c_double
     L.oa_sound_speed.argtypes = [vp, dp, C.c_double]
     L.oc_create.restype = vp
@@ -102,6 +106,10 @@ class OracleCase:
         oracle_lib().oa_residual(self.h, _d(P), _d(R), dt)
         return R
 
+    def newstate(self, P: np.ndarray, PO: np.ndarray, R: np.ndarray, dt: float, rkc: float):
+        """one RK stage update of P from the base PO and the residual R (Newstate_Conservative_blk)"""
+        oracle_lib().oa_newstate(self.h, _d(P), _d(PO), _d(R), dt, rkc)
+
     def dt(self, P: np.ndarray) -> float:
         return oracle_lib().oa_dt(self.h, _d(P))
 
@@ -120,6 +128,21 @@ class OracleCase:
         a = [np.ascontiguousarray(x, dtype=float) for x in (pm1, p0, pp1, pp2, dlc, nrm)]
         oracle_lib().oa_conv_face(self.h, *[_d(x) for x in a], float(area), _d(res))
         return res
+
+    def set_chemistry(self, ochem: "OracleChem | None", method: int = 1, rtol: float = 1e-8, atol: float = 1e-12):
+        """Strang-split steps with the chemistry oracle's mechanism (None: plain steps), as mose_case_chem_attach."""
+        nz = self.case.scheme.nsc + 1
+        self._chem = ochem
+        self._tol = (np.full(nz, rtol), np.full(nz, atol))
+        oracle_lib().oa_set_chemistry(self.h, None if ochem is None else ochem.h, method, _d(self._tol[0]),
+                                      _d(self._tol[1]))
+
+    def chem_failed(self) -> int:
+        return oracle_lib().oa_chem_failed(self.h)
+
+    def react(self, P: np.ndarray, dt: float):
+        """Chemistry_Newstate_blk alone on the interior of P (the chemistry of set_chemistry)."""
+        oracle_lib().oa_react_only(self.h, _d(P), dt)
 
     def sound_speed(self, rhoi, p):
         return oracle_lib().oa_sound_speed(self.h, _d(np.ascontiguousarray(rhoi, dtype=float)), float(p))

@@ -152,3 +152,83 @@ def test_degenerate_cell_is_refused():
     nodes[:, :, 1, :] = nodes[:, :, 0, :]          # collapse the first column of cells
     with pytest.raises(Exception):
         build(dims, nodes, {f: BC_EXTRAPOLATION for f in range(1, 7)})
+
+
+def brute_wall_distance(lay, nodes, rec):
+    """Compute_Yn by a scan of every wall-face centroid (the k-d tree's answer must be this one)."""
+    from mosegpu.core import BC_WALL_ADIABATIC, BC_WALL_ISOTHERMAL
+    cen = []
+    for r in rec:
+        if r["typ"] not in (BC_WALL_ADIABATIC, BC_WALL_ISOTHERMAL):
+            continue
+        nd = nodes[r["b"]]
+        i, j, k, f = int(r["i"]), int(r["j"]), int(r["k"]), int(r["f"])
+        m2 = f % 2
+        if f <= 2:
+            c = [(i - m2, j - 1, k - 1), (i - m2, j, k - 1), (i - m2, j - 1, k), (i - m2, j, k)]
+        elif f <= 4:
+            c = [(i - 1, j - m2, k - 1), (i, j - m2, k - 1), (i - 1, j - m2, k), (i, j - m2, k)]
+        else:
+            c = [(i - 1, j - 1, k - m2), (i, j - 1, k - m2), (i - 1, j, k - m2), (i, j, k - m2)]
+        p = [nd[kk, jj, ii] for ii, jj, kk in c]
+        cen.append([0.25 * (p[0][a] + p[1][a] + p[2][a] + p[3][a]) for a in range(3)])
+    cen = np.array(cen)
+    out = []
+    for b, nd in enumerate(nodes):
+        order = [(0, 0, 0), (0, -1, 0), (0, -1, -1), (0, 0, -1), (-1, 0, 0), (-1, -1, 0), (-1, -1, -1), (-1, 0, -1)]
+        n3, n2, n1 = nd.shape[0] - 1, nd.shape[1] - 1, nd.shape[2] - 1
+        s = None
+        for (di, dj, dk) in order:
+            v = nd[1 + dk:n3 + 1 + dk, 1 + dj:n2 + 1 + dj, 1 + di:n1 + 1 + di]
+            s = v.copy() if s is None else s + v
+        ctr = 1.0 / 8.0 * s                                    # (n3, n2, n1, 3)
+        dx = cen[None, None, None, :, 0] - ctr[..., 0:1]
+        dy = cen[None, None, None, :, 1] - ctr[..., 1:2]
+        dz = cen[None, None, None, :, 2] - ctr[..., 2:3]
+        out.append(np.sqrt((dx * dx + dy * dy + dz * dz).min(axis=-1)))
+    return out
+
+
+def test_wall_distance_is_the_nearest_wall_centroid():
+    """mose_wall_distance on the channel lattice of seeded uneven blocks (wall at y = 0), its nodes sheared: the
+    k-d tree gives the scan's distances bit for bit; on the unsheared lattice the distance is the cell centre's y."""
+    from mosegpu.core import wall_distance
+    case = cases.channel_lattice(nb=(2, 2, 2), wmin=3, wmax=6, seed=11)
+    lay = case.layout
+    yn = np.zeros(lay.NC)
+    wall_distance(lay, case.nodes, case.records, yn)
+    for b, nd in enumerate(case.nodes):
+        ctr = 0.125 * (nd[:-1, :-1, :-1] + nd[1:, :-1, :-1] + nd[:-1, 1:, :-1] + nd[:-1, :-1, 1:] + nd[1:, 1:, :-1]
+                       + nd[1:, :-1, 1:] + nd[:-1, 1:, 1:] + nd[1:, 1:, 1:])
+        assert np.allclose(lay.interior(yn, b)[0] if lay.interior(yn, b).ndim == 4 else lay.interior(yn, b),
+                           ctr[..., 1], rtol=1e-14, atol=0)
+    sheared = [nd + np.stack([0.3 * nd[..., 1] + 0.1 * nd[..., 2], np.zeros_like(nd[..., 1]),
+                              0.2 * nd[..., 0] * nd[..., 1]], axis=-1) for nd in case.nodes]
+    yn2 = np.zeros(lay.NC)
+    wall_distance(lay, sheared, case.records, yn2)
+    ref = brute_wall_distance(lay, sheared, case.records)
+    for b in range(lay.nb):
+        got = lay.interior(yn2.reshape(1, -1), b)[0]
+        assert np.array_equal(got, ref[b]), np.max(np.abs(got - ref[b]))
+
+
+def test_wall_distance_connection_ghosts():
+    """Yn_Connection: the first ghost layer behind a connection face holds the donor cell's distance."""
+    from mosegpu.core import BC_CONNECTION, wall_distance
+    case = cases.channel_lattice(nb=(2, 2, 2), wmin=3, wmax=6, seed=11)
+    lay = case.layout
+    yn = np.full(lay.NC, -1.0)
+    wall_distance(lay, case.nodes, case.records, yn)
+    guide = {1: (1, 0, 0), 2: (-1, 0, 0), 3: (0, 1, 0), 4: (0, -1, 0), 5: (0, 0, 1), 6: (0, 0, -1)}
+    nconn = 0
+    for r in case.records:
+        if r["typ"] != BC_CONNECTION:
+            continue
+        g = guide[int(r["f"])]
+        vb = lay.block_view(yn.reshape(1, -1), int(r["b"]))[0]
+        vs = lay.block_view(yn.reshape(1, -1), int(r["bs"]))[0]
+        ghost = vb[r["k"] - g[2] + GC - 1, r["j"] - g[1] + GC - 1, r["i"] - g[0] + GC - 1]
+        donor = vs[r["ks"] + GC - 1, r["js"] + GC - 1, r["is_"] + GC - 1]
+        assert ghost == donor
+        nconn += 1
+    assert nconn > 0

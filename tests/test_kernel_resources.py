@@ -30,35 +30,44 @@ def kernels(tmp_path_factory):
         pytest.skip("built library or LLVM tools missing")
     objcopy, bundler, readelf = tools
     d = tmp_path_factory.mktemp("co")
-    fat, co = str(d / "fat.bin"), str(d / "gfx950.co")
+    fat = str(d / "fat.bin")
     subprocess.run([objcopy, "--dump-section=.hip_fatbin=" + fat, lib, str(d / "host.o")], check=True,
                    capture_output=True)
-    subprocess.run([bundler, "--unbundle", "--type=o", "--input=" + fat,
-                    "--targets=hipv4-amdgcn-amd-amdhsa--gfx950", "--output=" + co], check=True, capture_output=True)
-    notes = subprocess.run([readelf, "--notes", co], check=True, capture_output=True, text=True).stdout
+    # one offload bundle per translation unit with device code, concatenated by the linker
+    data = open(fat, "rb").read()
+    starts = [m.start() for m in re.finditer(b"__CLANG_OFFLOAD_BUNDLE__", data)] + [len(data)]
     out = {}
-    for item in re.split(r"\n  - \.", notes)[1:]:
-        kv = {}
-        for line in ("." + item).splitlines():
-            m = re.match(r"^(?:    )?\.([a-z_]+):\s+(\S+)", line) if not line.startswith("      ") else None
-            if m:
-                kv.setdefault(m.group(1), m.group(2))
-        if "name" in kv:
-            out[kv["name"]] = kv
+    for n, (a, b) in enumerate(zip(starts[:-1], starts[1:])):
+        part, co = str(d / f"fat{n}.bin"), str(d / f"gfx950_{n}.co")
+        with open(part, "wb") as f:
+            f.write(data[a:b])
+        subprocess.run([bundler, "--unbundle", "--type=o", "--input=" + part,
+                        "--targets=hipv4-amdgcn-amd-amdhsa--gfx950", "--output=" + co], check=True,
+                       capture_output=True)
+        notes = subprocess.run([readelf, "--notes", co], check=True, capture_output=True, text=True).stdout
+        for item in re.split(r"\n  - \.", notes)[1:]:
+            kv = {}
+            for line in ("." + item).splitlines():
+                m = re.match(r"^(?:    )?\.([a-z_]+):\s+(\S+)", line) if not line.startswith("      ") else None
+                if m:
+                    kv.setdefault(m.group(1), m.group(2))
+            if "name" in kv:
+                out[kv["name"]] = kv
     assert out, "no kernel metadata found"
     return out
 
 
-def _stage(kernels, nr, visc, rs, sh):
-    name = f"_ZN4mose7k_stageILi1ELi{nr}ELb{int(visc)}ELi{rs}ELi512ELi{sh}EEEvNS_7CaseDevENS_9StageArgsE"
+def _stage(kernels, nr, visc, rs, sh, fast=False):
+    name = (f"_ZN4mose7k_stageILi1ELi{nr}ELb{int(visc)}ELi{rs}ELi512ELi{sh}ELb{int(fast)}EEEvNS_7CaseDevENS_9StageArgsE")
     assert name in kernels, name
     return kernels[name]
 
 
+@pytest.mark.parametrize("fast", [False, True])
 @pytest.mark.parametrize("visc", [True, False])
 @pytest.mark.parametrize("rs", [1, 4, 12])
-def test_stage_8x8x4_fits_two_workgroups_per_cu(kernels, visc, rs):
-    k = _stage(kernels, 0, visc, rs, 1)
+def test_stage_8x8x4_fits_two_workgroups_per_cu(kernels, visc, rs, fast):
+    k = _stage(kernels, 0, visc, rs, 1, fast)
     assert int(k["vgpr_count"]) <= 128
     assert int(k["vgpr_spill_count"]) == 0 and int(k["private_segment_fixed_size"]) == 0
 
@@ -68,8 +77,9 @@ def test_stage_8x8x4_fits_two_workgroups_per_cu(kernels, visc, rs):
 def test_stage_kernels_do_not_spill(kernels, nr, sh):
     for visc in (True, False):
         for rs in (1, 4, 12):
-            k = _stage(kernels, nr, visc, rs, sh)
-            assert int(k["vgpr_spill_count"]) == 0, (nr, sh, visc, rs)
+            for fast in (False, True):
+                k = _stage(kernels, nr, visc, rs, sh, fast)
+                assert int(k["vgpr_spill_count"]) == 0, (nr, sh, visc, rs, fast)
 
 
 def test_chemistry_occupancy(kernels):
