@@ -230,6 +230,52 @@ module mose_gpu_c
       type(mose_ode_opts), intent(in) :: opts
       type(mose_chem_report), intent(out) :: report
     end function
+    ! chemistry on a case's resident state and the Strang-split step (Lib_Chemistry.f90:33-76, Mod_Explicit.f90:155-294)
+    integer(c_int) function mose_case_chem_attach(ctx, case_id, chem_id, opts) bind(C, name="mose_case_chem_attach")
+      import :: c_int, c_ptr, mose_ode_opts
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: case_id, chem_id
+      type(mose_ode_opts), intent(in) :: opts
+    end function
+    integer(c_int) function mose_case_chem_advance(ctx, case_id, dt, report) bind(C, name="mose_case_chem_advance")
+      import :: c_int, c_ptr, c_double, mose_chem_report
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: case_id
+      real(c_double), value :: dt
+      type(mose_chem_report), intent(out) :: report
+    end function
+    integer(c_int) function mose_case_chem_report(ctx, case_id, report) bind(C, name="mose_case_chem_report")
+      import :: c_int, c_ptr, mose_chem_report
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: case_id
+      type(mose_chem_report), intent(out) :: report
+    end function
+    ! serving: asynchronous steps of a case, the stage kernel's timing, the device's memory
+    integer(c_int) function mose_case_launch(ctx, case_id, nsteps) bind(C, name="mose_case_launch")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: case_id, nsteps
+    end function
+    integer(c_int) function mose_case_poll(ctx, case_id, done, rep) bind(C, name="mose_case_poll")
+      import :: c_int, c_ptr, mose_report
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: case_id
+      integer(c_int), intent(out) :: done
+      type(mose_report), intent(out) :: rep
+    end function
+    integer(c_int) function mose_case_timing(ctx, case_id, avg_ms, total_ms, launches) bind(C, name="mose_case_timing")
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: case_id
+      real(c_double), intent(out) :: avg_ms, total_ms
+      integer(c_int64_t), intent(out) :: launches
+    end function
+    integer(c_int) function mose_ctx_info(ctx, total, freeb, cus, lds) bind(C, name="mose_ctx_info")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), intent(out) :: total, freeb
+      integer(c_int), intent(out) :: cus, lds
+    end function
   end interface
 
 contains

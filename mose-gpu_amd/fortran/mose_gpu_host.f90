@@ -8,14 +8,20 @@ module mose_gpu_host
 
 contains
 
-  !> n explicit steps of one case on the device (Explicit_Step, Mod_Explicit.f90:52-397, time accurate, without
-  !> chemistry).  On a failed state the run stops with the first bad cell (block, i, j, k).
+  !> n explicit steps of one case on the device (Explicit_Step, Mod_Explicit.f90:52-397, time accurate; Strang-split
+  !> with chemistry once MOSE_GPU_Chemistry_Attach has coupled a mechanism).  On a failed state the run stops with the
+  !> first bad cell (block, i, j, k).  Chemistry cells whose integration ended with idid < 0 are reported and the run
+  !> goes on, as Chemistry_Newstate_blk writes such a cell back and carries on (Lib_Chemistry.f90:62-70).
   subroutine MOSE_GPU_Explicit_Step(ctx, case_id, nsteps, rep)
     type(c_ptr), intent(in) :: ctx
     integer(c_int), intent(in) :: case_id, nsteps
     type(mose_report), intent(out) :: rep
     integer(c_int) :: rc
     rc = mose_case_step(ctx, case_id, nsteps, rep)
+    if (rc == MOSE_ERR_ODE) then
+      write(*,'(2a)') ' [WARNING] chemistry: ', mose_error_text(ctx)
+      rc = MOSE_OK
+    end if
     if (rc == MOSE_ERR_STATE .or. rc == MOSE_ERR_THERMO) then
       write(*,'(a,i0,a,4(1x,i0))') ' [ERROR] state check: ', rep%bad_cells, ' bad cells; first (block i j k):', &
         rep%first_bad_block + 1, rep%first_bad_i, rep%first_bad_j, rep%first_bad_k
@@ -59,6 +65,23 @@ contains
     allocate(idid(ncell), nstep(ncell))
     call mose_check(ctx, mose_chem_download(ctx, chem_id, state, hr, idid, nstep), 'mose_chem_download')
   end subroutine MOSE_GPU_Chemistry_Newstate
+
+  !> Couples the mechanism of chemistry object chem_id to a case: its steps become Strang-split steps (half transport,
+  !> Chemistry_Newstate_blk over dt on the resident state, half transport; set_StrangSplitting, Lib_Strang.f90:10-22)
+  !> with the integrator `method` and raw tolerances per component.
+  subroutine MOSE_GPU_Chemistry_Attach(ctx, case_id, chem_id, ns, method, rtol, atol)
+    type(c_ptr), intent(in) :: ctx
+    integer(c_int), intent(in) :: case_id, chem_id, ns, method
+    real(c_double), intent(in) :: rtol(ns + 1), atol(ns + 1)
+    type(mose_ode_opts) :: opts
+    opts%method = method
+    opts%reserved = 0
+    opts%rtol = 0.0d0
+    opts%atol = 0.0d0
+    opts%rtol(1:ns + 1) = rtol
+    opts%atol(1:ns + 1) = atol
+    call mose_check(ctx, mose_case_chem_attach(ctx, case_id, chem_id, opts), 'mose_case_chem_attach')
+  end subroutine MOSE_GPU_Chemistry_Attach
 
   !> Stops with the library's message when a call failed.
   subroutine mose_check(ctx, rc, what)

@@ -10,8 +10,10 @@ that this library runs, and its solution written in the reference's output forma
   bc.txt         one record per boundary cell-face, 3-digit MOSE_open types (IO_BC_Open.f90:132-399)
 
 Only what this path runs is accepted; anything else is refused by name (local time stepping, prim/prec
-integration, Riemann solvers other than HLLC / AUSM+ / LMRoe, boundary types outside 0/100, 101/201, 301 with zero
-heat flux, 302 and 400, axisymmetric meshes).  Files may be stored gzip-compressed (name + ".gz").
+integration, Riemann solvers other than HLLC / HLLE / AUSM+ / LMRoe, boundary types outside 0/100, 101/201, 300
+(symmetry), 301 with zero heat flux, 302 and 400, axisymmetric meshes).  Files may be stored gzip-compressed
+(name + ".gz").  write_fortran_case / read_fortran_result are the raw case and result files of the Fortran driver
+mose_case_run.
 """
 from __future__ import annotations
 
@@ -545,3 +547,95 @@ def read_native(case: Case, path: str) -> tuple[np.ndarray, float]:
     time = float(np.frombuffer(recs[r], ">f8")[0])
     fill_ghosts_nearest(case.layout, P)
     return P, time
+
+
+def run_state_path(field_path: str, rank: int = 0) -> str:
+    """OUTPUT/<field>_state_r<rank>.bin beside the native restart file <field>.bin (Write_Restart_State,
+    Wrap_Postprocess.f90:337-357)."""
+    base = field_path[:-4] if field_path.endswith(".bin") else field_path
+    return f"{base}_state_r{rank}.bin"
+
+
+def write_run_state(field_path: str, time: float, iteration: int, iter_general: int | None = None, n_tdump: int = 0,
+                    time_from_call: float = 0.0, output_index: int = 0, rank: int = 0) -> str:
+    """The run-state sidecar of a native restart file (Write_Restart_State, Wrap_Postprocess.f90:337-357): a
+    big-endian stream of the version (2), the time, the iteration, the general iteration, the count of time-based
+    dumps, the time of the call and the output file index, then the multirate partition flag (a Fortran logical,
+    .false.: no multirate partition on this path, MR_Write_State, Mod_Multirate_Partition.f90:425-446)."""
+    path = run_state_path(field_path, rank)
+    with open(path, "wb") as f:
+        f.write(np.array([2], dtype=">i4").tobytes())
+        f.write(np.array([time], dtype=">f8").tobytes())
+        f.write(np.array([iteration, iteration if iter_general is None else iter_general, n_tdump], dtype=">i4").tobytes())
+        f.write(np.array([time_from_call], dtype=">f8").tobytes())
+        f.write(np.array([output_index, 0], dtype=">i4").tobytes())
+    return path
+
+
+def read_run_state(field_path: str, time: float, rank: int = 0) -> dict | None:
+    """The counters of a run-state sidecar, applied only when its time is the restart field's time to the bit
+    (Read_Restart_State, Wrap_Setup.f90:318-364): None when the file is absent or of another time."""
+    path = run_state_path(field_path, rank)
+    if not os.path.exists(path):
+        return None
+    raw = open(path, "rb").read()
+    ver = int(np.frombuffer(raw, ">i4", 1, 0)[0])
+    t = float(np.frombuffer(raw, ">f8", 1, 4)[0])
+    it, itg, ntd = (int(x) for x in np.frombuffer(raw, ">i4", 3, 12))
+    tfc = float(np.frombuffer(raw, ">f8", 1, 24)[0])
+    ids = int(np.frombuffer(raw, ">i4", 1, 32)[0]) if ver >= 2 else -1
+    if t != time:
+        return None
+    return {"version": ver, "time": t, "iteration": it, "iter_general": itg, "n_tdump": ntd, "time_from_call": tfc,
+            "output_index": ids}
+
+
+def run_case(dc, case):
+    """Runs a loaded reference case to its end on a DeviceCase: iter-threshold steps when the case gives them, else
+    until its time-threshold (DeviceCase.run_until).  Returns the last step report."""
+    if case.nsteps > 0:
+        return dc.step(case.nsteps)
+    t_end = case.meta.get("time_threshold", 0.0)
+    if not t_end > 0.0:
+        raise CaseFormatError("the case gives neither iter-threshold nor time-threshold")
+    return dc.run_until(t_end)
+
+
+def write_fortran_case(case, path, nsteps: int, tune_path: int = 0, tile=(0, 0, 0), mech=None, opts=None):
+    """The raw case file of the Fortran driver mose_case_run (fortran/mose_case_run.f90 gives the layout): the scheme,
+    blocks, geometry, records, far-field rows, tables and initial state of `case`, the step count and tuning, and for a
+    reacting case the mechanism (mosegpu.chem.Mechanism) with the integrator options (chem.ode_opts)."""
+    t = case.tables
+    visc_tabs = int(t.mu is not None and t.k is not None)
+    nz = case.scheme.nsc + 1
+    with open(path, "wb") as f:
+        hdr = [1, case.layout.nb, len(case.records), len(case.farfield), nsteps, tune_path, tile[0], tile[1], tile[2],
+               int(mech is not None), 0 if opts is None else opts.method, visc_tabs]
+        np.array(hdr, dtype=np.int32).tofile(f)
+        f.write(bytes(case.scheme))
+        np.ascontiguousarray(case.layout.dims_arr, dtype=np.int32).tofile(f)
+        for a in (t.wm, t.cp, t.h) + ((t.mu, t.k) if visc_tabs else ()):
+            np.ascontiguousarray(a, dtype=np.float64).tofile(f)
+        lay, g = case.layout, case.geom
+        np.array([lay.NC, lay.NF1, lay.NF2, lay.NF3], dtype=np.int64).tofile(f)
+        for a in (g.dl, g.m, g.vol, g.yn, g.fm1, g.fm2, g.fm3):
+            np.ascontiguousarray(a, dtype=np.float64).tofile(f)
+        np.ascontiguousarray(case.records).tofile(f)
+        np.ascontiguousarray(case.farfield).tofile(f)
+        np.ascontiguousarray(case.P, dtype=np.float64).tofile(f)
+        if mech is not None:
+            np.array([mech.ns, mech.n_arrh, mech.n_troe, mech.n_lind, 0, mech.cp.shape[1] - 1], dtype=np.int32).tofile(f)
+            for a in (mech.wm, mech.cp, mech.h, mech.ni1, mech.ni2, mech.eps, mech.kf, mech.kb, mech.troe, mech.lind):
+                np.ascontiguousarray(a, dtype=np.float64).tofile(f)
+            np.array([opts.rtol[q] for q in range(nz)], dtype=np.float64).tofile(f)
+            np.array([opts.atol[q] for q in range(nz)], dtype=np.float64).tofile(f)
+
+
+def read_fortran_result(case, path):
+    """(P, time, steps) of mose_case_run's output file."""
+    raw = np.fromfile(path, dtype=np.uint8)
+    n = case.nprim * case.layout.NC
+    P = raw[:8 * n].view(np.float64).reshape(case.nprim, case.layout.NC).copy()
+    time = float(raw[8 * n:8 * n + 8].view(np.float64)[0])
+    steps = int(raw[8 * n + 8:8 * n + 16].view(np.int64)[0])
+    return P, time, steps

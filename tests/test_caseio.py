@@ -257,3 +257,21 @@ def test_shu_osher_inputs_and_shock_speed():
     assert 1.7 < t < 1.9
     assert abs(jump - (-4.0 + 3.0 * np.sqrt(1.4) * t)) <= 0.1, (jump, t)
     assert W[0].min() > 0.7 and W[4].min() > 0.9
+
+
+def test_run_state_sidecar(tmp_path):
+    """The run-state sidecar of a native restart (Write_Restart_State / Read_Restart_State): 40 big-endian bytes,
+    version 2, and the counters come back only for the field of the same time, to the bit."""
+    from mosegpu import caseio
+    field = str(tmp_path / "field.bin")
+    path = caseio.write_run_state(field, 0.125, 340, 341, 2, 0.0625, 7)
+    assert path.endswith("field_state_r0.bin")
+    raw = open(path, "rb").read()
+    assert len(raw) == 40
+    assert raw[:4] == b"\x00\x00\x00\x02" and raw[-4:] == b"\x00\x00\x00\x00"      # version, no multirate partition
+    assert np.frombuffer(raw, ">f8", 1, 4)[0] == 0.125
+    st = caseio.read_run_state(field, 0.125)
+    assert st == {"version": 2, "time": 0.125, "iteration": 340, "iter_general": 341, "n_tdump": 2,
+                  "time_from_call": 0.0625, "output_index": 7}
+    assert caseio.read_run_state(field, np.nextafter(0.125, 1.0)) is None
+    assert caseio.read_run_state(str(tmp_path / "other.bin"), 0.125) is None

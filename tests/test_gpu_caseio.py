@@ -174,3 +174,30 @@ def test_hlle_cases_device_matches_oracle(gpu_ready, name, path, tile):
     print(f"{name} HLLE (reference inputs), path {path} tile {chosen} x 100: rel Linf {err:.3e}, values differing "
           f"bitwise {int(np.sum(a != b))} of {a.size}", flush=True)
     assert err <= TOL
+
+
+def test_run_to_time_threshold(gpu_ready):
+    """caseio.run_case on the reference's test/1D/Noh87 inputs, which end on time-threshold (no iter-threshold): the
+    device steps while its time is below the threshold, as the reference's loop does, and lands on the oracle's step
+    count, time and state."""
+    c = caseio.load_case(os.path.join(os.path.dirname(EIN91), "noh87"))
+    t_end = c.meta["time_threshold"]
+    assert c.nsteps == 0 and t_end > 0
+    ctx = Context(c.tables)
+    dc = DeviceCase(ctx, c)
+    dc.upload(c.P)
+    rep = caseio.run_case(dc, c)
+    Pg = dc.download()
+    dc.destroy()
+    ctx.close()
+    o = OracleCase(c)
+    Po = c.P.copy()
+    t, n = 0.0, 0
+    while t < t_end:
+        t += o.step(Po, 1)[0]
+        n += 1
+    assert rep.steps == n and rep.time >= t_end and abs(rep.time - t) <= 1e-13 * t
+    a, b = c.layout.interior(Pg, 0), c.layout.interior(Po, 0)
+    err = max(np.abs(a[s] - b[s]).max() / max(np.abs(b[s]).max(), 1.0) for s in range(c.nprim))
+    print(f"noh87 to time-threshold {t_end}: {n} steps, time {rep.time:.6f}, rel Linf {err:.3e}", flush=True)
+    assert err <= TOL

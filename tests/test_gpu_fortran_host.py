@@ -67,3 +67,44 @@ def test_fortran_chemistry_driver(gpu_ready, tmp_path):
     print(f"fortran chemistry driver: {r.stdout.strip()}; max |dev - oracle| {np.abs(dev - ref).max():.3e}")
     assert np.array_equal(dev, ref)
     assert np.array_equal(hr, hr_o)
+
+
+def run_case_driver(tmp_path, case, nsteps, path=0, tile=(0, 0, 0), mech=None, opts=None):
+    from mosegpu import caseio
+    drv = mosegpu.LIB_PATH.parent / "mose_case_run"
+    assert drv.exists(), "build the Fortran host first (make -C mose-gpu_amd)"
+    caseio.write_fortran_case(case, tmp_path / "case.bin", nsteps, path, tile, mech, opts)
+    r = subprocess.run([str(drv), "case.bin", "out.bin"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return caseio.read_fortran_result(case, tmp_path / "out.bin") + (r.stdout.strip(),)
+
+
+@pytest.mark.parametrize("path", [0, 1])
+def test_fortran_case_driver_taylor_green(gpu_ready, tmp_path, path):
+    """mose_case_run (the whole residual path through the ISO_C_BINDING module from a case file) on the viscous
+    Taylor-Green vortex 16^3: the oracle's state and time."""
+    from test_gpu_walls_sa import compare
+    case = cases.taylor_green(16, 4)
+    P, t, steps, out = run_case_driver(tmp_path, case, 4, path)
+    Po = case.P.copy()
+    dts = OracleCase(case).step(Po, 4)
+    assert steps == 4 and abs(t - dts.sum()) <= 1e-13 * dts.sum()
+    assert compare(case, P, Po, f"fortran case driver, TGV 16^3, path {path} ({out})") <= 1e-12
+
+
+def test_fortran_case_driver_reacting(gpu_ready, tmp_path):
+    """mose_case_run on the nine-species reacting box: MOSE_GPU_Chemistry_Attach couples the mechanism, the steps are
+    Strang-split, the state is the oracle's to the chemistry bar."""
+    from mosegpu import chem
+    from oracle_bind import OracleChem
+    from test_gpu_walls_sa import compare
+    mech = chem.h2_air_mechanism()
+    case = cases.reacting_box((12, 10, 8), T_range=(1200.0, 1700.0))
+    P, t, steps, out = run_case_driver(tmp_path, case, 3, 0, (0, 0, 0), mech, chem.ode_opts())
+    o = OracleCase(case)
+    o.set_chemistry(OracleChem(mech, nthreads=16))
+    Po = case.P.copy()
+    dts = o.step(Po, 3)
+    assert o.chem_failed() == 0
+    assert steps == 3 and abs(t - dts.sum()) <= 1e-13 * dts.sum()
+    assert compare(case, P, Po, f"fortran case driver, reacting box 12x10x8 x 3 Strang steps ({out})") <= 1e-8
