@@ -33,10 +33,11 @@ def algorithmic_bytes_per_cell_stage(nprim: int, viscous: bool, rans: bool) -> i
     return 8 * (3 * nprim + G)
 
 
-def profiled_counts(n: int, nprim: int, viscous: bool, tile) -> tuple[int | None, int | None]:
+def profiled_counts(n: int, nprim: int, viscous: bool, tile, path: str = "fused") -> tuple[int | None, int | None]:
     """(HBM bytes, VALU wavefront instructions) per stage launch, measured by rocprofv3 counters for this workload
     (profiles/*_traffic.txt), or None where the workload has not been profiled."""
-    key = ("k_stage", str(n), str(nprim), str(int(viscous)), "x".join(str(t) for t in tile))
+    kern = "k_stage" if path == "fused" else f"k_stage_{path}"
+    key = (kern, str(n), str(nprim), str(int(viscous)), "x".join(str(t) for t in tile))
     for f in sorted((REPO / "profiles").glob("*_traffic.txt")):
         for line in f.read_text().splitlines():
             w = line.split()
@@ -82,7 +83,7 @@ def cpu_baseline(n: int, steps: int, threads: int):
     rate1, dt1 = oracle_rate(n1, s1, 1)
     return {"value": rate, "unit": "cell-updates/s", "cores": threads, "kind": "port",
             "sample": f"taylor-green {n}^3, {steps} SSP-RK3 steps after 1 warm-up step, CPU oracle "
-                      f"(gcc -O2 -ffp-contract=off, OpenMP {threads} threads), {dt:.1f} s",
+                      f"(gcc -O3 -march=x86-64-v3 -ffp-contract=off, OpenMP {threads} threads), {dt:.1f} s",
             "single_thread": {"value": rate1, "sample": f"taylor-green {n1}^3, {s1} steps, 1 thread, {dt1:.1f} s"},
             "host": host_cpu()}
 
@@ -112,6 +113,14 @@ def chemistry_leg(ncell: int, cpu_cells: int, threads: int, radau_cells: int = 0
            "steps_max": int(nstep.max()), "rhs_evals": int(rep.rhs_total), "failed": int((idid < 0).sum()),
            "workload": "H2-O2-N2 9 species, 17 Arrhenius + 2 Troe, 1 atm, T U[900,2500] K, phi U[0.5,2], "
                        "dt 1e-6 s, RODAS rtol 1e-8 atol 1e-12"}
+    per = chem_valu_per_attempt()
+    if per:
+        # FP64 VALU issue: a wave64 instruction takes 4 cycles of one SIMD (1,024 SIMDs at 2.4 GHz); the share of the
+        # launch that the profiled instructions per step attempt, times this launch's attempts, take to issue
+        issue_s = per * rep.steps_total / VALU_ISSUE_PER_S      # steps_total counts every attempt (NSTEP)
+        res["roofline"] = {"bound": "fp64-valu", "valu_insts_per_step_attempt": per,
+                           "step_attempts": int(rep.steps_total),
+                           "valu_issue_s": issue_s, "valu_issue_frac": issue_s / (rep.ms * 1e-3)}
     print(f"bench: chemistry {ncell} cells in {rep.ms / 1e3:.2f} s: {res['value']:.3e} cells/s", file=sys.stderr,
           flush=True)
     if radau_cells > 0:
@@ -125,8 +134,8 @@ def chemistry_leg(ncell: int, cpu_cells: int, threads: int, radau_cells: int = 0
         o.advance(sample, 1e-6)
         dt = time.perf_counter() - t0
         res["cpu_baseline"] = {"value": cpu_cells / dt, "unit": "cells/s", "cores": threads, "kind": "port",
-                               "sample": f"the first {cpu_cells} cells of the same batch, CPU oracle RODAS (gcc -O2, "
-                                         f"OpenMP {threads} threads), {dt:.1f} s"}
+                               "sample": f"the first {cpu_cells} cells of the same batch, CPU oracle RODAS (gcc -O3 "
+                                         f"-march=x86-64-v3, OpenMP {threads} threads), {dt:.1f} s"}
     return res
 
 
@@ -163,9 +172,79 @@ def radau5_leg(M, st, cpu_cells: int, threads: int):
         o.advance(sample, 1e-6, method=2)
         dt = time.perf_counter() - t0
         res["cpu_baseline"] = {"value": cpu_cells / dt, "unit": "cells/s", "cores": threads, "kind": "port",
-                               "sample": f"the first {cpu_cells} cells of the same batch, CPU oracle RADAU5 (gcc -O2, "
-                                         f"OpenMP {threads} threads), {dt:.1f} s"}
+                               "sample": f"the first {cpu_cells} cells of the same batch, CPU oracle RADAU5 (gcc -O3 "
+                                         f"-march=x86-64-v3, OpenMP {threads} threads), {dt:.1f} s"}
     return res
+
+
+def shock_tube_leg(threads: int):
+    """The CPU-runnable configuration: the 1D shock tube, 1000 cells, minmod + HLLC, SSP-RK3 at CFL 0.5, 200 steps,
+    on the device (fused parity path, the whole run with its launches) and on the CPU oracle (1 thread and all);
+    the device's final state must be the oracle's bit for bit.  Launch-bound on the device: 2e5 cell-updates."""
+    import numpy as np
+    import mosegpu
+    from mosegpu import cases
+    case = cases.sod(1000, 200)
+    ctx = mosegpu.Context(case.tables)
+    dc = mosegpu.DeviceCase(ctx, case)
+    dc.upload(case.P)
+    dc.tune(mosegpu.PATH_FUSED, (0, 0, 0))
+    dc.step(0)
+    t0 = time.perf_counter()
+    dc.step(200)
+    el = time.perf_counter() - t0
+    Pg = dc.download()
+    dc.destroy()
+    ctx.close()
+    res = {"workload": "shock tube 1000 cells, (1,0,1)|(0.125,0,0.1), minmod + HLLC, SSP-RK3 CFL 0.5, 200 steps",
+           "cells": 1000, "steps": 200, "gpu_s": el, "gpu_cell_updates_per_s": 1000 * 200 / el}
+    if threads > 0:
+        sys.path.insert(0, str(REPO / "tests"))
+        from oracle_bind import OracleCase
+        for nt in sorted({1, threads}):
+            P = case.P.copy()
+            o = OracleCase(case, nthreads=nt)
+            t0 = time.perf_counter()
+            o.step(P, 200)
+            dt = time.perf_counter() - t0
+            res[f"cpu_{nt}_threads_s"] = dt
+            res["same_state_as_oracle"] = bool(np.array_equal(case.layout.interior(P, 0),
+                                                              case.layout.interior(Pg, 0)))
+    print(f"bench: shock tube 1000 x 200 steps: device {el * 1e3:.1f} ms" +
+          "".join(f", CPU {k.split('_')[1]} thread(s) {v * 1e3:.1f} ms" for k, v in res.items()
+                  if k.startswith("cpu_")), file=sys.stderr, flush=True)
+    return res
+
+
+def chem_valu_per_attempt() -> float | None:
+    """VALU wavefront instructions per RODAS step attempt of k_chem, from the profiled counters
+    (profiles/*_traffic.txt key k_chem_valu_per_attempt), or None."""
+    for f in sorted((REPO / "profiles").glob("*_traffic.txt")):
+        for line in f.read_text().splitlines():
+            w = line.split()
+            if len(w) >= 2 and w[0] == "k_chem_valu_per_attempt":
+                return float(w[1])
+    return None
+
+
+def fast_vs_parity(dc, case, tile, steps: int):
+    """The performance path against the parity path (the oracle's bits) on the benchmark's own case: both from the
+    initial state for `steps` steps, the relative L-infinity of every primitive variable over the whole box."""
+    import numpy as np
+    import mosegpu
+    out = {}
+    for name, path in (("fast", mosegpu.PATH_FAST), ("parity", mosegpu.PATH_FUSED)):
+        dc.upload(case.P)
+        dc.tune(path, tile, timing=False)
+        dc.step(steps)
+        out[name] = case.layout.interior(dc.download(), 0).reshape(case.nprim, -1)
+    a, b = out["fast"], out["parity"]
+    ns = case.scheme.nsc
+    scale = np.abs(b).max(axis=1)
+    scale[ns:ns + 3] = np.abs(b[ns:ns + 3]).max()        # the velocity as a vector: one scale for its components
+    err = float((np.abs(a - b).max(axis=1) / scale).max())
+    print(f"bench: fast path vs parity path after {steps} steps: rel Linf {err:.3e}", file=sys.stderr, flush=True)
+    return {"steps": steps, "rel_linf": err, "bar": 1e-12, "within_bar": err <= 1e-12}
 
 
 def lattice_leg(steps: int, warmup: int = 2):
@@ -237,7 +316,9 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--path", choices=["fused", "plain"], default="fused")
+    ap.add_argument("--path", choices=["fused", "plain", "fast"], default="fused",
+                    help="fused: the parity kernels (the oracle's bits); fast: the performance formulation (within "
+                         "1e-12 of them, checked here on the benchmark's own run); plain: the per-block launches")
     ap.add_argument("--tile", default="auto", help="TIxTJxTK or auto")
     ap.add_argument("--cpu-n", type=int, default=64)
     ap.add_argument("--cpu-steps", type=int, default=96)
@@ -245,12 +326,13 @@ def main():
     ap.add_argument("--probe-bw", action="store_true", help="only run the stream-copy probe (1 GiB buffers)")
     ap.add_argument("--serve", type=int, default=16, help="requests of the serving leg (0: skip)")
     ap.add_argument("--lattice", type=int, default=20, help="timed steps of the multi-block lattice leg (0: skip)")
-    ap.add_argument("--chem", type=int, default=262144, help="cells of the chemistry leg (0: skip)")
+    ap.add_argument("--chem", type=int, default=2000000, help="cells of the chemistry leg (0: skip)")
     ap.add_argument("--chem-cpu", type=int, default=2048, help="cells of the chemistry CPU baseline sample")
     ap.add_argument("--chem-radau", type=int, default=65536, help="cells of the RADAU5 chemistry leg (0: skip)")
     ap.add_argument("--chem-radau-cpu", type=int, default=1024, help="cells of the RADAU5 CPU baseline sample")
-    ap.add_argument("--serve-plain", action="store_true", help="also run the serving requests one by one on the "
+    ap.add_argument("--no-serve-plain", action="store_true", help="skip the serving requests one by one on the "
                     "plain per-block path (the comparison path)")
+    ap.add_argument("--c1", type=int, default=1, help="the shock-tube leg (1000 cells, 200 steps; 0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this benchmark runs on one GPU; --gpus {args.gpus} is not supported (the multi-GPU sharding "
@@ -276,13 +358,14 @@ def main():
         tile = (0, 0, 0)
     else:
         tile = tuple(int(x) for x in args.tile.split("x"))
-    path = mosegpu.PATH_FUSED if args.path == "fused" else mosegpu.PATH_PLAIN
+    path = {"fused": mosegpu.PATH_FUSED, "plain": mosegpu.PATH_PLAIN, "fast": mosegpu.PATH_FAST}[args.path]
+    fused_like = path in (mosegpu.PATH_FUSED, mosegpu.PATH_FAST)
     chosen = dc.tune(path, tile, timing=False)
     t_setup = time.perf_counter() - t_setup
     print(f"bench: case built and tuned in {t_setup:.1f} s, tile {chosen}", file=sys.stderr, flush=True)
     if args.warmup > 0:
         dc.step(args.warmup)
-    dc.tune(path, chosen if path == mosegpu.PATH_FUSED else (0, 0, 0), timing=True)
+    dc.tune(path, chosen if fused_like else (0, 0, 0), timing=True)
     dc.step(0)                          # drains the stream
     t0 = time.perf_counter()
     rep = dc.step(args.steps)           # returns after the device has finished (the report is read back)
@@ -293,14 +376,15 @@ def main():
     value = cells * args.steps / elapsed
     B = algorithmic_bytes_per_cell_stage(case.nprim, True, False)
     achieved = (B * cells) / (ms_avg * 1e-3) / 1e9 if ms_avg > 0 else None
-    traffic, valu = profiled_counts(args.n, case.nprim, True, chosen) if path == mosegpu.PATH_FUSED else (None, None)
+    traffic, valu = profiled_counts(args.n, case.nprim, True, chosen, args.path) if fused_like else (None, None)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
                 "traffic": traffic,
                 # the share of the launch that FP64 VALU issue alone would take (SQ_INSTS_VALU from the profile)
                 "valu_issue_frac": (valu / VALU_ISSUE_PER_S) / (ms_avg * 1e-3) if valu and ms_avg > 0 else None,
-                "kernel": "k_stage (fused residual + RK stage update)" if path == mosegpu.PATH_FUSED
-                          else "face kernels + cell update (one stage)",
+                "kernel": ("k_stage (fused residual + RK stage update)" + (", performance formulation"
+                                                                              if args.path == "fast" else ""))
+                          if fused_like else "face kernels + cell update (one stage)",
                 "kernel_ms_avg": ms_avg, "launches": nlaunch, "bytes_per_cell_stage": B}
     out = {
         "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
@@ -312,12 +396,16 @@ def main():
         "roofline": roofline,
         "sim_time": rep.time, "dt_last": rep.dt, "setup_s": t_setup, "device": ctx.info(),
     }
+    if args.path == "fast":
+        out["parity_check"] = fast_vs_parity(dc, case, chosen, min(args.steps, 20))
     dc.destroy()
     ctx.close()
     if args.lattice > 0:
         out["lattice"] = lattice_leg(args.lattice)
     if args.serve > 0:
-        out["serving"] = serving_leg(args.serve, args.serve_plain)
+        out["serving"] = serving_leg(args.serve, not args.no_serve_plain)
+    if args.c1 > 0:
+        out["shock_tube"] = shock_tube_leg(0 if args.no_cpu else min(16, os.cpu_count() or 1))
     if args.chem > 0:
         out["chemistry"] = chemistry_leg(args.chem, 0 if args.no_cpu else args.chem_cpu, min(16, os.cpu_count() or 1),
                                          args.chem_radau, 0 if args.no_cpu else args.chem_radau_cpu)

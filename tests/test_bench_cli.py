@@ -38,3 +38,7 @@ def test_algorithmic_bytes():
     assert bench.algorithmic_bytes_per_cell_stage(5, True, False) == 320
     assert bench.algorithmic_bytes_per_cell_stage(5, False, False) == 8 * (15 + 16)
     assert bench.algorithmic_bytes_per_cell_stage(6, True, True) == 8 * (18 + 26)
+
+
+def test_chemistry_instruction_count_from_the_profile():
+    assert bench.chem_valu_per_attempt() == 3921.0
