@@ -666,6 +666,8 @@ static int stage_r0(const Case& c, int TI, int TJ, int TK) {
   return (r0 + 31) & ~31;
 }
 static size_t stage_lds(const Case& c, int TI, int TJ, int TK) {
+  if (c.path == MOSE_PATH_FAST)
+    return fast_lds_doubles(c.sc.nsc, c.sc.nrans, c.nprim, c.sc.viscous != 0, TI, TJ, TK) * sizeof(double);
   return ((size_t)stage_r0(c, TI, TJ, TK) + (size_t)tile_slots(TI, TJ, TK) * c.nprim) * sizeof(double);
 }
 static size_t stage_lds_(int nprim, bool visc, int TI, int TJ, int TK) {
@@ -728,7 +730,7 @@ static void launch_stage(Case& c, const CaseDev& D, hipStream_t s, double rkc, b
   size_t lds = stage_lds(c, c.TI, c.TJ, c.TK);
   unsigned grid = (unsigned)(((c.ntiles + 7) / 8) * 8);
   if (c.path == MOSE_PATH_FAST) launch_stage_fast<NS, NR>(D, A, c.sc.riemann, c.sc.viscous != 0, grid, lds, s);
-  else launch_stage_tiles<NS, NR, false>(D, A, c.sc.riemann, c.sc.viscous != 0, grid, lds, s);
+  else launch_stage_tiles<NS, NR>(D, A, c.sc.riemann, c.sc.viscous != 0, grid, lds, s);
 }
 
 // records the step's dt in the history at the device-side position (the same kernel serves eager and graph steps)

@@ -17,7 +17,6 @@
 // one launch per stage; the workgroup -> tile map deals a contiguous run of tiles to each XCD (k_stage below).
 #pragma once
 
-#include "mose_fast.hpp"
 #include "mose_kernels.hpp"
 
 namespace mose {
@@ -73,7 +72,7 @@ __device__ __forceinline__ TileGeo tile_geo(const StageArgs& A) {
 // variable at a time is assembled in computational space, taken to physical space and reduced to its flux terms, so
 // that only the three velocity gradients are held at once.  Every value is the same expression as in
 // diff_flux_interior (diff_physics.inc:59-161), evaluated in the same order.
-template <int NS, int NR, int D, bool FAST>
+template <int NS, int NR, int D>
 __device__ __forceinline__ void visc_face(const CaseDev& C, const mose_blk& B, const Tile& tl, const TileGeo& G_,
                                           const double* cache, int i, int j, int k, int a, int b, int c, double* res) {
   using V = Vars<NS, NR>;
@@ -96,7 +95,7 @@ __device__ __forceinline__ void visc_face(const CaseDev& C, const mose_blk& B, c
   double Pf[NP];
 #pragma unroll
   for (int s = 0; s < NP; ++s) Pf[s] = 0.5 * (Pc[s * NCa + k1] + Pc[s * NCa + k2]);
-  const FaceThermo ft = FAST ? face_thermo_fast<NS>(T, Pf, Pf[V::np], C.vc.prl) : face_thermo<NS>(T, Pf, Pf[V::np], C.vc.prl);
+  const FaceThermo ft = face_thermo<NS>(T, Pf, Pf[V::np], C.vc.prl);
   const double* fm = D == 0 ? C.fm1 : (D == 1 ? C.fm2 : C.fm3);
   const int64_t NFg = D == 0 ? C.NF1 : (D == 1 ? C.NF2 : C.NF3);
   const int64_t fg = D == 0 ? B.f1off + i + (int64_t)(B.n1 + 1) * ((j - 1) + (int64_t)B.n2 * (k - 1))
@@ -179,9 +178,8 @@ __device__ __forceinline__ void visc_face(const CaseDev& C, const mose_blk& B, c
   (void)tl;
 }
 
-// One tile of one RK stage (the phases above).  Every thread of the workgroup calls it with the same tile.  FAST: the
-// performance formulation of the face fluxes (mose_fast.hpp), instantiated only by the contracted translation unit.
-template <int NS, int NR, bool VISC, int RS, int STI, int STJ, int STK, bool FAST>
+// One tile of one RK stage (the phases above).  Every thread of the workgroup calls it with the same tile.
+template <int NS, int NR, bool VISC, int RS, int STI, int STJ, int STK>
 __device__ __forceinline__ void stage_tile(const CaseDev& C, const StageArgs& A, int tidx) {
   using V = Vars<NS, NR>;
   constexpr int NP = V::nprim;
@@ -219,10 +217,7 @@ __device__ __forceinline__ void stage_tile(const CaseDev& C, const StageArgs& A,
     double idm = 1.0 / (0.5 * (dm + d0)), idp = 1.0 / (0.5 * (d0 + dp)), hd = 0.5 * d0;
 #pragma unroll
     for (int s = 0; s < NP; ++s) {
-      if constexpr (FAST)
-        Hs[s * G.nhs + q] = half_slope_fast(C.lim, P[s * NC + c0 - st], P[s * NC + c0], P[s * NC + c0 + st], idm, idp, hd);
-      else
-        Hs[s * G.nhs + q] = half_slope(C.lim, P[s * NC + c0 - st], P[s * NC + c0], P[s * NC + c0 + st], idm, idp, hd);
+      Hs[s * G.nhs + q] = half_slope(C.lim, P[s * NC + c0 - st], P[s * NC + c0], P[s * NC + c0 + st], idm, idp, hd);
     }
   }
   __syncthreads();
@@ -255,10 +250,7 @@ __device__ __forceinline__ void stage_tile(const CaseDev& C, const StageArgs& A,
     int64_t fg = d == 0 ? B.f1off + i + (int64_t)(B.n1 + 1) * ((j - 1) + (int64_t)B.n2 * (k - 1))
                : d == 1 ? B.f2off + (i - 1) + (int64_t)B.n1 * (j + (int64_t)(B.n2 + 1) * (k - 1))
                         : B.f3off + (i - 1) + (int64_t)B.n1 * ((j - 1) + (int64_t)B.n2 * k);
-    if constexpr (FAST)
-      conv_face_fast<NS, NR, RS>(T, C.rs, C.mco, L, R, fm[fg], fm[NFg + fg], fm[2 * NFg + fg], fm[3 * NFg + fg], res);
-    else
-      conv_face_lr<NS, NR, RS>(T, C.rs, C.mco, L, R, fm[fg], fm[NFg + fg], fm[2 * NFg + fg], fm[3 * NFg + fg], res);
+    conv_face_lr<NS, NR, RS>(T, C.rs, C.mco, L, R, fm[fg], fm[NFg + fg], fm[2 * NFg + fg], fm[3 * NFg + fg], res);
 #pragma unroll
     for (int s = 0; s < NP; ++s) Fb[s * NF + f] = res[s];
   }
@@ -321,9 +313,9 @@ __device__ __forceinline__ void stage_tile(const CaseDev& C, const StageArgs& A,
       if (!ok) continue;
       // local cache coordinates of the left cell c1 (tile-local + 1)
       double res[NP];
-      if (d == 0) visc_face<NS, NR, 0, FAST>(C, B, tl, G, cache, i, j, k, a, b, c, res);
-      else if (d == 1) visc_face<NS, NR, 1, FAST>(C, B, tl, G, cache, i, j, k, a, b, c, res);
-      else visc_face<NS, NR, 2, FAST>(C, B, tl, G, cache, i, j, k, a, b, c, res);
+      if (d == 0) visc_face<NS, NR, 0>(C, B, tl, G, cache, i, j, k, a, b, c, res);
+      else if (d == 1) visc_face<NS, NR, 1>(C, B, tl, G, cache, i, j, k, a, b, c, res);
+      else visc_face<NS, NR, 2>(C, B, tl, G, cache, i, j, k, a, b, c, res);
 #pragma unroll
       for (int s = 0; s < NP; ++s) Fb[s * NF + f] = res[s];
     }
@@ -419,54 +411,63 @@ __device__ __forceinline__ int xcd_tile(int wg, int n) {
 // Without RANS the 8 x 8 x 4 kernels take at most 128 VGPRs (127 for the viscous LMRoe one), so two 512-thread
 // workgroups share a CU (4 waves per SIMD).  An amdgpu_waves_per_eu(4) attribute gives the same count but a schedule
 // that hides less latency: 6.41 against 5.96 ms per stage on the 256^3 Taylor-Green case.
-// The performance path's one-species 8 x 8 x 4 kernel asks for 4 waves per SIMD (at most 128 VGPRs; two of its
-// 76 KB workgroups share a CU): contracted, it would take 129 and run at 3.
-template <int NS, int NR, bool VISC, int RS, int BS, int SH, bool FAST = false>
-__global__ void __launch_bounds__(BS, (FAST && SH == 1 && NS == 1 && NR == 0) ? 4 : 1) k_stage(CaseDev C, StageArgs A) {
+template <int NS, int NR, bool VISC, int RS, int BS, int SH>
+__global__ void __launch_bounds__(BS) k_stage(CaseDev C, StageArgs A) {
   const int t = xcd_tile(blockIdx.x, A.ntiles);
   if (t < 0) return;
   constexpr int STI = SH ? 8 : 0, STJ = SH ? 8 : 0, STK = SH == 1 ? 4 : (SH == 2 ? 8 : 0);
-  stage_tile<NS, NR, VISC, RS, STI, STJ, STK, FAST>(C, A, t);
+  stage_tile<NS, NR, VISC, RS, STI, STJ, STK>(C, A, t);
 }
 
 // The launch of one stage over a case's tiles: the kernel of the case's solver, viscosity and tile shape (the
 // shapes 8 x 8 x 4 and 8 x 8 x 8 fixed at compile time, the others read at run time; 128-cell tiles run 256 threads,
-// two per cell, the others 512).  FAST = true is instantiated by the contracted translation unit only.
-template <int NS, int NR, int BS, int SH, bool FAST>
+// two per cell, the others 512).
+template <int NS, int NR, int BS, int SH>
 void launch_stage_shape(const CaseDev& D, const StageArgs& A, int rs, bool visc, unsigned grid, size_t lds,
                         hipStream_t s) {
   if (visc) {
-    if (rs == MOSE_RS_LMROE) k_stage<NS, NR, true, MOSE_RS_LMROE, BS, SH, FAST><<<grid, BS, lds, s>>>(D, A);
-    else if (rs == MOSE_RS_AUSMP) k_stage<NS, NR, true, MOSE_RS_AUSMP, BS, SH, FAST><<<grid, BS, lds, s>>>(D, A);
-    else if (rs == MOSE_RS_HLLE) k_stage<NS, NR, true, MOSE_RS_HLLE, BS, SH, FAST><<<grid, BS, lds, s>>>(D, A);
-    else k_stage<NS, NR, true, MOSE_RS_HLLC, BS, SH, FAST><<<grid, BS, lds, s>>>(D, A);
+    if (rs == MOSE_RS_LMROE) k_stage<NS, NR, true, MOSE_RS_LMROE, BS, SH><<<grid, BS, lds, s>>>(D, A);
+    else if (rs == MOSE_RS_AUSMP) k_stage<NS, NR, true, MOSE_RS_AUSMP, BS, SH><<<grid, BS, lds, s>>>(D, A);
+    else if (rs == MOSE_RS_HLLE) k_stage<NS, NR, true, MOSE_RS_HLLE, BS, SH><<<grid, BS, lds, s>>>(D, A);
+    else k_stage<NS, NR, true, MOSE_RS_HLLC, BS, SH><<<grid, BS, lds, s>>>(D, A);
   } else {
-    if (rs == MOSE_RS_LMROE) k_stage<NS, NR, false, MOSE_RS_LMROE, BS, SH, FAST><<<grid, BS, lds, s>>>(D, A);
-    else if (rs == MOSE_RS_AUSMP) k_stage<NS, NR, false, MOSE_RS_AUSMP, BS, SH, FAST><<<grid, BS, lds, s>>>(D, A);
-    else if (rs == MOSE_RS_HLLE) k_stage<NS, NR, false, MOSE_RS_HLLE, BS, SH, FAST><<<grid, BS, lds, s>>>(D, A);
-    else k_stage<NS, NR, false, MOSE_RS_HLLC, BS, SH, FAST><<<grid, BS, lds, s>>>(D, A);
+    if (rs == MOSE_RS_LMROE) k_stage<NS, NR, false, MOSE_RS_LMROE, BS, SH><<<grid, BS, lds, s>>>(D, A);
+    else if (rs == MOSE_RS_AUSMP) k_stage<NS, NR, false, MOSE_RS_AUSMP, BS, SH><<<grid, BS, lds, s>>>(D, A);
+    else if (rs == MOSE_RS_HLLE) k_stage<NS, NR, false, MOSE_RS_HLLE, BS, SH><<<grid, BS, lds, s>>>(D, A);
+    else k_stage<NS, NR, false, MOSE_RS_HLLC, BS, SH><<<grid, BS, lds, s>>>(D, A);
   }
 }
 
-template <int NS, int NR, bool FAST>
+template <int NS, int NR>
 void launch_stage_tiles(const CaseDev& D, const StageArgs& A, int rs, bool visc, unsigned grid, size_t lds,
                         hipStream_t s) {
   const int TI = A.TI, TJ = A.TJ, TK = A.TK;
   if constexpr (NS >= 9) {
     // nine species: the 8 x 8 x 4 and 8 x 8 x 8 tiles do not fit the LDS, every shape is read at run time
-    if (TI * TJ * TK == 128) launch_stage_shape<NS, NR, 256, 0, FAST>(D, A, rs, visc, grid, lds, s);
-    else launch_stage_shape<NS, NR, 512, 0, FAST>(D, A, rs, visc, grid, lds, s);
+    if (TI * TJ * TK == 128) launch_stage_shape<NS, NR, 256, 0>(D, A, rs, visc, grid, lds, s);
+    else launch_stage_shape<NS, NR, 512, 0>(D, A, rs, visc, grid, lds, s);
     return;
   }
-  if (TI * TJ * TK == 128) launch_stage_shape<NS, NR, 256, 0, FAST>(D, A, rs, visc, grid, lds, s);
-  else if (TI == 8 && TJ == 8 && TK == 4) launch_stage_shape<NS, NR, 512, 1, FAST>(D, A, rs, visc, grid, lds, s);
-  else if (TI == 8 && TJ == 8 && TK == 8) launch_stage_shape<NS, NR, 512, 2, FAST>(D, A, rs, visc, grid, lds, s);
-  else launch_stage_shape<NS, NR, 512, 0, FAST>(D, A, rs, visc, grid, lds, s);
+  if (TI * TJ * TK == 128) launch_stage_shape<NS, NR, 256, 0>(D, A, rs, visc, grid, lds, s);
+  else if (TI == 8 && TJ == 8 && TK == 4) launch_stage_shape<NS, NR, 512, 1>(D, A, rs, visc, grid, lds, s);
+  else if (TI == 8 && TJ == 8 && TK == 8) launch_stage_shape<NS, NR, 512, 2>(D, A, rs, visc, grid, lds, s);
+  else launch_stage_shape<NS, NR, 512, 0>(D, A, rs, visc, grid, lds, s);
 }
 
-// defined in the contracted translation unit (mose_stage_fast.hip) for every instantiated (NS, NR)
+// The performance path (mose_stage_fast.hpp), defined in its own contracted translation unit (mose_stage_fast.hip)
+// for every instantiated (NS, NR); its LDS per tile in doubles: the primitives of the tile's one-cell halo box and
+// of six slabs one cell further out, the viscous cell scalars (T alone for one species without RANS, else rho,
+// 1/rho, T), the face slots.
 template <int NS, int NR>
 void launch_stage_fast(const CaseDev& D, const StageArgs& A, int rs, bool visc, unsigned grid, size_t lds,
                        hipStream_t s);
+
+inline size_t fast_lds_doubles(int ns, int nr, int nprim, bool visc, int TI, int TJ, int TK) {
+  const size_t nbox = (size_t)(TI + 2) * (TJ + 2) * (TK + 2);
+  const size_t next = nbox + 2 * ((size_t)TJ * TK + (size_t)TI * TK + (size_t)TI * TJ);
+  const size_t nslot = (size_t)(TI + 1) * TJ * TK + (size_t)TI * (TJ + 1) * TK + (size_t)TI * TJ * (TK + 1);
+  const size_t ncc = visc ? ((ns == 1 && nr == 0) ? 1 : 3) : 0;
+  return (size_t)nprim * next + ncc * nbox + (size_t)nprim * nslot;
+}
 
 }  // namespace mose

@@ -4,8 +4,8 @@ form of the Roe flux) against the oracle on every family of parity cases of the 
 The parity path equals the oracle bit for bit; this path differs from it by rounding only, and the bar is the
 residual path's: a relative L-infinity <= 1e-12 per primitive variable after N steps (<= 1e-8 with chemistry, whose
 adaptive step counts turn a rounding into a different step sequence), the velocity in units of |u| + sqrt(p / rho)
-(fast_error).
-Each comparison reports how many values differ bitwise, and the time, the sum of the steps' dt, matches to 1e-13.
+(fast_error).  Noh87 is held to 1e-9: a change of rounding alone moves it by ~2e-11 (test_oracle_rounding.py).
+Each comparison reports how many values differ bitwise; the time, the sum of the steps' dt, is held to the same bar.
 """
 import os
 
@@ -54,9 +54,11 @@ def check(case, nsteps, tile=(0, 0, 0), what="", tol=TOL):
     Pg, rep, chosen = run_device(case, nsteps, FAST, tile)
     Po = case.P.copy()
     dts = OracleCase(case, nthreads=16).step(Po, nsteps)
-    assert abs(rep.time - dts.sum()) <= 1e-13 * dts.sum()
     err = fast_error(case, Pg, Po, f"fast path: {what} tile {chosen} x {nsteps}")
+    dtime = abs(rep.time - dts.sum()) / dts.sum()
+    print(f"  time (the sum of the steps' dt): relative difference {dtime:.3e}", flush=True)
     assert err <= tol
+    assert dtime <= tol
 
 
 def test_fast_sod(gpu_ready):
@@ -113,8 +115,10 @@ def test_fast_reference_cases(gpu_ready):
     check(caseio.load_case(DIFF1, 100), 100, (0, 0, 0), "diff1 (reference inputs)")
     check(caseio.load_case(os.path.join(GOLD, "isentropic-vortex"), 20), 20, (0, 0, 0), "isentropic vortex")
     check(caseio.load_case(os.path.join(GOLD, "shu-osher"), 330), 330, (0, 0, 0), "Shu-Osher (reference inputs)")
-    for name, n in (("ein91", 100), ("noh87", 100)):
-        check(caseio.load_case(os.path.join(GOLD, name), n), n, (0, 0, 0), f"{name} (reference inputs, HLLE)")
+    check(caseio.load_case(os.path.join(GOLD, "ein91"), 100), 100, (0, 0, 0), "ein91 (reference inputs, HLLE)")
+    # Noh87 moves by ~2e-11 (time ~2e-12) under a change of rounding alone: the oracle compiled with contraction
+    # against the oracle (test_oracle_rounding.py); its bar is 1e-9
+    check(caseio.load_case(os.path.join(GOLD, "noh87"), 100), 100, (0, 0, 0), "noh87 (reference inputs, HLLE)", 1e-9)
 
 
 def test_fast_riemann2d(gpu_ready):

@@ -58,7 +58,8 @@ def kernels(tmp_path_factory):
 
 
 def _stage(kernels, nr, visc, rs, sh, fast=False):
-    name = (f"_ZN4mose7k_stageILi1ELi{nr}ELb{int(visc)}ELi{rs}ELi512ELi{sh}ELb{int(fast)}EEEvNS_7CaseDevENS_9StageArgsE")
+    kern = "12k_stage_fast" if fast else "7k_stage"
+    name = f"_ZN4mose{kern}ILi1ELi{nr}ELb{int(visc)}ELi{rs}ELi512ELi{sh}EEEvNS_7CaseDevENS_9StageArgsE"
     assert name in kernels, name
     return kernels[name]
 
