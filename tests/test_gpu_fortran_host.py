@@ -108,3 +108,24 @@ def test_fortran_case_driver_reacting(gpu_ready, tmp_path):
     assert o.chem_failed() == 0
     assert steps == 3 and abs(t - dts.sum()) <= 1e-13 * dts.sum()
     assert compare(case, P, Po, f"fortran case driver, reacting box 12x10x8 x 3 Strang steps ({out})") <= 1e-8
+
+
+@pytest.mark.parametrize("name,nsteps,path", [("isentropic-vortex", 20, 0), ("isentropic-vortex", 20, 1),
+                                              ("shu-osher", 330, 0), ("diff1", 100, 0), ("noh87", 100, 0)])
+def test_fortran_case_dir_runs_the_reference_case(gpu_ready, tmp_path, name, nsteps, path):
+    """mose_case_dir reads the reference's own case directory (input.ini and INPUT/) in Fortran, builds the geometry,
+    steps the case through MOSE_GPU_Explicit_Step and writes the state: the oracle's state and time on the same
+    inputs (read by the Python loader)."""
+    import os
+    from fortran_case_dir import GOLD, expand, read_result, run
+    from mosegpu import caseio
+    from test_gpu_walls_sa import compare
+    d = expand(name, tmp_path)
+    r = run(d, nsteps, str(tmp_path / "out.bin"), path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    case = caseio.load_case(os.path.join(GOLD, name), nsteps)
+    P, t, steps = read_result(str(tmp_path / "out.bin"), case.layout.NC, case.nprim)
+    Po = case.P.copy()
+    dts = OracleCase(case, nthreads=16).step(Po, nsteps)
+    assert steps == nsteps and abs(t - dts.sum()) <= 1e-13 * dts.sum()
+    assert compare(case, P, Po, f"fortran case directory {name}, path {path} x {nsteps} ({r.stdout.strip()})") <= 1e-12
