@@ -250,17 +250,20 @@ def fast_vs_parity(dc, case, tile, steps: int):
 def lattice_leg(steps: int, warmup: int = 2):
     """The 4 x 4 x 4 lattice of blocks with seeded uneven widths in [24, 96] per axis (cases.channel_lattice, seed 7),
     one-to-one interfaces, an adiabatic no-slip wall at y = 0, far field on the other outer faces, M 0.3, Re 1e6,
-    Spalart-Allmaras, LMRoe (Mco = 1) + van Albada, SSP-RK3: the fused stage kernel over the tiles of every block
-    against the plain per-block launch path, on the same start state; the two final state hashes must be equal."""
+    Spalart-Allmaras, LMRoe (Mco = 1) + van Albada, SSP-RK3: the stage kernel over the tiles of every block (the
+    performance path, and the parity path) against the plain per-block launch path, on the same start state.  The
+    parity path's final state hash must equal the plain path's; the performance path's final state is compared with
+    the parity path's (relative L-infinity per variable)."""
+    import numpy as np
     import mosegpu
-    from mosegpu import PATH_FUSED, PATH_PLAIN, cases
+    from mosegpu import PATH_FAST, PATH_FUSED, PATH_PLAIN, cases
     case = cases.channel_lattice(riemann="LMRoe", nsteps=steps)
     cells = case.interior_cells()
     out = {"blocks": len(case.layout.dims), "cells": cells, "steps": steps,
            "workload": "4x4x4 block lattice, widths U{24..96} (seed 7), wall + far field, SA, M 0.3, Re 1e6, "
                        "LMRoe + van Albada, SSP-RK3"}
-    hashes = {}
-    for name, path in (("fused", PATH_FUSED), ("plain", PATH_PLAIN)):
+    hashes, states = {}, {}
+    for name, path in (("fast", PATH_FAST), ("fused", PATH_FUSED), ("plain", PATH_PLAIN)):
         ctx = mosegpu.Context(case.tables)
         dc = mosegpu.DeviceCase(ctx, case)
         dc.upload(case.P)
@@ -271,29 +274,48 @@ def lattice_leg(steps: int, warmup: int = 2):
         dc.step(steps)
         el = time.perf_counter() - t0
         hashes[name] = dc.state_hash()
+        if name != "plain":
+            states[name] = np.hstack([case.layout.interior(dc.download(), b).reshape(case.nprim, -1)
+                                      for b in range(case.layout.nb)])
         out[name] = {"cell_updates_per_s": cells * steps / el, "ms_per_step": 1e3 * el / steps,
-                     "tile": list(chosen) if path == PATH_FUSED else None}
+                     "tile": list(chosen) if path != PATH_PLAIN else None}
         dc.destroy()
         ctx.close()
     out["fused_over_plain"] = out["fused"]["cell_updates_per_s"] / out["plain"]["cell_updates_per_s"]
+    out["fast_over_plain"] = out["fast"]["cell_updates_per_s"] / out["plain"]["cell_updates_per_s"]
     out["same_state"] = hashes["fused"] == hashes["plain"]
-    print(f"bench: lattice {cells} cells in 64 blocks: fused {out['fused']['ms_per_step']:.2f} ms/step, plain "
-          f"{out['plain']['ms_per_step']:.2f} ms/step, same state {out['same_state']}", file=sys.stderr, flush=True)
+    a, b = states["fast"], states["fused"]
+    ns = case.scheme.nsc
+    scale = np.abs(b).max(axis=1)
+    scale[ns:ns + 3] = float((np.sqrt((b[ns:ns + 3] ** 2).sum(axis=0)) + np.sqrt(b[ns + 3] / b[:ns].sum(axis=0))).max())
+    scale[scale == 0] = 1.0
+    out["fast_vs_parity_rel_linf"] = float((np.abs(a - b).max(axis=1) / scale).max())
+    print(f"bench: lattice {cells} cells in 64 blocks: fast {out['fast']['ms_per_step']:.2f}, fused "
+          f"{out['fused']['ms_per_step']:.2f}, plain {out['plain']['ms_per_step']:.2f} ms/step; fused == plain "
+          f"{out['same_state']}, fast vs parity {out['fast_vs_parity_rel_linf']:.3e}", file=sys.stderr, flush=True)
     return out
 
 
 def serving_leg(count: int, plain: bool):
     """A Poisson stream (20 cases/s) of 2D four-quadrant Riemann (n log-uniform in [64, 1024]) and 3D TGV (n
-    log-uniform in [48, 192]) cases of 50-500 steps, served with continuous batching; p50/p99 time-to-solution."""
-    from mosegpu import PATH_PLAIN, serve
+    log-uniform in [48, 192]) cases of 50-500 steps, served with continuous batching on the performance path, and
+    again on the parity path; p50/p99 time-to-solution.  The comparison path runs the same requests one by one on the
+    plain per-block launch path."""
+    from mosegpu import PATH_FAST, PATH_FUSED, PATH_PLAIN, serve
     reqs = serve.request_stream(count, rate=20.0, seed=2024)
-    st = serve.serve(reqs, quantum=10)
-    res = {"requests": count, "rate_per_s": 20.0, "cases_per_s": st.cases_per_s, "p50_s": st.p50, "p99_s": st.p99,
-           "wall_s": st.wall, "max_resident": st.max_resident, "cell_updates_per_s": st.cell_updates / st.wall,
+    st = serve.serve(reqs, quantum=10, path=PATH_FAST)
+    res = {"requests": count, "rate_per_s": 20.0, "path": "fast", "cases_per_s": st.cases_per_s, "p50_s": st.p50,
+           "p99_s": st.p99, "wall_s": st.wall, "max_resident": st.max_resident,
+           "cell_updates_per_s": st.cell_updates / st.wall,
            "workload": "poisson 20/s: 2D 4-quadrant Riemann n^2 (n in [64,1024]) | 3D TGV n^3 (n in [48,192]), "
                        "50-500 steps"}
-    print(f"bench: served {count} cases in {st.wall:.2f} s, p50 {st.p50:.3f} s, p99 {st.p99:.3f} s", file=sys.stderr,
-          flush=True)
+    print(f"bench: served {count} cases (performance path) in {st.wall:.2f} s, p50 {st.p50:.3f} s, p99 {st.p99:.3f} s",
+          file=sys.stderr, flush=True)
+    reqs_p = serve.request_stream(count, rate=20.0, seed=2024)
+    sp = serve.serve(reqs_p, quantum=10, path=PATH_FUSED)
+    res["parity_path"] = {"cases_per_s": sp.cases_per_s, "p50_s": sp.p50, "p99_s": sp.p99, "wall_s": sp.wall}
+    print(f"bench: served {count} cases (parity path) in {sp.wall:.2f} s, p50 {sp.p50:.3f} s, p99 {sp.p99:.3f} s",
+          file=sys.stderr, flush=True)
     if plain:
         # the comparison path: each request alone, one after another, on the plain per-block launch path; its
         # whole-request time (host preparation included, as in the served time-to-solution) and its stepping time
@@ -304,6 +326,7 @@ def serving_leg(count: int, plain: bool):
             t_req += tr
         res["plain_one_by_one_cases_per_s"] = count / t_req
         res["speedup_vs_plain"] = st.cases_per_s / (count / t_req)
+        res["parity_path"]["speedup_vs_plain"] = sp.cases_per_s / (count / t_req)
         res["plain_device_cell_updates_per_s"] = st.cell_updates / t_dev
         print(f"bench: plain one by one {t_req:.2f} s per stream ({t_dev:.2f} s stepping)", file=sys.stderr,
               flush=True)
@@ -316,9 +339,10 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--path", choices=["fused", "plain", "fast"], default="fused",
-                    help="fused: the parity kernels (the oracle's bits); fast: the performance formulation (within "
-                         "1e-12 of them, checked here on the benchmark's own run); plain: the per-block launches")
+    ap.add_argument("--path", choices=["fused", "plain", "fast"], default="fast",
+                    help="fast: the performance formulation (within 1e-12 of the parity kernels, checked here on the "
+                         "benchmark's own run); fused: the parity kernels (the oracle's bits); plain: the per-block "
+                         "launches")
     ap.add_argument("--tile", default="auto", help="TIxTJxTK or auto")
     ap.add_argument("--cpu-n", type=int, default=64)
     ap.add_argument("--cpu-steps", type=int, default=96)

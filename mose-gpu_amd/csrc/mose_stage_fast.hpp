@@ -273,44 +273,59 @@ __device__ __forceinline__ void stage_tile_fast(const CaseDev& C, const StageArg
   const int64_t si = 1, sj = B.pi, sk = (int64_t)B.pi * B.pj;
 
   // ---------------- phase L: the primitives of the box and the slabs into LDS ----------------
-  for (int q = tid; q < NX; q += BS) {
-    int la, lb, lc;
-    if (q < NB) {
-      la = q % CI - 1; const int r = q / CI; lb = r % CJ - 1; lc = r / CJ - 1;
-    } else {
-      int g = q - NB;
-      if (g < 2 * G.s0) {
-        const int side = g >= G.s0; g -= side * G.s0;
-        la = side ? TI + 1 : -2; lb = g % TJ; lc = g / TJ;
-      } else if ((g -= 2 * G.s0) < 2 * G.s1) {
-        const int side = g >= G.s1; g -= side * G.s1;
-        lb = side ? TJ + 1 : -2; la = g % TI; lc = g / TI;
-      } else {
-        g -= 2 * G.s1;
-        const int side = g >= G.s2; g -= side * G.s2;
-        lc = side ? TK + 1 : -2; la = g % TI; lb = g / TI;
-      }
-    }
-    const int i = tl.i0 + la, j = tl.j0 + lb, k = tl.k0 + lc;
-    if (i < 1 - GC || i > B.n1 + GC || j < 1 - GC || j > B.n2 + GC || k < 1 - GC || k > B.n3 + GC) continue;
-    const int64_t r0 = row(B, i, j, k);
-    double y[NP];
+  // up to LU cells per thread per pass, every load of the pass issued before the first LDS store (LU x NP doubles
+  // in flight per thread: fewer cells per pass for more variables)
+  constexpr int LU = NP <= 5 ? 4 : (NP <= 8 ? 2 : 1);
+  for (int q0 = tid; q0 < NX; q0 += LU * BS) {
+    double y[LU][NP];
+    int qs[LU];
+    bool ok[LU];
 #pragma unroll
-    for (int s = 0; s < NP; ++s) y[s] = P[s * NC + r0];
-#pragma unroll
-    for (int s = 0; s < NP; ++s) Pl[s * NX + q] = y[s];
-    if constexpr (VISC) {
+    for (int u = 0; u < LU; ++u) {
+      const int q = q0 + u * BS;
+      qs[u] = q;
+      int la = 0, lb = 0, lc = 0;
       if (q < NB) {
-        if constexpr (FC::n == 1) {
-          Cc[q] = y[V::np] * rcp_nr(y[0] * T.Ri[0]);
+        la = q % CI - 1; const int r = q / CI; lb = r % CJ - 1; lc = r / CJ - 1;
+      } else {
+        int g = q - NB;
+        if (g < 2 * G.s0) {
+          const int side = g >= G.s0; g -= side * G.s0;
+          la = side ? TI + 1 : -2; lb = g % TJ; lc = g / TJ;
+        } else if ((g -= 2 * G.s0) < 2 * G.s1) {
+          const int side = g >= G.s1; g -= side * G.s1;
+          lb = side ? TJ + 1 : -2; la = g % TI; lc = g / TI;
         } else {
-          const double rho = rho_sum<NS>(y), ir = rcp_nr(rho);
-          double Rg = 0.0;
+          g -= 2 * G.s1;
+          const int side = g >= G.s2; g -= side * G.s2;
+          lc = side ? TK + 1 : -2; la = g % TI; lb = g / TI;
+        }
+      }
+      const int i = tl.i0 + la, j = tl.j0 + lb, k = tl.k0 + lc;
+      ok[u] = q < NX && !(i < 1 - GC || i > B.n1 + GC || j < 1 - GC || j > B.n2 + GC || k < 1 - GC || k > B.n3 + GC);
+      const int64_t r0 = ok[u] ? row(B, i, j, k) : 0;
 #pragma unroll
-          for (int s = 0; s < NS; ++s) Rg = Rg + T.Ri[s] * y[s] * ir;
-          Cc[q] = rho;
-          Cc[NB + q] = ir;
-          Cc[2 * NB + q] = y[V::np] * ir * rcp_nr(Rg);
+      for (int s = 0; s < NP; ++s) y[u][s] = ok[u] ? P[s * NC + r0] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < LU; ++u) {
+      if (!ok[u]) continue;
+      const int q = qs[u];
+#pragma unroll
+      for (int s = 0; s < NP; ++s) Pl[s * NX + q] = y[u][s];
+      if constexpr (VISC) {
+        if (q < NB) {
+          if constexpr (FC::n == 1) {
+            Cc[q] = y[u][V::np] * rcp_nr(y[u][0] * T.Ri[0]);
+          } else {
+            const double rho = rho_sum<NS>(y[u]), ir = rcp_nr(rho);
+            double Rg = 0.0;
+#pragma unroll
+            for (int s = 0; s < NS; ++s) Rg = Rg + T.Ri[s] * y[u][s] * ir;
+            Cc[q] = rho;
+            Cc[NB + q] = ir;
+            Cc[2 * NB + q] = y[u][V::np] * ir * rcp_nr(Rg);
+          }
         }
       }
     }
