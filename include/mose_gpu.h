@@ -213,8 +213,10 @@ int mose_case_state_hash(mose_ctx* ctx, int case_id, uint64_t* h_lo, uint64_t* h
 /* Execution path and tile shape of a case.  path: MOSE_PATH_FUSED (default) runs one fused stage kernel per RK
  * stage over the tiles of every block; MOSE_PATH_PLAIN runs the per-block face kernels + cell update (the
  * per-block launch structure of the reference, kept as the comparison path); MOSE_PATH_FAST is the fused path in
- * its performance formulation: the same discretisation with fused multiply-adds, reciprocals and a shorter form of
- * the Roe (LMRoe) flux, within rounding of the other two (relative 1e-12 on the parity cases), not bit for bit.
+ * its performance formulation: the same discretisation with each tile's primitives staged in LDS once, fused
+ * multiply-adds, Newton-refined reciprocals and a shorter form of the Roe (LMRoe) flux; it differs from the other
+ * two by rounding only (relative 1e-12 on the parity cases, except where a change of rounding alone moves a case
+ * further: Noh87's cold gas moves by ~2e-11 under contraction), not bit for bit.
  * A tile holds 128, 256 or 512 cells
  * (ti * tj * tk); ti = tj = tk = 0 autotunes the tile (each candidate timed on the case itself, the state left
  * unchanged).  timing = 1 records HIP events around

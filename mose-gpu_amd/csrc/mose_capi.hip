@@ -482,7 +482,9 @@ static int ensure_rpart(mose_ctx* ctx, Case& c, int64_t nparts) {
   return MOSE_OK;
 }
 
+// The case's step graph is destroyed only once its stream has finished every replay queued on it.
 static void drop_graph(Case& c) {
+  if ((c.gexec || c.graph) && c.stream) (void)hipStreamSynchronize(c.stream);
   if (c.gexec) (void)hipGraphExecDestroy(c.gexec);
   if (c.graph) (void)hipGraphDestroy(c.graph);
   c.gexec = nullptr;
@@ -895,6 +897,7 @@ extern "C" int mose_case_tune(mose_ctx* ctx, int id, const mose_tuning* t, mose_
   if (!c || !t) return MOSE_ERR_ARG;
   if (t->path != MOSE_PATH_FUSED && t->path != MOSE_PATH_PLAIN && t->path != MOSE_PATH_FAST)
     return fail(ctx, MOSE_ERR_ARG, "unknown path");
+  if (c->launched || c->pending) return fail(ctx, MOSE_ERR_ARG, "mose_case_tune: a launch of the case awaits its poll");
   HIPCHK(ctx, hipSetDevice(ctx->device));
   c->path = t->path;
   c->timing = t->timing != 0;

@@ -274,8 +274,52 @@ __device__ __forceinline__ void stage_tile_fast(const CaseDev& C, const StageArg
 
   // ---------------- phase L: the primitives of the box and the slabs into LDS ----------------
   // up to LU cells per thread per pass, every load of the pass issued before the first LDS store (LU x NP doubles
-  // in flight per thread: fewer cells per pass for more variables)
+  // in flight per thread: fewer cells per pass for more variables); more than eight variables take one cell per pass
   constexpr int LU = NP <= 5 ? 4 : (NP <= 8 ? 2 : 1);
+  if constexpr (LU == 1) {
+  for (int q = tid; q < NX; q += BS) {
+    int la, lb, lc;
+    if (q < NB) {
+      la = q % CI - 1; const int r = q / CI; lb = r % CJ - 1; lc = r / CJ - 1;
+    } else {
+      int g = q - NB;
+      if (g < 2 * G.s0) {
+        const int side = g >= G.s0; g -= side * G.s0;
+        la = side ? TI + 1 : -2; lb = g % TJ; lc = g / TJ;
+      } else if ((g -= 2 * G.s0) < 2 * G.s1) {
+        const int side = g >= G.s1; g -= side * G.s1;
+        lb = side ? TJ + 1 : -2; la = g % TI; lc = g / TI;
+      } else {
+        g -= 2 * G.s1;
+        const int side = g >= G.s2; g -= side * G.s2;
+        lc = side ? TK + 1 : -2; la = g % TI; lb = g / TI;
+      }
+    }
+    const int i = tl.i0 + la, j = tl.j0 + lb, k = tl.k0 + lc;
+    if (i < 1 - GC || i > B.n1 + GC || j < 1 - GC || j > B.n2 + GC || k < 1 - GC || k > B.n3 + GC) continue;
+    const int64_t r0 = row(B, i, j, k);
+    double y[NP];
+#pragma unroll
+    for (int s = 0; s < NP; ++s) y[s] = P[s * NC + r0];
+#pragma unroll
+    for (int s = 0; s < NP; ++s) Pl[s * NX + q] = y[s];
+    if constexpr (VISC) {
+      if (q < NB) {
+        if constexpr (FC::n == 1) {
+          Cc[q] = y[V::np] * rcp_nr(y[0] * T.Ri[0]);
+        } else {
+          const double rho = rho_sum<NS>(y), ir = rcp_nr(rho);
+          double Rg = 0.0;
+#pragma unroll
+          for (int s = 0; s < NS; ++s) Rg = Rg + T.Ri[s] * y[s] * ir;
+          Cc[q] = rho;
+          Cc[NB + q] = ir;
+          Cc[2 * NB + q] = y[V::np] * ir * rcp_nr(Rg);
+        }
+      }
+    }
+  }
+  } else {
   for (int q0 = tid; q0 < NX; q0 += LU * BS) {
     double y[LU][NP];
     int qs[LU];
@@ -329,6 +373,7 @@ __device__ __forceinline__ void stage_tile_fast(const CaseDev& C, const StageArg
         }
       }
     }
+  }
   }
   __syncthreads();
 

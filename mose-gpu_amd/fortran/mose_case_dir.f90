@@ -5,7 +5,9 @@
 !>   mose_case_dir <case_dir> <nsteps> <out.bin> [path [ti tj tk]]
 !>   mose_case_dir <case_dir> <nsteps> <out.bin> dump
 !>
-!> nsteps < 0 takes [MOSE-Parameters] iter-threshold.  path: 0 = fused (the default), 1 = plain, 2 = performance;
+!> nsteps < 0 takes [MOSE-Parameters] iter-threshold; a case without one runs while its time is below time-threshold,
+!> one step at a time (the reference's time loop, do while time < timemax).  path: 0 = fused (the default), 1 = plain,
+!> 2 = performance;
 !> the tile 0 0 0 (the default) asks the autotuner.  out.bin (little-endian stream): float64 P(NC, nprim), time;
 !> int64 steps.  "dump" writes what was read instead and touches no GPU:
 !>   int32 nb, nsc, nrans, nrec, ndir; mose_scheme (raw); int32 dims(3, nb); float64 wm(nsc), cp(NTAB, nsc),
@@ -27,6 +29,7 @@ program mose_case_dir
   type(mose_farfield), target :: ff(1)
   real(c_double), allocatable, target :: wm(:), cp(:, :), h(:, :), mu(:, :), kk(:, :), nodes(:), fields(:, :)
   real(c_double), allocatable, target :: dl(:), m(:), vol(:), yn(:), fm1(:), fm2(:), fm3(:), P(:, :)
+  real(c_double) :: tmax
   integer, allocatable :: dims(:, :)
   integer(c_int), allocatable, target :: cdims(:)
   integer(c_int64_t), allocatable :: boff(:)
@@ -78,7 +81,11 @@ program mose_case_dir
   end if
   call read_bc_open(trim(dir) // '/INPUT/bc.txt', rec, nrec)
   call sort_records(rec, nrec)
-  if (nsteps < 0) nsteps = int(num_or_zero(ini_get(ini, 'MOSE-Parameters', 'iter-threshold', '0')))
+  tmax = 0.0_c_double
+  if (nsteps < 0) then
+    nsteps = int(num_or_zero(ini_get(ini, 'MOSE-Parameters', 'iter-threshold', '0')))
+    if (nsteps == 0) tmax = num_or_zero(ini_get(ini, 'MOSE-Parameters', 'time-threshold', '0'))
+  end if
 
   ! ---- the arena: the initial state of every block's interior, ghosts from the nearest interior cell
   allocate(boff(nb), cdims(3 * nb))
@@ -143,7 +150,14 @@ contains
                                           case_id), 'mose_case_create')
     call mose_check(ctx, mose_state_upload(ctx, case_id, P), 'mose_state_upload')
     call mose_check(ctx, mose_case_tune(ctx, case_id, tun, chosen), 'mose_case_tune')
-    call MOSE_GPU_Explicit_Step(ctx, case_id, nsteps, rep)
+    if (tmax > 0.0_c_double) then
+      rep%time = 0.0_c_double
+      do while (rep%time < tmax)
+        call MOSE_GPU_Explicit_Step(ctx, case_id, 1, rep)
+      end do
+    else
+      call MOSE_GPU_Explicit_Step(ctx, case_id, nsteps, rep)
+    end if
     call mose_check(ctx, mose_state_download(ctx, case_id, P), 'mose_state_download')
     write(*, '(a,a,a,i0,a,es23.15e3,a,4(1x,i0))') ' case ', trim(dir), ' steps ', rep%steps, ' time ', rep%time, &
       ' tile', chosen%ti, chosen%tj, chosen%tk, chosen%path

@@ -129,3 +129,28 @@ def test_fortran_case_dir_runs_the_reference_case(gpu_ready, tmp_path, name, nst
     dts = OracleCase(case, nthreads=16).step(Po, nsteps)
     assert steps == nsteps and abs(t - dts.sum()) <= 1e-13 * dts.sum()
     assert compare(case, P, Po, f"fortran case directory {name}, path {path} x {nsteps} ({r.stdout.strip()})") <= 1e-12
+
+
+def test_fortran_case_dir_runs_to_the_time_threshold(gpu_ready, tmp_path):
+    """Noh87 has no iter-threshold: mose_case_dir with nsteps -1 steps while the time is below its time-threshold
+    (the reference's time loop), and the oracle stepped the same way gives the same state, time and step count."""
+    import os
+    from fortran_case_dir import GOLD, expand, read_result, run
+    from mosegpu import caseio
+    from test_gpu_walls_sa import compare
+    d = expand("noh87", tmp_path)
+    r = run(d, -1, str(tmp_path / "out.bin"))
+    assert r.returncode == 0, r.stdout + r.stderr
+    case = caseio.load_case(os.path.join(GOLD, "noh87"))
+    tmax = case.meta["time_threshold"]
+    assert case.nsteps == 0 and tmax > 0
+    P, t, steps = read_result(str(tmp_path / "out.bin"), case.layout.NC, case.nprim)
+    Po = case.P.copy()
+    o = OracleCase(case, nthreads=16)
+    time, n = 0.0, 0
+    while time < tmax:
+        time += float(o.step(Po, 1).sum())
+        n += 1
+    print(f"noh87 to t = {tmax}: {steps} steps on the device, {n} in the oracle", flush=True)
+    assert steps == n and abs(t - time) <= 1e-13 * time
+    assert compare(case, P, Po, f"fortran case directory noh87 to the time threshold ({r.stdout.strip()})") <= 1e-12
