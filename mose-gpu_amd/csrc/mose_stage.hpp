@@ -467,7 +467,9 @@ inline size_t fast_lds_doubles(int ns, int nr, int nprim, bool visc, int TI, int
   const size_t next = nbox + 2 * ((size_t)TJ * TK + (size_t)TI * TK + (size_t)TI * TJ);
   const size_t nslot = (size_t)(TI + 1) * TJ * TK + (size_t)TI * (TJ + 1) * TK + (size_t)TI * TJ * (TK + 1);
   const size_t ncc = visc ? ((ns == 1 && nr == 0) ? 1 : 3) : 0;
-  return (size_t)nprim * next + ncc * nbox + (size_t)nprim * nslot;
+  // one species without RANS on 8 x 8 x 4 tiles: 128 doubles for the staged cell metrics of the viscous faces
+  const size_t mst = (visc && ns == 1 && nr == 0 && TI == 8 && TJ == 8 && TK == 4) ? 128 : 0;
+  return (size_t)nprim * next + ncc * nbox + mst + (size_t)nprim * nslot;
 }
 
 }  // namespace mose

@@ -79,11 +79,13 @@ __device__ __forceinline__ double limiter_nr(int lim, double x, double y) {
 
 // One interior viscous face of direction D (diff_physics.inc:59-161) from the LDS stencil: Pl = the primitives
 // (variable stride nx), Cc = the cell scalars (stride nb); k1/k2 = the box indices of the face's left/right cells,
-// kb/kc = the box strides of the two other directions; c0/c1 = the cells' rows (metric).
-template <int NS, int NR, int D>
+// kb/kc = the box strides of the two other directions; c0/c1 = the cells' rows (metric), or, with MG, mg = the
+// metric averaged over the face's two cells, 0.5 (m_left + m_right), component q = x + 3 y at mg[q].
+template <int NS, int NR, int D, bool MG = false>
 __device__ __forceinline__ void visc_face_lds(const CaseDev& C, const double* Pl, int nx_, const double* Cc, int nb,
                                               int k1, int k2, int kb, int kc, int64_t c0, int64_t c1, double nx,
-                                              double ny, double nz, double area, double* res) {
+                                              double ny, double nz, double area, double* res,
+                                              const double* mg = nullptr) {
   using V = Vars<NS, NR>;
   using FC = FastCache<NS, NR, true>;
   constexpr int NP = V::nprim;
@@ -100,7 +102,10 @@ __device__ __forceinline__ void visc_face_lds(const CaseDev& C, const double* Pl
 #pragma unroll
   for (int x = 0; x < 3; ++x)
 #pragma unroll
-    for (int y = 0; y < 3; ++y) Mav[x][y] = 0.5 * (C.m[(int64_t)(x + 3 * y) * NC + c0] + C.m[(int64_t)(x + 3 * y) * NC + c1]);
+    for (int y = 0; y < 3; ++y) {
+      if constexpr (MG) Mav[x][y] = mg[x + 3 * y];
+      else Mav[x][y] = 0.5 * (C.m[(int64_t)(x + 3 * y) * NC + c0] + C.m[(int64_t)(x + 3 * y) * NC + c1]);
+    }
   const double drho = ft.rho, dmil = ft.mil, ird = rcp_nr(drho);
   double mie = 0.0;
   if (NR > 0) {
@@ -269,8 +274,16 @@ __device__ __forceinline__ void stage_tile_fast(const CaseDev& C, const StageArg
   const int NI = G.NI, NJ = G.NJ, NF = G.nslot;
   double* Pl = sm;                              // [NP][next]
   double* Cc = sm + NP * NX;                    // [FC::n][nbox]
-  double* Fb = Cc + FC::n * NB;                 // [NP][nslot]
+  // one species without RANS on 8 x 8 x 4 tiles: phase C reads the tile's cell metrics from LDS (below), and 128
+  // doubles between Cc and Fb make room for them
+  constexpr bool MST = VISC && NS == 1 && NR == 0 && STI == 8 && STJ == 8 && STK == 4;
+  double* Fb = Cc + FC::n * NB + (MST ? 128 : 0);   // [NP][nslot]
   const int64_t si = 1, sj = B.pi, sk = (int64_t)B.pi * B.pj;
+  // The i-stride of the viscous faces' cell-metric loads, for one species without RANS as a run-time value (B.pi is
+  // never 0): with the constant 1 the compiler merges each pair m[c0], m[c0 + 1] into one 16-byte load and issues the
+  // nine merged loads one after another, each waiting for the last (3.4 % of the 256^3 Taylor-Green stage); as two
+  // 8-byte loads they are issued together, as on the j- and k-faces.
+  const int64_t siv = (NS == 1 && NR == 0) ? (int64_t)(B.pi > 0) : si;
 
   // ---------------- phase L: the primitives of the box and the slabs into LDS ----------------
   // up to LU cells per thread per pass, every load of the pass issued before the first LDS store (LU x NP doubles
@@ -448,8 +461,99 @@ __device__ __forceinline__ void stage_tile_fast(const CaseDev& C, const StageArg
     }
   }
 
+  // ---------------- the tile's cell metrics into LDS (MST) ----------------
+  // During phase C the slabs of Pl (read only by phase A), the species row of Fb (one species carries no diffusion
+  // flux) and the 128 doubles before Fb hold the nine metric components of the tile's 256 cells: components 0..4 in
+  // the slabs of the five variables (256 doubles each), 5..8 in the 1024 doubles from Cc + NB.  The faces with both
+  // cells in the tile (640 of 896) then take their metric from LDS instead of 18 global loads; the 256 on the tile's
+  // sides load the outer cell's.  Threads 0..255 store components 0..4 while phase B runs; threads 256..511 hold
+  // 5..8 in registers until phase B's reads of Fb are over.
+  double mreg[4] = {0.0, 0.0, 0.0, 0.0};
+  if constexpr (MST) {
+    const int l = tid & 255, mi = tl.i0 + (l & 7), mj = tl.j0 + ((l >> 3) & 7), mk = tl.k0 + (l >> 6);
+    const bool inb = mi <= B.n1 && mj <= B.n2 && mk <= B.n3;
+    const int64_t cm = inb ? row(B, mi, mj, mk) : 0;
+    if (tid < 256) {
+#pragma unroll
+      for (int q = 0; q < 5; ++q) Pl[q * NX + NB + l] = inb ? C.m[(int64_t)q * NC + cm] : 0.0;
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) mreg[q] = inb ? C.m[(int64_t)(5 + q) * NC + cm] : 0.0;
+    }
+  }
+
   // ---------------- phase C: viscous faces ----------------
-  if constexpr (VISC) {
+  if constexpr (MST) {
+    double* Mx = Cc + NB;   // metric components 5..8 of the tile's cells
+    __syncthreads();
+    if (tid >= 256) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) Mx[q * 256 + (tid & 255)] = mreg[q];
+    }
+    __syncthreads();
+    for (int g = tid; g < NF; g += BS) {
+      // the faces with both cells in the tile first (224 i-, 224 j-, 192 k-faces), then the 256 on the tile's sides,
+      // so that every wavefront takes one of the two kinds; side 0 = the lower cell outside, 1 = the upper
+      int d, a, b, c, side = -1;
+      if (g < 224) { d = 0; a = 1 + g % 7; b = (g / 7) & 7; c = g / 56; }
+      else if (g < 448) { const int h = g - 224; d = 1; a = h & 7; b = 1 + (h >> 3) % 7; c = h / 56; }
+      else if (g < 640) { const int h = g - 448; d = 2; a = h & 7; b = (h >> 3) & 7; c = 1 + (h >> 6); }
+      else {
+        const int h = g - 640;
+        if (h < 64) { d = 0; side = h >> 5; b = h & 7; c = (h & 31) >> 3; a = side ? 8 : 0; }
+        else if (h < 128) { d = 1; side = (h - 64) >> 5; a = h & 7; c = ((h - 64) & 31) >> 3; b = side ? 8 : 0; }
+        else { d = 2; side = (h - 128) >> 6; a = h & 7; b = ((h - 128) & 63) >> 3; c = side ? 4 : 0; }
+      }
+      const int f = d == 0 ? a + 9 * (b + 8 * c) : (d == 1 ? NI + a + 8 * (b + 9 * c) : NI + NJ + a + 8 * (b + 8 * c));
+      const int i = tl.i0 + a - (d == 0), j = tl.j0 + b - (d == 1), k = tl.k0 + c - (d == 2);
+      const int n_d = d == 0 ? B.n1 : (d == 1 ? B.n2 : B.n3);
+      const int l_d = d == 0 ? i : (d == 1 ? j : k);
+      if (!((l_d >= 1 && l_d <= n_d - 1) && i <= B.n1 && j <= B.n2 && k <= B.n3)) continue;
+      const int k1 = (a + 1 - (d == 0)) + CI * ((b + 1 - (d == 1)) + CJ * (c + 1 - (d == 2)));
+      const int64_t c0 = row(B, i, j, k);
+      const double* fm = d == 0 ? C.fm1 : (d == 1 ? C.fm2 : C.fm3);
+      const int64_t NFg = d == 0 ? C.NF1 : (d == 1 ? C.NF2 : C.NF3);
+      const int64_t fg = d == 0 ? B.f1off + i + (int64_t)(B.n1 + 1) * ((j - 1) + (int64_t)B.n2 * (k - 1))
+                       : d == 1 ? B.f2off + (i - 1) + (int64_t)B.n1 * (j + (int64_t)(B.n2 + 1) * (k - 1))
+                                : B.f3off + (i - 1) + (int64_t)B.n1 * ((j - 1) + (int64_t)B.n2 * k);
+      const double fnx = fm[fg], fny = fm[NFg + fg], fnz = fm[2 * NFg + fg], fa = fm[3 * NFg + fg];
+      // the tile cells of the face (lower lo, upper up) and the face-averaged metric
+      const int lo = (a - (d == 0)) + 8 * ((b - (d == 1)) + 8 * (c - (d == 2)));
+      const int up = lo + (d == 0 ? 1 : (d == 1 ? 8 : 64));
+      auto mt = [&](int q, int l) { return q < 5 ? Pl[q * NX + NB + l] : Mx[(q - 5) * 256 + l]; };
+      double mg[9];
+      if (side < 0) {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) mg[q] = 0.5 * (mt(q, lo) + mt(q, up));
+      } else {
+        const int lin = side ? lo : up;
+        const int64_t cout = side ? c0 + (d == 0 ? si : (d == 1 ? sj : sk)) : c0;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) mg[q] = 0.5 * (mt(q, lin) + C.m[(int64_t)q * NC + cout]);
+      }
+      double res[NP];
+      if (d == 0) visc_face_lds<NS, NR, 0, true>(C, Pl, NX, Cc, NB, k1, k1 + 1, CI, CI * CJ, c0, c0, fnx, fny, fnz, fa, res, mg);
+      else if (d == 1) visc_face_lds<NS, NR, 1, true>(C, Pl, NX, Cc, NB, k1, k1 + CI, 1, CI * CJ, c0, c0, fnx, fny, fnz, fa, res, mg);
+      else visc_face_lds<NS, NR, 2, true>(C, Pl, NX, Cc, NB, k1, k1 + CI * CJ, 1, CI, c0, c0, fnx, fny, fnz, fa, res, mg);
+      // the species row of Fb holds metric components: one species has no diffusion flux (res[0] = 0)
+#pragma unroll
+      for (int s = 1; s < NP; ++s) Fb[s * NF + f] = res[s];
+    }
+    __syncthreads();
+    if (mine) {
+#pragma unroll
+      for (int s = 1; s < NP; ++s) {
+        double t = tmp[s];
+        if (ci < B.n1) t = t - Fb[s * NF + sIp];
+        if (ci > 1) t = t + Fb[s * NF + sIm];
+        if (cj < B.n2) t = t - Fb[s * NF + sJp];
+        if (cj > 1) t = t + Fb[s * NF + sJm];
+        if (ck < B.n3) t = t - Fb[s * NF + sKp];
+        if (ck > 1) t = t + Fb[s * NF + sKm];
+        tmp[s] = t;
+      }
+    }
+  } else if constexpr (VISC) {
     __syncthreads();
     for (int f = tid; f < NF; f += BS) {
       int d, a, b, c;
@@ -469,7 +573,7 @@ __device__ __forceinline__ void stage_tile_fast(const CaseDev& C, const StageArg
                                 : B.f3off + (i - 1) + (int64_t)B.n1 * ((j - 1) + (int64_t)B.n2 * k);
       const double fnx = fm[fg], fny = fm[NFg + fg], fnz = fm[2 * NFg + fg], fa = fm[3 * NFg + fg];
       double res[NP];
-      if (d == 0) visc_face_lds<NS, NR, 0>(C, Pl, NX, Cc, NB, k1, k1 + 1, CI, CI * CJ, c0, c0 + si, fnx, fny, fnz, fa, res);
+      if (d == 0) visc_face_lds<NS, NR, 0>(C, Pl, NX, Cc, NB, k1, k1 + 1, CI, CI * CJ, c0, c0 + siv, fnx, fny, fnz, fa, res);
       else if (d == 1) visc_face_lds<NS, NR, 1>(C, Pl, NX, Cc, NB, k1, k1 + CI, 1, CI * CJ, c0, c0 + sj, fnx, fny, fnz, fa, res);
       else visc_face_lds<NS, NR, 2>(C, Pl, NX, Cc, NB, k1, k1 + CI * CJ, 1, CI, c0, c0 + sk, fnx, fny, fnz, fa, res);
 #pragma unroll
