@@ -69,6 +69,10 @@ enum {
 /* limiter ids (Lib_Limiters.f90:21-61) and Riemann solver ids (Lib_Riemann.f90:23-107) */
 enum { MOSE_LIM_MINMOD = 1, MOSE_LIM_VANLEER = 2, MOSE_LIM_VANALBADA = 3, MOSE_LIM_MC = 4, MOSE_LIM_SUPERBEE = 5, MOSE_LIM_FIRST = 6 };
 enum { MOSE_RS_HLLC = 1, MOSE_RS_HLLE = 2, MOSE_RS_AUSMP = 4, MOSE_RS_LMROE = 12 };
+/* integration variables of the state update (Assign_Integration_Variables, Lib_Newstate.f90:43-60): conservative
+ * (Newstate_Conservative_blk) or primitive (Newstate_Primitive_blk, Lib_Newstate.f90:340-531, with Theta = (drho/dp)_T;
+ * the preconditioned variant, integration-variables = prec, is not built) */
+enum { MOSE_INT_CONS = 0, MOSE_INT_PRIM = 1 };
 
 /* one boundary cell-face: bc_bi/bc_bj/bc_bk/bc_bf/bc_typ of Mod_BC.f90, the connection donor of
  * Lib_Extrapolation.f90:1023-1135 (conn_is/js/ks/fs, conn_d11..d22) and the wall / far-field data */
@@ -104,7 +108,8 @@ typedef struct {
   int32_t ndir;         /* 1, 2 or 3 computational directions */
   int32_t point_implicit;
   int32_t thermo_guard; /* 1: T outside (1, 9990) K is an error (MOSE_ERR_THERMO) */
-  int32_t reserved[7];
+  int32_t integration;  /* MOSE_INT_CONS (default) or MOSE_INT_PRIM */
+  int32_t reserved[6];
   double mco;           /* riemann-options-Mco (LMRoe) */
   double cfl, vnn;
   double prandtl;       /* > 0: k = mu cp / Pr */

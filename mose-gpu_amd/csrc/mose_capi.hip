@@ -275,6 +275,10 @@ static int plan_case(const mose_scheme* sc, int nb, const int* dims, int nrec, c
   }
   if (sc->n_rk < 1 || sc->n_rk > 3) { why = "n_rk must be 1..3"; return MOSE_ERR_ARG; }
   if (sc->ndir < 1 || sc->ndir > 3) { why = "ndir must be 1..3"; return MOSE_ERR_ARG; }
+  if (sc->integration != MOSE_INT_CONS && sc->integration != MOSE_INT_PRIM) {
+    why = "integration must be MOSE_INT_CONS (0) or MOSE_INT_PRIM (1)";
+    return MOSE_ERR_ARG;
+  }
   c.sc = *sc;
   c.nb = nb;
   c.nprim = sc->nsc + 4 + sc->nrans;
@@ -620,7 +624,7 @@ static CaseDev make_dev(mose_ctx* ctx, Case& c, const double* Pin, double* Pout)
   D.vc.sc = c.sc.sc; D.vc.sct = c.sc.sct; D.vc.prt = c.sc.prt; D.vc.prl = c.sc.prandtl;
   D.tab = ctx->dtab;
   D.dt = c.dt; D.dtmin = c.dtmin; D.bad = c.bad; D.first_bad = c.first_bad; D.strang = c.strang;
-  D.cfl = c.sc.cfl; D.vnn = c.sc.vnn; D.ndir = c.sc.ndir;
+  D.cfl = c.sc.cfl; D.vnn = c.sc.vnn; D.ndir = c.sc.ndir; D.prim = c.sc.integration == MOSE_INT_PRIM;
   return D;
 }
 
@@ -898,6 +902,8 @@ extern "C" int mose_case_tune(mose_ctx* ctx, int id, const mose_tuning* t, mose_
   if (t->path != MOSE_PATH_FUSED && t->path != MOSE_PATH_PLAIN && t->path != MOSE_PATH_FAST)
     return fail(ctx, MOSE_ERR_ARG, "unknown path");
   if (c->launched || c->pending) return fail(ctx, MOSE_ERR_ARG, "mose_case_tune: a launch of the case awaits its poll");
+  if (t->path == MOSE_PATH_FAST && c->sc.nsc == 9 && c->sc.integration == MOSE_INT_PRIM)
+    return fail(ctx, MOSE_ERR_ARG, "the performance path's nine-species kernels take the conservative update only");
   HIPCHK(ctx, hipSetDevice(ctx->device));
   c->path = t->path;
   c->timing = t->timing != 0;

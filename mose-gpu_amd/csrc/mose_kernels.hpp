@@ -47,6 +47,7 @@ struct CaseDev {
   unsigned long long* first_bad;
   double cfl, vnn;
   int ndir;
+  int prim;              // integration-variables = prim: the primitive update (update_cell_prim); in the padding after ndir
 };
 
 __device__ __forceinline__ int64_t row(const mose_blk& B, int i, int j, int k) {
@@ -620,7 +621,8 @@ __device__ __forceinline__ void update_one(const CaseDev& C, const mose_blk& B, 
     Tseed = dmin2(dmax2(Tseed, 1.0), 9990.0);
   }
   if (NR > 0) cell_sa<NS, NR>(C, B, i, j, k, c0, rho, cell_mil<NS>(T, p, rho, Tseed), res);
-  update_cell<NS, NR>(T, p, po, res, C.vol[c0], *C.dt, C.strang, rkc, Tseed, out);
+  if (C.prim) update_cell_prim<NS, NR>(T, p, po, res, C.vol[c0], *C.dt, C.strang, rkc, out);
+  else update_cell<NS, NR>(T, p, po, res, C.vol[c0], *C.dt, C.strang, rkc, Tseed, out);
   bool ok = out[V::np] >= 0.0;
 #pragma unroll
   for (int s = 0; s < V::nprim; ++s) ok = ok && isfinite(out[s]);

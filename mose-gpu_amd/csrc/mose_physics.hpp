@@ -680,4 +680,125 @@ __device__ __forceinline__ void update_cell(const mose_dtab& T, const double* p,
   for (int s = V::nt; s < V::nprim; ++s) if (out[s] < 1e-10) out[s] = 1e-10;
 }
 
+// Newstate_Primitive_blk (Lib_Newstate.f90:340-531), integration-variables = prim: Compute_Prim_Residual turns the
+// conservative residual res into primitive increments through the inverse of d(cons)/d(prim) at the current state p
+// (Weiss-Smith form, Theta = (drho/dp)_T = 1/(R T)); the increments are scaled by dt/vol*strang and RK_stage runs on
+// the primitives; then the floors of Check_And_Fix_State.  In the statement order of the reference (the zero entries
+// of the one-species inverse are multiplied as it multiplies them).
+template <int NS, int NR>
+__device__ __forceinline__ void update_cell_prim(const mose_dtab& T, const double* p, const double* po,
+                                                 const double* res, double vol, double dt, double strang, double rkc,
+                                                 double* out) {
+  using V = Vars<NS, NR>;
+  constexpr int NP = V::nprim;
+  double r[NP];
+#pragma unroll
+  for (int s = 0; s < NP; ++s) r[s] = res[s];
+  const double rho = rho_sum<NS>(p), Rgas = f_Rtot<NS>(T, p);
+  const double temperature = p[V::np] / (rho * Rgas);
+  const int Ti = ftrunc(temperature);
+  const double Tdiff = temperature - Ti;
+  const double inv_rho = 1.0 / rho;
+  const double vel[3] = {p[V::nu], p[V::nv], p[V::nw]};
+  const double modvel = sqrt(vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2]);   // norm2
+  const double q2 = modvel * modvel;
+  const double cp = f_cp<NS>(T, p, Ti, Tdiff, rho);
+  const double rho_T = -rho / temperature;
+  const double rho_p = 1.0 / (Rgas * temperature);
+  const double enthalpy = H0<NS>(T, p[V::np], p, modvel);
+  const double Theta = rho_p;
+  const double Denom = 1.0 / (rho * cp * Theta + rho_T);
+  if constexpr (NS == 1) {
+    // invGamma: (drho, drhoV, drhoE) -> (dp, du, dv, dw, dT)
+    double invG[5][5];
+    invG[0][0] = (rho_T * (enthalpy - q2) + rho * cp) * Denom;
+    invG[0][1] = rho_T * vel[0] * Denom;
+    invG[0][2] = rho_T * vel[1] * Denom;
+    invG[0][3] = rho_T * vel[2] * Denom;
+    invG[0][4] = -rho_T * Denom;
+#pragma unroll
+    for (int q = 1; q <= 3; ++q) {
+#pragma unroll
+      for (int c = 0; c < 5; ++c) invG[q][c] = 0.0;
+      invG[q][0] = -vel[q - 1] * inv_rho;
+      invG[q][q] = inv_rho;
+    }
+    invG[4][0] = -(Theta * (enthalpy - q2) - 1.0) * Denom;
+    invG[4][1] = -vel[0] * Theta * Denom;
+    invG[4][2] = -vel[1] * Theta * Denom;
+    invG[4][3] = -vel[2] * Theta * Denom;
+    invG[4][4] = Theta * Denom;
+    const double cres[5] = {-r[0], -r[V::nu], -r[V::nv], -r[V::nw], -r[V::np]};
+    double pres[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      pres[q] = 0.0;
+#pragma unroll
+      for (int c = 0; c < 5; ++c) pres[q] = pres[q] + invG[q][c] * cres[c];
+    }
+    r[V::nu] = pres[1]; r[V::nv] = pres[2]; r[V::nw] = pres[3];
+    r[V::np] = pres[0];
+    r[0] = rho_p * pres[0] + rho_T * pres[4];   // density from dp and dT
+  } else {
+    // several species: the mass-fraction sensitivities enter the p and T rows
+    constexpr int M = NS - 1;
+    double Yi[M], drdyi[M], dhdyi[M], dyi[M], Rs[NP + 1], dPa[NP + 1];
+#pragma unroll
+    for (int s = 0; s < M; ++s) {
+      Yi[s] = p[s] * inv_rho;
+      if (Yi[s] <= 1e-9) Yi[s] = 0.0;
+      drdyi[s] = -rho * (T.Ri[s] - T.Ri[NS - 1]) / Rgas;
+      dhdyi[s] = tabT(T.h, s, Ti, Tdiff) - tabT(T.h, NS - 1, Ti, Tdiff);
+    }
+    double sres = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) sres = sres + r[s];
+    Rs[1] = -sres;
+    Rs[2] = -r[V::nu]; Rs[3] = -r[V::nv]; Rs[4] = -r[V::nw]; Rs[5] = -r[V::np];
+#pragma unroll
+    for (int s = 0; s < M; ++s) Rs[6 + s] = -r[s];
+    double sYd = 0.0, sdY = 0.0, sR1 = 0.0, sR2 = 0.0;
+#pragma unroll
+    for (int s = 0; s < M; ++s) {
+      sYd = sYd + Yi[s] * drdyi[s];
+      sdY = sdY + dhdyi[s] * Yi[s];
+      sR1 = sR1 + Rs[6 + s] * (rho_T * dhdyi[s] - cp * drdyi[s]);
+      sR2 = sR2 + Rs[6 + s] * (Theta * dhdyi[s] + drdyi[s] * inv_rho);
+    }
+    const double sv = vel[0] * Rs[2] + vel[1] * Rs[3] + vel[2] * Rs[4];
+    dPa[1] = (Rs[1] * (cp * (rho + sYd) - rho_T * (sdY + q2 - enthalpy)) + sR1 - rho_T * Rs[5] + rho_T * sv) * Denom;
+    dPa[2] = -Rs[1] * vel[0] * inv_rho + Rs[2] * inv_rho;
+    dPa[3] = -Rs[1] * vel[1] * inv_rho + Rs[3] * inv_rho;
+    dPa[4] = -Rs[1] * vel[2] * inv_rho + Rs[4] * inv_rho;
+    dPa[5] = (Rs[1] * (Theta * (sdY - enthalpy + q2) + sYd * inv_rho + 1.0) - sR2 + Theta * (Rs[5] - sv)) * Denom;
+#pragma unroll
+    for (int s = 0; s < M; ++s) dPa[6 + s] = (Rs[6 + s] - Rs[1] * Yi[s]) * inv_rho;
+    double sdrdy = 0.0, dyN = 0.0;
+#pragma unroll
+    for (int s = 0; s < M; ++s) {
+      dyi[s] = dPa[6 + s];
+      sdrdy = sdrdy + drdyi[s] * dyi[s];
+      dyN = dyN + dyi[s];
+    }
+    const double drho = rho_p * dPa[1] + rho_T * dPa[5] + sdrdy;
+    dyN = -dyN;
+    const double drhoN = rho * dyN + p[NS - 1] * inv_rho * drho;
+#pragma unroll
+    for (int s = 0; s < M; ++s) r[s] = rho * dyi[s] + Yi[s] * drho;
+    r[NS - 1] = drhoN;
+    r[V::nu] = dPa[2]; r[V::nv] = dPa[3]; r[V::nw] = dPa[4];
+    r[V::np] = dPa[1];
+  }
+#pragma unroll
+  for (int s = V::np + 1; s < NP; ++s) r[s] = -r[s];   // turbulence variables: not transformed
+#pragma unroll
+  for (int s = 0; s < NP; ++s) r[s] = r[s] * dt / vol * strang;
+#pragma unroll
+  for (int s = 0; s < NP; ++s) out[s] = po[s] + (p[s] - po[s] + r[s]) * rkc;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) if (out[s] < 1e-20) out[s] = 1e-20;
+#pragma unroll
+  for (int s = V::nt; s < NP; ++s) if (out[s] < 1e-10) out[s] = 1e-10;
+}
+
 }  // namespace mose

@@ -373,7 +373,8 @@ __device__ __forceinline__ void stage_tile(const CaseDev& C, const StageArgs& A,
       double rho = cache[(ca + 1) + (TI + 2) * ((cb + 1) + (TJ + 2) * (cc + 1))];
       cell_sa<NS, NR>(C, B, ci, cj, ck, c0, rho, cell_mil<NS>(T, p, rho, Tseed), tmp);
     }
-    update_cell<NS, NR>(T, p, po, tmp, C.vol[c0], *C.dt, C.strang, A.rkc, Tseed, out);
+    if (C.prim) update_cell_prim<NS, NR>(T, p, po, tmp, C.vol[c0], *C.dt, C.strang, A.rkc, out);
+    else update_cell<NS, NR>(T, p, po, tmp, C.vol[c0], *C.dt, C.strang, A.rkc, Tseed, out);
     bool good = out[V::np] >= 0.0;
 #pragma unroll
     for (int s = 0; s < NP; ++s) good = good && isfinite(out[s]);

@@ -633,7 +633,13 @@ __device__ __forceinline__ void stage_tile_fast(const CaseDev& C, const StageArg
       const double rho = VISC ? Cc[kc0] : rho_sum<NS>(p);
       cell_sa<NS, NR>(C, B, ci, cj, ck, c0, rho, cell_mil<NS>(T, p, rho, Tseed), tmp);
     }
-    update_cell_nr<NS, NR>(T, p, po, tmp, w, A.rkc, Tseed, out);
+    // the primitive update below nine species (the nine-species kernels take the conservative one only)
+    if constexpr (NS < 9) {
+      if (C.prim) update_cell_prim<NS, NR>(T, p, po, tmp, C.vol[c0], *C.dt, C.strang, A.rkc, out);
+      else update_cell_nr<NS, NR>(T, p, po, tmp, w, A.rkc, Tseed, out);
+    } else {
+      update_cell_nr<NS, NR>(T, p, po, tmp, w, A.rkc, Tseed, out);
+    }
     bool good = out[V::np] >= 0.0;
 #pragma unroll
     for (int s = 0; s < NP; ++s) good = good && isfinite(out[s]);

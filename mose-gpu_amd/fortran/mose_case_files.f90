@@ -128,9 +128,9 @@ contains
     type(ini_t), intent(in) :: ini
     integer, intent(in) :: ndir, nsc
     type(mose_scheme), intent(out) :: sc
-    character(len=256) :: eq, turb, ts, recon, lim, rs
+    character(len=256) :: eq, turb, ts, recon, lim, rs, iv
     logical :: viscous
-    sc%nsc = nsc; sc%nrans = 0; sc%reserved = 0; sc%reserved_d = 0.0_c_double
+    sc%nsc = nsc; sc%nrans = 0; sc%reserved = 0; sc%reserved_d = 0.0_c_double; sc%integration = MOSE_INT_CONS
     sc%thermo_guard = 1; sc%point_implicit = 1; sc%ndir = ndir
     sc%sct = 0.9_c_double; sc%prt = 0.9_c_double; sc%vnn = 0.5_c_double; sc%prandtl = 0.0_c_double
     sc%sc = 0.5_c_double
@@ -156,8 +156,14 @@ contains
     case ('RK3'); sc%n_rk = 3
     case default; call case_stop('time-scheme = ' // trim(ts) // ': euler, RK2 or RK3')
     end select
-    if (trim(ini_get(ini, 'MOSE-Numerics', 'integration-variables', 'cons')) /= 'cons') &
-      call case_stop('integration-variables other than cons are not on this path')
+    ! the state update's variables (Assign_Integration_Variables, Lib_Newstate.f90:43-60)
+    iv = ini_get(ini, 'MOSE-Numerics', 'integration-variables', 'cons')
+    select case (trim(iv))
+    case ('cons'); sc%integration = MOSE_INT_CONS
+    case ('prim'); sc%integration = MOSE_INT_PRIM
+    case default; call case_stop('integration-variables = ' // trim(iv) // &
+                                 ': cons or prim (the preconditioned update, prec, is not on this path)')
+    end select
     recon = ini_get(ini, 'MOSE-Numerics', 'space-reconstruction', '')
     if (trim(recon) /= 'MUSCL' .and. trim(recon) /= 'first-order') &
       call case_stop('space-reconstruction = ' // trim(recon) // ': MUSCL or first-order')

@@ -20,6 +20,7 @@ module mose_gpu_c
                                MOSE_LIM_SUPERBEE = 5, MOSE_LIM_FIRST = 6
   integer(c_int), parameter :: MOSE_RS_HLLC = 1, MOSE_RS_HLLE = 2, MOSE_RS_AUSMP = 4, MOSE_RS_LMROE = 12
   integer(c_int), parameter :: MOSE_PATH_FUSED = 0, MOSE_PATH_PLAIN = 1, MOSE_PATH_FAST = 2
+  integer(c_int), parameter :: MOSE_INT_CONS = 0, MOSE_INT_PRIM = 1
   integer(c_int), parameter :: MOSE_NTAB = 10001, MOSE_GC = 2, MOSE_MAX_SPECIES = 16
 
   type, bind(C) :: mose_bc_record
@@ -39,7 +40,8 @@ module mose_gpu_c
 
   type, bind(C) :: mose_scheme
     integer(c_int32_t) :: nsc, nrans, limiter, riemann, viscous, n_rk, ndir, point_implicit, thermo_guard
-    integer(c_int32_t) :: reserved(7)
+    integer(c_int32_t) :: integration
+    integer(c_int32_t) :: reserved(6)
     real(c_double) :: mco, cfl, vnn, prandtl, sc, sct, prt
     real(c_double) :: reserved_d(8)
   end type mose_scheme

@@ -292,8 +292,10 @@ def scheme_from_ini(ini: dict, ndir: int, nsc: int):
     rk = {"euler": 1, "RK2": 2, "RK3": 3}.get(nu.get("time-scheme", ""))
     if rk is None:
         raise CaseFormatError(f"time-scheme = {nu.get('time-scheme')!r}: euler, RK2 or RK3")
-    if nu.get("integration-variables", "cons") != "cons":
-        raise CaseFormatError("integration-variables other than cons are not on this path")
+    intvar = nu.get("integration-variables", "cons")
+    if intvar not in ("cons", "prim"):
+        raise CaseFormatError(f"integration-variables = {intvar!r}: cons or prim (the preconditioned update, prec, "
+                              "is not on this path)")
     recon = nu.get("space-reconstruction", "")
     if recon not in ("MUSCL", "first-order"):
         raise CaseFormatError(f"space-reconstruction = {recon!r}: MUSCL or first-order")
@@ -319,7 +321,7 @@ def scheme_from_ini(ini: dict, ndir: int, nsc: int):
     if turb == "SA":
         pimp = 1 if _truth(ini.get("MOSE-Turbulence", {}).get("point-implicit", "true")) else 0
     return make_scheme(nsc=nsc, nrans=1 if turb == "SA" else 0, limiter=lim, riemann=rs, viscous=viscous, n_rk=rk,
-                       ndir=ndir, cfl=float(nu["cfl"]), mco=mco, point_implicit=pimp, **kw)
+                       ndir=ndir, cfl=float(nu["cfl"]), mco=mco, point_implicit=pimp, integration=intvar, **kw)
 
 
 def load_case(case_dir: str, nsteps: int | None = None) -> Case:

@@ -19,7 +19,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from .core import (BC_CONNECTION, BC_EXTRAPOLATION, BC_NONE, BC_FARFIELD, BC_WALL_ADIABATIC, BC_WALL_ISOTHERMAL,
-                   FARFIELD_DTYPE, GC, LIM, NTAB, RECORD_DTYPE, RS, Geometry, Layout, Scheme, ThermoTables,
+                   FARFIELD_DTYPE, GC, INTEGRATION, LIM, NTAB, RECORD_DTYPE, RS, Geometry, Layout, Scheme, ThermoTables,
                    metrics_build, records_tangent, wall_distance)
 
 RUNIV = 8314.46261815324
@@ -140,10 +140,14 @@ class Case:
 
 
 def make_scheme(nsc=1, nrans=0, limiter="minmod", riemann="HLLC", viscous=False, n_rk=3, ndir=3, cfl=0.5,
-                vnn=0.5, mco=1.0, prandtl=0.0, sc=0.5, sct=0.9, prt=0.9, point_implicit=1, guard=1) -> Scheme:
+                vnn=0.5, mco=1.0, prandtl=0.0, sc=0.5, sct=0.9, prt=0.9, point_implicit=1, guard=1,
+                integration="cons") -> Scheme:
+    """integration: the state update's variables, "cons" (Newstate_Conservative_blk) or "prim"
+    (Newstate_Primitive_blk), as [MOSE-Numerics] integration-variables (Lib_Newstate.f90:43-60)."""
     s = Scheme()
     s.nsc = nsc; s.nrans = nrans; s.limiter = LIM[limiter]; s.riemann = RS[riemann]; s.viscous = int(viscous)
     s.n_rk = n_rk; s.ndir = ndir; s.point_implicit = point_implicit; s.thermo_guard = guard
+    s.integration = INTEGRATION[integration]
     s.mco = mco; s.cfl = cfl; s.vnn = vnn; s.prandtl = prandtl; s.sc = sc; s.sct = sct; s.prt = prt
     return s
 

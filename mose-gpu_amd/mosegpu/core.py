@@ -27,6 +27,9 @@ ERRORS = {
 BC_NONE, BC_CONNECTION, BC_SYMMETRY, BC_FARFIELD, BC_WALL_ADIABATIC, BC_WALL_ISOTHERMAL, BC_EXTRAPOLATION = 0, 1, 3, 4, 5, 6, 11
 LIM = {"minmod": 1, "vanleer": 2, "vanalbada": 3, "mc": 4, "superbee": 5, "first-order": 6}
 RS = {"HLLC": 1, "HLLE": 2, "AUSM+": 4, "AUSMp": 4, "LMRoe": 12, "Roe": 12}
+# integration-variables of the state update (Lib_Newstate.f90:43-60): conservative or primitive
+INTEGRATION = {"cons": 0, "prim": 1}
+INT_CONS, INT_PRIM = 0, 1
 
 # one boundary cell-face (mose_bc_record)
 RECORD_DTYPE = np.dtype(
@@ -58,7 +61,7 @@ class Scheme(C.Structure):
     _fields_ = [
         ("nsc", C.c_int32), ("nrans", C.c_int32), ("limiter", C.c_int32), ("riemann", C.c_int32),
         ("viscous", C.c_int32), ("n_rk", C.c_int32), ("ndir", C.c_int32), ("point_implicit", C.c_int32),
-        ("thermo_guard", C.c_int32), ("reserved", C.c_int32 * 7),
+        ("thermo_guard", C.c_int32), ("integration", C.c_int32), ("reserved", C.c_int32 * 6),
         ("mco", C.c_double), ("cfl", C.c_double), ("vnn", C.c_double), ("prandtl", C.c_double),
         ("sc", C.c_double), ("sct", C.c_double), ("prt", C.c_double), ("reserved_d", C.c_double * 8),
     ]

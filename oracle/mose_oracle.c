@@ -1673,7 +1673,7 @@ double o_compute_dt(const o_case *C, const double *P) {
 /* ------------------------------------------------------------------------------------------------ */
 /* conservative update: Newstate_Conservative_blk + newstate_cons_body.inc:5-89                       */
 /* ------------------------------------------------------------------------------------------------ */
-void o_newstate(const o_case *C, double *P, const double *PO, const double *R, double dt, double rkc) {
+static void o_newstate_cons(const o_case *C, double *P, const double *PO, const double *R, double dt, double rkc) {
   const o_scheme *S = &C->sc;
   const o_tables *t = &C->tab;
   const int64_t NC = C->NC;
@@ -1755,6 +1755,132 @@ void o_newstate(const o_case *C, double *P, const double *PO, const double *R, d
           for (int s = 0; s < nprim; ++s) PV(P, s, c0) = p_l[s];
         }
   }
+}
+
+/* ------------------------------------------------------------------------------------------------ */
+/* primitive update: Newstate_Primitive_blk (Lib_Newstate.f90:340-531), integration-variables = prim   */
+/* ------------------------------------------------------------------------------------------------ */
+/* One cell: Compute_Prim_Residual (the conservative residual turned into primitive increments through the inverse
+ * of d(cons)/d(prim), Weiss-Smith form with Theta = (drho/dp)_T = 1/(R T)), the scaling dt/vol*strang, RK_stage on
+ * the primitives, and the floors of Check_And_Fix_State.  The thermodynamics of the current state: co_rotot_Rtot,
+ * f_cp_dev, H0_dev and f_tabT_expr (table rows clamped as everywhere in the oracle). */
+static void o_prim_cell(const o_tables *t, const o_scheme *S, double *p_l, const double *po_l, double *res_l, double dt_,
+                 double vol_, double strang, double rkc) {
+  const int nsc = S->nsc, nu = S->nu, nv = S->nv, nw = S->nw, np = S->np, nprim = S->nprim;
+  double rho, Rgas;
+  co_rotot_Rtot(t, p_l, &rho, &Rgas);
+  const double temperature = p_l[np] / (rho * Rgas);
+  const int T_i = fint(temperature);
+  const double Tdiff = temperature - T_i;
+  const double inv_rho = 1.0 / rho;
+  const double vel[3] = {p_l[nu], p_l[nv], p_l[nw]};
+  const double modvel = sqrt(vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2]);   /* norm2 */
+  const double q2 = modvel * modvel;
+  const double cp = f_cp_dev(t, p_l, T_i, Tdiff, rho);
+  const double rho_T = -rho / temperature;
+  const double rho_p = 1.0 / (Rgas * temperature);
+  const double enthalpy = H0_dev(t, p_l[np], p_l, modvel);
+  const double Theta = rho_p;   /* prim (prec would take 1/(beta2 a^2) + 1/(cp T)) */
+  const double Denom = 1.0 / (rho * cp * Theta + rho_T);
+  if (nsc == 1) {
+    /* invGamma: (drho, drhoV, drhoE) -> (dp, du, dv, dw, dT) */
+    double invG[5][5];
+    invG[0][0] = (rho_T * (enthalpy - q2) + rho * cp) * Denom;
+    invG[0][1] = rho_T * vel[0] * Denom;
+    invG[0][2] = rho_T * vel[1] * Denom;
+    invG[0][3] = rho_T * vel[2] * Denom;
+    invG[0][4] = -rho_T * Denom;
+    for (int r = 1; r <= 3; ++r) {
+      for (int c = 0; c < 5; ++c) invG[r][c] = 0.0;
+      invG[r][0] = -vel[r - 1] * inv_rho;
+      invG[r][r] = inv_rho;
+    }
+    invG[4][0] = -(Theta * (enthalpy - q2) - 1.0) * Denom;
+    invG[4][1] = -vel[0] * Theta * Denom;
+    invG[4][2] = -vel[1] * Theta * Denom;
+    invG[4][3] = -vel[2] * Theta * Denom;
+    invG[4][4] = Theta * Denom;
+    const double cres[5] = {-res_l[0], -res_l[nu], -res_l[nv], -res_l[nw], -res_l[np]};
+    double pres[5];
+    for (int r = 0; r < 5; ++r) {
+      pres[r] = 0.0;
+      for (int c = 0; c < 5; ++c) pres[r] = pres[r] + invG[r][c] * cres[c];
+    }
+    res_l[nu] = pres[1]; res_l[nv] = pres[2]; res_l[nw] = pres[3];
+    res_l[np] = pres[0];
+    res_l[0] = rho_p * pres[0] + rho_T * pres[4];   /* density from dp and dT */
+  } else {
+    /* several species: the mass-fraction sensitivities enter the p and T rows */
+    double Yi[ONSMAX], drdyi[ONSMAX], dhdyi[ONSMAX], dyi[ONSMAX], Rs[ONPRIM + 1], dPa[ONPRIM + 1];
+    for (int s = 0; s < nsc - 1; ++s) {
+      Yi[s] = p_l[s] * inv_rho;
+      if (Yi[s] <= 1e-9) Yi[s] = 0.0;
+      drdyi[s] = -rho * (t->Ri[s] - t->Ri[nsc - 1]) / Rgas;
+      dhdyi[s] = tabT(t->h, t, s, T_i, T_i + 1, Tdiff) - tabT(t->h, t, nsc - 1, T_i, T_i + 1, Tdiff);
+    }
+    double sres = 0.0;
+    for (int s = 0; s < nsc; ++s) sres = sres + res_l[s];
+    Rs[1] = -sres;
+    Rs[2] = -res_l[nu]; Rs[3] = -res_l[nv]; Rs[4] = -res_l[nw]; Rs[5] = -res_l[np];
+    for (int s = 0; s < nsc - 1; ++s) Rs[6 + s] = -res_l[s];
+    double sYd = 0.0, sdY = 0.0, sR1 = 0.0, sR2 = 0.0;
+    for (int s = 0; s < nsc - 1; ++s) {
+      sYd = sYd + Yi[s] * drdyi[s];
+      sdY = sdY + dhdyi[s] * Yi[s];
+      sR1 = sR1 + Rs[6 + s] * (rho_T * dhdyi[s] - cp * drdyi[s]);
+      sR2 = sR2 + Rs[6 + s] * (Theta * dhdyi[s] + drdyi[s] * inv_rho);
+    }
+    const double sv = vel[0] * Rs[2] + vel[1] * Rs[3] + vel[2] * Rs[4];
+    dPa[1] = (Rs[1] * (cp * (rho + sYd) - rho_T * (sdY + q2 - enthalpy)) + sR1 - rho_T * Rs[5] + rho_T * sv) * Denom;
+    dPa[2] = -Rs[1] * vel[0] * inv_rho + Rs[2] * inv_rho;
+    dPa[3] = -Rs[1] * vel[1] * inv_rho + Rs[3] * inv_rho;
+    dPa[4] = -Rs[1] * vel[2] * inv_rho + Rs[4] * inv_rho;
+    dPa[5] = (Rs[1] * (Theta * (sdY - enthalpy + q2) + sYd * inv_rho + 1.0) - sR2 + Theta * (Rs[5] - sv)) * Denom;
+    for (int s = 0; s < nsc - 1; ++s) dPa[6 + s] = (Rs[6 + s] - Rs[1] * Yi[s]) * inv_rho;
+    double sdrdy = 0.0, dyN = 0.0;
+    for (int s = 0; s < nsc - 1; ++s) {
+      dyi[s] = dPa[6 + s];
+      sdrdy = sdrdy + drdyi[s] * dyi[s];
+      dyN = dyN + dyi[s];
+    }
+    const double drho = rho_p * dPa[1] + rho_T * dPa[5] + sdrdy;
+    dyN = -dyN;
+    const double drhoN = rho * dyN + p_l[nsc - 1] * inv_rho * drho;
+    for (int s = 0; s < nsc - 1; ++s) res_l[s] = rho * dyi[s] + Yi[s] * drho;
+    res_l[nsc - 1] = drhoN;
+    res_l[nu] = dPa[2]; res_l[nv] = dPa[3]; res_l[nw] = dPa[4];
+    res_l[np] = dPa[1];
+  }
+  for (int s = np + 1; s < nprim; ++s) res_l[s] = -res_l[s];   /* turbulence variables, not transformed */
+  for (int s = 0; s < nprim; ++s) res_l[s] = res_l[s] * dt_ / vol_ * strang;
+  for (int s = 0; s < nprim; ++s) p_l[s] = po_l[s] + (p_l[s] - po_l[s] + res_l[s]) * rkc;
+  for (int s = 0; s < nsc; ++s) if (p_l[s] < 1e-20) p_l[s] = 1e-20;
+  if (S->rans) for (int s = S->nt; s < S->nt + S->nrans; ++s) if (p_l[s] < 1e-10) p_l[s] = 1e-10;
+}
+
+static void o_newstate_prim(const o_case *C, double *P, const double *PO, const double *R, double dt, double rkc) {
+  const o_scheme *S = &C->sc;
+  const int64_t NC = C->NC;
+  const int nprim = S->nprim;
+  for (int b = 0; b < C->nb; ++b) {
+    const o_block *B = &C->blk[b];
+#pragma omp parallel for collapse(2) schedule(static) num_threads(C->nthreads > 0 ? C->nthreads : 1)
+    for (int k = 1; k <= B->n3; ++k)
+      for (int j = 1; j <= B->n2; ++j)
+        for (int i = 1; i <= B->n1; ++i) {
+          const int64_t c0 = ROW(B, i, j, k);
+          double p_l[ONPRIM], po_l[ONPRIM], res_l[ONPRIM];
+          for (int s = 0; s < nprim; ++s) { p_l[s] = PV(P, s, c0); po_l[s] = PV(PO, s, c0); res_l[s] = PV(R, s, c0); }
+          o_prim_cell(&C->tab, S, p_l, po_l, res_l, dt, C->vol[c0], C->strang, rkc);
+          for (int s = 0; s < nprim; ++s) PV(P, s, c0) = p_l[s];
+        }
+  }
+}
+
+/* the state update of the case's integration variables (Assign_Integration_Variables, Lib_Newstate.f90:43-60) */
+void o_newstate(const o_case *C, double *P, const double *PO, const double *R, double dt, double rkc) {
+  if (C->sc.prim) o_newstate_prim(C, P, PO, R, dt, rkc);
+  else o_newstate_cons(C, P, PO, R, dt, rkc);
 }
 
 /* RKcoeff(3,3): src/lib/numerics/time/explicit/Lib_RK.f90:6-9 */

@@ -55,6 +55,7 @@ typedef struct {
   int ndir;                     /* computational directions (Lib_Metrics.f90:139-196) */
   int point_implicit;           /* SA Patankar factor (sa_body.inc:113-116) */
   int guard;                    /* thermo guard on (Mod_Thermo_Cache.f90:323-335) */
+  int prim;                     /* integration-variables = prim: Newstate_Primitive_blk (Lib_Newstate.f90:340-531) */
 } o_scheme;
 
 typedef struct {

@@ -75,6 +75,7 @@ void *oa_create(const mose_scheme *sc, const mose_tables *tb, int nb, const int 
   S->rans = sc->nrans > 0 ? 2 : 0;
   S->cfl = sc->cfl; S->vnn = sc->vnn; S->prl = sc->prandtl; S->sc = sc->sc; S->sct = sc->sct; S->prt = sc->prt;
   S->n_rk = sc->n_rk; S->ndir = sc->ndir; S->point_implicit = sc->point_implicit; S->guard = sc->thermo_guard;
+  S->prim = sc->integration == MOSE_INT_PRIM;
   a->tabs = (double *)calloc((size_t)4 * tb->nsc * MOSE_NTAB, sizeof(double));
   tables_init(&C->tab, tb, a->tabs);
   a->blk = (o_block *)calloc((size_t)nb, sizeof(o_block));

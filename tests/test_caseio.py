@@ -86,7 +86,7 @@ def test_vortex_oracle_conserves_and_advects(vortex):
 @pytest.mark.parametrize("key,value,msg", [
     ("time-accurate", "false", "local time stepping"),
     ("riemann-solver", "SLAU", "SLAU"),
-    ("integration-variables", "prim", "integration-variables"),
+    ("integration-variables", "prec", "integration-variables"),
 ])
 def test_refusals(tmp_path, key, value, msg):
     d = tmp_path / "case"
@@ -98,6 +98,17 @@ def test_refusals(tmp_path, key, value, msg):
     (d / "input.ini").write_text("\n".join(out) + "\n")
     with pytest.raises(caseio.CaseFormatError, match=msg):
         caseio.load_case(str(d))
+
+
+def test_primitive_update_key(tmp_path):
+    """integration-variables = prim selects the primitive update (Assign_Integration_Variables, Lib_Newstate.f90:43-60)."""
+    d = tmp_path / "case"
+    shutil.copytree(VORTEX, d)
+    ini = (d / "input.ini").read_text()
+    assert "integration-variables = cons" in ini
+    (d / "input.ini").write_text(ini.replace("integration-variables = cons", "integration-variables = prim"))
+    assert caseio.load_case(str(d)).scheme.integration == 1
+    assert caseio.load_case(str(VORTEX)).scheme.integration == 0
 
 
 def test_unsupported_boundary_type(tmp_path):

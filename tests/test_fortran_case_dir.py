@@ -43,6 +43,8 @@ def _edit(path, old, new):
     ("boundary type", "INPUT/bc.txt", "201", "501", "boundary type 501"),
     ("local time stepping", "input.ini", "time-accurate = true", "time-accurate = false", "local time stepping"),
     ("solver", "input.ini", "riemann-solver = HLLC", "riemann-solver = SLAU", "riemann-solver = SLAU"),
+    ("preconditioned update", "input.ini", "integration-variables = cons", "integration-variables = prec",
+     "integration-variables = prec"),
 ])
 def test_fortran_refuses_what_is_off_the_path(tmp_path, what, file, old, new, msg):
     d = expand("isentropic-vortex", tmp_path)
@@ -50,3 +52,15 @@ def test_fortran_refuses_what_is_off_the_path(tmp_path, what, file, old, new, ms
     r = run(d, 5, str(tmp_path / "dump.bin"), "dump")
     assert r.returncode != 0, what
     assert msg in r.stderr, r.stderr
+
+
+def test_fortran_reads_the_primitive_update_key(tmp_path):
+    """integration-variables = prim: the Fortran reader's scheme equals the Python loader's (integration = 1)."""
+    d = expand("isentropic-vortex", tmp_path)
+    _edit(os.path.join(d, "input.ini"), "integration-variables = cons", "integration-variables = prim")
+    r = run(d, 5, str(tmp_path / "dump.bin"), "dump")
+    assert r.returncode == 0, r.stdout + r.stderr
+    case = caseio.load_case(d, 5)
+    assert case.scheme.integration == 1
+    got = read_dump(str(tmp_path / "dump.bin"), bool(case.scheme.viscous), case.nprim)
+    assert got["scheme"] == bytes(case.scheme)
