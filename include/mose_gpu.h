@@ -109,12 +109,15 @@ typedef struct {
   int32_t point_implicit;
   int32_t thermo_guard; /* 1: T outside (1, 9990) K is an error (MOSE_ERR_THERMO) */
   int32_t integration;  /* MOSE_INT_CONS (default) or MOSE_INT_PRIM */
-  int32_t reserved[6];
+  int32_t irs;          /* 1: implicit residual smoothing before the update (IRS_apply, Lib_IRS.f90:53-96); plain path */
+  int32_t irs_njacobi; /* its Jacobi sweeps per direction (the reference's 2) */
+  int32_t reserved[4];
   double mco;           /* riemann-options-Mco (LMRoe) */
   double cfl, vnn;
   double prandtl;       /* > 0: k = mu cp / Pr */
   double sc, sct, prt;  /* Schmidt, turbulent Schmidt and Prandtl numbers */
-  double reserved_d[8];
+  double irs_beta;      /* the smoothing coefficient (irs-beta) */
+  double reserved_d[7];
 } mose_scheme;
 
 typedef struct {

@@ -43,6 +43,8 @@ struct Case {
   double *dl = nullptr, *m = nullptr, *vol = nullptr, *yn = nullptr, *fm1 = nullptr, *fm2 = nullptr, *fm3 = nullptr;
   double *Fc[3] = {nullptr, nullptr, nullptr}, *Fd[3] = {nullptr, nullptr, nullptr};
   double* binc = nullptr;
+  // implicit residual smoothing (scheme irs = 1): the residual and the two Jacobi buffers (flow variables)
+  double *dR = nullptr, *drs1 = nullptr, *drs2 = nullptr;
   mose_blk* dblk = nullptr;
   mose_bc_record* drec = nullptr;
   mose_farfield* dff = nullptr;
@@ -279,6 +281,11 @@ static int plan_case(const mose_scheme* sc, int nb, const int* dims, int nrec, c
     why = "integration must be MOSE_INT_CONS (0) or MOSE_INT_PRIM (1)";
     return MOSE_ERR_ARG;
   }
+  if (sc->irs != 0 && sc->irs != 1) { why = "irs must be 0 or 1"; return MOSE_ERR_ARG; }
+  if (sc->irs && (sc->irs_njacobi < 1 || sc->irs_njacobi > 64 || !(sc->irs_beta >= 0.0) || !std::isfinite(sc->irs_beta))) {
+    why = "implicit residual smoothing needs 1..64 Jacobi sweeps and a finite irs_beta >= 0";
+    return MOSE_ERR_ARG;
+  }
   c.sc = *sc;
   c.nb = nb;
   c.nprim = sc->nsc + 4 + sc->nrans;
@@ -389,6 +396,7 @@ static size_t case_bytes(const Case& c) {
   b += (size_t)(c.NF1 + c.NF2 + c.NF3) * 4 * d;            // fm1..3
   for (int q = 0; q < 3; ++q) b += 2 * (size_t)c.NFI[q] * c.nprim * d;   // face buffers
   b += 2 * (size_t)std::max<int64_t>(c.NBP, 1) * c.nprim * d;           // boundary increments
+  if (c.sc.irs) b += (size_t)c.NC * (c.nprim + 2 * (c.sc.nsc + 4)) * d + 3 * 256;   // IRS residual and sweeps
   b += c.blk.size() * sizeof(mose_blk) + c.rec.size() * sizeof(mose_bc_record) + 8192 + 64 * 1024;
   b += (std::max<size_t>(c.ff.size(), 1) * sizeof(mose_farfield) + 255) / 256 * 256;
   b += ((size_t)std::max<int64_t>(c.NBP, 1) + 255) / 256 * 256;
@@ -402,6 +410,7 @@ extern "C" int mose_case_bytes(const mose_scheme* sc, int nb, const int* dims, i
   if (!bytes) return MOSE_ERR_ARG;
   // sizing does not need the records themselves
   if (!sc || nb <= 0 || !dims) return MOSE_ERR_ARG;
+  c.sc = *sc;
   c.nprim = sc->nsc + 4 + sc->nrans;
   int64_t NC = 0, NF = 0, NFI[3] = {0, 0, 0}, NBP = 0;
   for (int b = 0; b < nb; ++b) {
@@ -447,6 +456,11 @@ static int alloc_case(mose_ctx* ctx, Case& c) {
     c.Fd[q] = (double*)take((size_t)c.NFI[q] * c.nprim * d);
   }
   c.binc = (double*)take(2 * (size_t)std::max<int64_t>(c.NBP, 1) * c.nprim * d);
+  if (c.sc.irs) {
+    c.dR = (double*)take((size_t)c.NC * c.nprim * d);
+    c.drs1 = (double*)take((size_t)c.NC * (c.sc.nsc + 4) * d);
+    c.drs2 = (double*)take((size_t)c.NC * (c.sc.nsc + 4) * d);
+  }
   c.dblk = (mose_blk*)take(c.blk.size() * sizeof(mose_blk));
   c.drec = (mose_bc_record*)take(std::max<size_t>(c.rec.size(), 1) * sizeof(mose_bc_record));
   c.dff = (mose_farfield*)take(std::max<size_t>(c.ff.size(), 1) * sizeof(mose_farfield));
@@ -564,6 +578,12 @@ extern "C" int mose_case_create(mose_ctx* ctx, const mose_scheme* sc, int nb, co
   HIPCHK(ctx, hipMemsetAsync(c->P0, 0, 3 * (size_t)c->NC * c->nprim * d, s));
   // the boundary planes start at zero: the record kernel leaves the entries of no-flux (MOSE 0) faces untouched
   HIPCHK(ctx, hipMemsetAsync(c->binc, 0, 2 * (size_t)std::max<int64_t>(c->NBP, 1) * c->nprim * d, s));
+  if (c->sc.irs) {
+    HIPCHK(ctx, hipMemsetAsync(c->dR, 0, (size_t)c->NC * c->nprim * d, s));
+    HIPCHK(ctx, hipMemsetAsync(c->drs1, 0, (size_t)c->NC * (c->sc.nsc + 4) * d, s));
+    HIPCHK(ctx, hipMemsetAsync(c->drs2, 0, (size_t)c->NC * (c->sc.nsc + 4) * d, s));
+    c->path = MOSE_PATH_PLAIN;   // the smoothing needs the whole residual: the per-block launches
+  }
   HIPCHK(ctx, hipStreamSynchronize(s));
   rc = ensure_rpart(ctx, *c, plain_parts(*c));
   if (rc != MOSE_OK) { case_free(c.get()); return rc; }
@@ -654,6 +674,42 @@ static void launch_flux(Case& c, const CaseDev& D, hipStream_t s) {
     if (n0 > 0) k_face<NS, NR, 0><<<nblocks(n0, 128), 128, 0, s>>>(D, b);
     if (n1_ > 0) k_face<NS, NR, 1><<<nblocks(n1_, 128), 128, 0, s>>>(D, b);
     if (n2_ > 0) k_face<NS, NR, 2><<<nblocks(n2_, 128), 128, 0, s>>>(D, b);
+  }
+}
+
+// Implicit residual smoothing (IRS_apply, Lib_IRS.f90:53-96) between the fluxes and the update of a plain-path stage:
+// the residual of every interior cell into dR (pass 2, boundary increments, SA source), its scaling for the
+// conservative update, then per computational direction the residual's ghost layer (block faces, then the connection
+// records' donors) and the Jacobi sweeps of each block (drs1 <- dR over the block, sweeps drs1 -> drs2 -> drs1,
+// dR <- drs1 on the interior).  The update then reads dR.
+template <int NS, int NR>
+static void launch_irs(Case& c, const CaseDev& D, hipStream_t s) {
+  const int nf = c.sc.nsc + 4;
+  for (int b = 0; b < c.nb; ++b) {
+    const int64_t ncell = (int64_t)c.blk[b].n1 * c.blk[b].n2 * c.blk[b].n3;
+    k_residual_out<NS, NR><<<nblocks(ncell, 256), 256, 0, s>>>(D, b, c.dR);
+    if (c.sc.integration != MOSE_INT_PRIM) k_irs_scale<NS, NR><<<nblocks(ncell, 256), 256, 0, s>>>(D, b, c.dR);
+  }
+  const int nrec = (int)c.rec.size();
+  for (int d = 0; d < c.sc.ndir; ++d) {
+    for (int b = 0; b < c.nb; ++b) {
+      const mose_blk& B = c.blk[b];
+      const int64_t nbp = 2 * ((int64_t)B.n2 * B.n3 + (int64_t)B.n1 * B.n3 + (int64_t)B.n1 * B.n2);
+      k_irs_ghost_faces<<<nblocks(nbp, 256), 256, 0, s>>>(c.dblk, b, c.dR, c.NC, nf);
+    }
+    if (nrec > 0) k_irs_ghost_conn<<<nblocks(nrec, 256), 256, 0, s>>>(c.dblk, c.drec, nrec, c.dR, c.NC, nf);
+    for (int b = 0; b < c.nb; ++b) {
+      const mose_blk& B = c.blk[b];
+      const int64_t ncell = (int64_t)B.n1 * B.n2 * B.n3;
+      const int64_t nflat = (int64_t)B.pi * B.pj * (B.n3 + 2 * MOSE_GC);
+      const int64_t off = d == 0 ? 1 : (d == 1 ? (int64_t)B.pi : (int64_t)B.pi * B.pj);
+      k_irs_copy_block<<<nblocks(nflat, 256), 256, 0, s>>>(B, c.dR, c.drs1, c.NC, nf);
+      for (int jac = 0; jac < c.sc.irs_njacobi; ++jac) {
+        k_irs_sweep<<<nblocks(ncell, 256), 256, 0, s>>>(B, c.dR, c.drs1, c.drs2, c.NC, nf, off, c.sc.irs_beta);
+        k_irs_copy_interior<<<nblocks(ncell, 256), 256, 0, s>>>(B, c.drs2, c.drs1, c.NC, nf);
+      }
+      k_irs_copy_interior<<<nblocks(ncell, 256), 256, 0, s>>>(B, c.drs1, c.dR, c.NC, nf);
+    }
   }
 }
 
@@ -811,11 +867,13 @@ static int run_steps(mose_ctx* ctx, Case& c, int nsteps) {
           if (last) k_dt_final<<<1, 64, 0, s>>>(c.dtnext, c.dt, nullptr, 0);
         } else {
           launch_flux<NS, NR>(c, D, s);
+          if (c.sc.irs) launch_irs<NS, NR>(c, D, s);
           if (c.timing) HIPCHK(ctx, hipEventRecord(c.tev[iev++], s));
           int64_t part0 = 0;
           for (int b = 0; b < c.nb; ++b) {
             int64_t ncell = (int64_t)c.blk[b].n1 * c.blk[b].n2 * c.blk[b].n3;
-            k_update<NS, NR><<<nblocks(ncell, 256), 256, 0, s>>>(D, b, rkc, last ? c.rpart : nullptr, part0);
+            k_update<NS, NR><<<nblocks(ncell, 256), 256, 0, s>>>(D, b, rkc, last ? c.rpart : nullptr, part0,
+                                                                 c.sc.irs ? c.dR : nullptr);
             part0 += nblocks(ncell, 256);
           }
           if (last) c.nparts_last = part0;
@@ -902,6 +960,8 @@ extern "C" int mose_case_tune(mose_ctx* ctx, int id, const mose_tuning* t, mose_
   if (t->path != MOSE_PATH_FUSED && t->path != MOSE_PATH_PLAIN && t->path != MOSE_PATH_FAST)
     return fail(ctx, MOSE_ERR_ARG, "unknown path");
   if (c->launched || c->pending) return fail(ctx, MOSE_ERR_ARG, "mose_case_tune: a launch of the case awaits its poll");
+  if (c->sc.irs && t->path != MOSE_PATH_PLAIN)
+    return fail(ctx, MOSE_ERR_ARG, "implicit residual smoothing runs on the plain path (MOSE_PATH_PLAIN)");
   if (t->path == MOSE_PATH_FAST && c->sc.nsc == 9 && c->sc.integration == MOSE_INT_PRIM)
     return fail(ctx, MOSE_ERR_ARG, "the performance path's nine-species kernels take the conservative update only");
   HIPCHK(ctx, hipSetDevice(ctx->device));

@@ -583,25 +583,27 @@ __device__ __forceinline__ double cell_mil(const mose_dtab& T, const double* p, 
 
 template <int NS, int NR>
 __device__ __forceinline__ void update_one(const CaseDev& C, const mose_blk& B, int b, int64_t t, double rkc,
-                                           double* dres);
+                                           double* dres, const double* Rin);
 
 // one thread per interior cell of block b: residual, SA source, RK stage update, state check; on the last stage
-// also the per-workgroup sums of the squared step changes (rpart != nullptr)
+// also the per-workgroup sums of the squared step changes (rpart != nullptr).  Rin != nullptr: the residual is read
+// from Rin (smoothed by IRS, SA source included; scaled too for the conservative update) instead of the faces.
 template <int NS, int NR>
-__global__ void __launch_bounds__(256) k_update(CaseDev C, int b, double rkc, double* rpart, int64_t part0) {
+__global__ void __launch_bounds__(256) k_update(CaseDev C, int b, double rkc, double* rpart, int64_t part0,
+                                                const double* Rin) {
   using V = Vars<NS, NR>;
   const mose_blk B = C.blk[b];
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double dres[V::nprim - NS + 1];
 #pragma unroll
   for (int q = 0; q < V::nprim - NS + 1; ++q) dres[q] = 0.0;
-  if (t < (int64_t)B.n1 * B.n2 * B.n3) update_one<NS, NR>(C, B, b, t, rkc, rpart ? dres : nullptr);
+  if (t < (int64_t)B.n1 * B.n2 * B.n3) update_one<NS, NR>(C, B, b, t, rkc, rpart ? dres : nullptr, Rin);
   if (rpart) block_sum_store<V::nprim - NS + 1>(dres, rpart + (part0 + blockIdx.x) * (V::nprim - NS + 1));
 }
 
 template <int NS, int NR>
 __device__ __forceinline__ void update_one(const CaseDev& C, const mose_blk& B, int b, int64_t t, double rkc,
-                                           double* dres) {
+                                           double* dres, const double* Rin) {
   using V = Vars<NS, NR>;
   int i = (int)(t % B.n1) + 1;
   int64_t rem = t / B.n1;
@@ -610,7 +612,12 @@ __device__ __forceinline__ void update_one(const CaseDev& C, const mose_blk& B, 
   const int64_t NC = C.NC;
   int64_t c0 = row(B, i, j, k);
   double res[V::nprim], p[V::nprim], po[V::nprim], out[V::nprim];
-  cell_residual<NS, NR>(C, B, i, j, k, res);
+  if (Rin) {
+#pragma unroll
+    for (int s = 0; s < V::nprim; ++s) res[s] = Rin[s * NC + c0];
+  } else {
+    cell_residual<NS, NR>(C, B, i, j, k, res);
+  }
 #pragma unroll
   for (int s = 0; s < V::nprim; ++s) { p[s] = C.Pin[s * NC + c0]; po[s] = C.PO[s * NC + c0]; }
   double rho = rho_sum<NS>(p), Rg = f_Rtot<NS>(T, p);
@@ -620,8 +627,9 @@ __device__ __forceinline__ void update_one(const CaseDev& C, const mose_blk& B, 
     if (Tseed != Tseed) Tseed = 300.0;
     Tseed = dmin2(dmax2(Tseed, 1.0), 9990.0);
   }
-  if (NR > 0) cell_sa<NS, NR>(C, B, i, j, k, c0, rho, cell_mil<NS>(T, p, rho, Tseed), res);
+  if (NR > 0 && !Rin) cell_sa<NS, NR>(C, B, i, j, k, c0, rho, cell_mil<NS>(T, p, rho, Tseed), res);
   if (C.prim) update_cell_prim<NS, NR>(T, p, po, res, C.vol[c0], *C.dt, C.strang, rkc, out);
+  else if (Rin) update_cell<NS, NR, true>(T, p, po, res, C.vol[c0], *C.dt, C.strang, rkc, Tseed, out);
   else update_cell<NS, NR>(T, p, po, res, C.vol[c0], *C.dt, C.strang, rkc, Tseed, out);
   bool ok = out[V::np] >= 0.0;
 #pragma unroll
@@ -659,6 +667,86 @@ __global__ void __launch_bounds__(256) k_residual_out(CaseDev C, int b, double* 
   }
 #pragma unroll
   for (int s = 0; s < V::nprim; ++s) R[s * C.NC + c0] = res[s];
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// implicit residual smoothing: IRS_apply (Lib_IRS.f90:53-96) on the residual array R (var-last, the arena's rows)
+// ------------------------------------------------------------------------------------------------------------
+// irs_scale_blk (:99-121): R = -R / vol * dt * strang on the interior cells, every variable (conservative update only)
+template <int NS, int NR>
+__global__ void __launch_bounds__(256) k_irs_scale(CaseDev C, int b, double* R) {
+  using V = Vars<NS, NR>;
+  const mose_blk B = C.blk[b];
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)B.n1 * B.n2 * B.n3) return;
+  const int i = (int)(t % B.n1) + 1;
+  const int64_t rem = t / B.n1;
+  const int j = (int)(rem % B.n2) + 1, k = (int)(rem / B.n2) + 1;
+  const int64_t c0 = row(B, i, j, k);
+  const double vol = C.vol[c0], dt = *C.dt;
+#pragma unroll
+  for (int s = 0; s < V::nprim; ++s) R[s * C.NC + c0] = -R[s * C.NC + c0] / vol * dt * C.strang;
+}
+
+// irs_ghost_residual (:125-152): the first ghost layer behind every face of block b takes its boundary cell's value
+// (the flow variables 0..nf-1); one thread per boundary cell-face
+static __global__ void __launch_bounds__(256) k_irs_ghost_faces(const mose_blk* blk, int b, double* R, int64_t NC, int nf) {
+  const mose_blk B = blk[b];
+  const int64_t a1 = (int64_t)B.n2 * B.n3, a2 = (int64_t)B.n1 * B.n3, a3 = (int64_t)B.n1 * B.n2;
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int i, j, k, f;
+  if (t < 2 * a1) { f = t < a1 ? 1 : 2; t -= (f - 1) * a1; j = (int)(t % B.n2) + 1; k = (int)(t / B.n2) + 1; i = f == 1 ? 1 : B.n1; }
+  else if ((t -= 2 * a1) < 2 * a2) { f = t < a2 ? 3 : 4; t -= (f - 3) * a2; i = (int)(t % B.n1) + 1; k = (int)(t / B.n1) + 1; j = f == 3 ? 1 : B.n2; }
+  else if ((t -= 2 * a2) < 2 * a3) { f = t < a3 ? 5 : 6; t -= (f - 5) * a3; i = (int)(t % B.n1) + 1; j = (int)(t / B.n1) + 1; k = f == 5 ? 1 : B.n3; }
+  else return;
+  const int64_t cm = row(B, i, j, k);
+  const int64_t cg = row(B, i - c_guide[f - 1][0], j - c_guide[f - 1][1], k - c_guide[f - 1][2]);
+  for (int s = 0; s < nf; ++s) R[s * NC + cg] = R[s * NC + cm];
+}
+
+// irs_ghost_conn (:175-192): the ghost behind each connection record's receiver cell takes the donor cell's value
+static __global__ void __launch_bounds__(256) k_irs_ghost_conn(const mose_blk* blk, const mose_bc_record* rec, int nrec,
+                                                        double* R, int64_t NC, int nf) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nrec) return;
+  const mose_bc_record r = rec[e];
+  if (r.typ != MOSE_BC_CONNECTION) return;
+  const mose_blk Bm = blk[r.b], Bs = blk[r.bs];
+  const int64_t cg = row(Bm, r.i - c_guide[r.f - 1][0], r.j - c_guide[r.f - 1][1], r.k - c_guide[r.f - 1][2]);
+  const int64_t cd = row(Bs, r.is, r.js, r.ks);
+  for (int s = 0; s < nf; ++s) R[s * NC + cg] = R[s * NC + cd];
+}
+
+// irs_smooth_blk (:194-235), its three loops: R* = R over the block's whole arena, one Jacobi sweep on the interior
+// (rs2 = (R + beta (rs1(c - off) + rs1(c + off))) / (1 + 2 beta)), and an interior copy
+static __global__ void __launch_bounds__(256) k_irs_copy_block(mose_blk B, const double* src, double* dst, int64_t NC, int nf) {
+  const int64_t n = (int64_t)B.pi * B.pj * (B.n3 + 2 * GC);
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  for (int s = 0; s < nf; ++s) dst[s * NC + B.boff + t] = src[s * NC + B.boff + t];
+}
+
+static __global__ void __launch_bounds__(256) k_irs_sweep(mose_blk B, const double* R, const double* rs1, double* rs2,
+                                                   int64_t NC, int nf, int64_t off, double beta) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)B.n1 * B.n2 * B.n3) return;
+  const int i = (int)(t % B.n1) + 1;
+  const int64_t rem = t / B.n1;
+  const int j = (int)(rem % B.n2) + 1, k = (int)(rem / B.n2) + 1;
+  const int64_t c0 = row(B, i, j, k);
+  for (int s = 0; s < nf; ++s)
+    rs2[s * NC + c0] = (R[s * NC + c0] + beta * (rs1[s * NC + c0 - off] + rs1[s * NC + c0 + off])) / (1.0 + 2.0 * beta);
+}
+
+static __global__ void __launch_bounds__(256) k_irs_copy_interior(mose_blk B, const double* src, double* dst, int64_t NC,
+                                                           int nf) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)B.n1 * B.n2 * B.n3) return;
+  const int i = (int)(t % B.n1) + 1;
+  const int64_t rem = t / B.n1;
+  const int j = (int)(rem % B.n2) + 1, k = (int)(rem / B.n2) + 1;
+  const int64_t c0 = row(B, i, j, k);
+  for (int s = 0; s < nf; ++s) dst[s * NC + c0] = src[s * NC + c0];
 }
 
 // ------------------------------------------------------------------------------------------------------------

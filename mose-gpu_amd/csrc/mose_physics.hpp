@@ -632,7 +632,8 @@ __device__ __forceinline__ void prim2cons(const mose_dtab& T, const double* q, d
   for (int s = V::np + 1; s < V::nprim; ++s) out[s] = q[s];
 }
 
-template <int NS, int NR>
+// PRESCALED: res already holds -R/vol*dt*strang, smoothed (implicit residual smoothing, newstate_cons_body.inc:47)
+template <int NS, int NR, bool PRESCALED = false>
 __device__ __forceinline__ void update_cell(const mose_dtab& T, const double* p, const double* po, const double* res,
                                             double vol, double dt, double strang, double rkc, double Tseed, double* out) {
   using V = Vars<NS, NR>;
@@ -641,7 +642,7 @@ __device__ __forceinline__ void update_cell(const mose_dtab& T, const double* p,
   prim2cons<NS, NR>(T, p, c);
 #pragma unroll
   for (int s = 0; s < V::nprim; ++s) {
-    double r = -res[s] / vol * dt * strang;
+    double r = PRESCALED ? res[s] : -res[s] / vol * dt * strang;
     c[s] = cold[s] + (c[s] - cold[s] + r) * rkc;
   }
   double rho = rho_sum<NS>(c), Rg = f_Rtot<NS>(T, c);

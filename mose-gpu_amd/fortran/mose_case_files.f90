@@ -130,7 +130,10 @@ contains
     type(mose_scheme), intent(out) :: sc
     character(len=256) :: eq, turb, ts, recon, lim, rs, iv
     logical :: viscous
+    real(c_double) :: x
+    integer :: ios
     sc%nsc = nsc; sc%nrans = 0; sc%reserved = 0; sc%reserved_d = 0.0_c_double; sc%integration = MOSE_INT_CONS
+    sc%irs = 0; sc%irs_njacobi = 2; sc%irs_beta = 0.0_c_double
     sc%thermo_guard = 1; sc%point_implicit = 1; sc%ndir = ndir
     sc%sct = 0.9_c_double; sc%prt = 0.9_c_double; sc%vnn = 0.5_c_double; sc%prandtl = 0.0_c_double
     sc%sc = 0.5_c_double
@@ -164,6 +167,12 @@ contains
     case default; call case_stop('integration-variables = ' // trim(iv) // &
                                  ': cons or prim (the preconditioned update, prec, is not on this path)')
     end select
+    ! implicit residual smoothing: on with irs = true or irs-beta > 0, beta = irs-beta (IO_Ini_Open.f90:138-146)
+    read(ini_get(ini, 'MOSE-Numerics', 'irs-beta', '0.0'), *, iostat=ios) x
+    if (ios /= 0) x = 0.0_c_double
+    if (x > 0.0_c_double .or. truth(ini_get(ini, 'MOSE-Numerics', 'irs', 'false'))) then
+      sc%irs = 1; sc%irs_beta = x
+    end if
     recon = ini_get(ini, 'MOSE-Numerics', 'space-reconstruction', '')
     if (trim(recon) /= 'MUSCL' .and. trim(recon) /= 'first-order') &
       call case_stop('space-reconstruction = ' // trim(recon) // ': MUSCL or first-order')

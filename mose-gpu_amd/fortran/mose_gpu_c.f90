@@ -41,9 +41,11 @@ module mose_gpu_c
   type, bind(C) :: mose_scheme
     integer(c_int32_t) :: nsc, nrans, limiter, riemann, viscous, n_rk, ndir, point_implicit, thermo_guard
     integer(c_int32_t) :: integration
-    integer(c_int32_t) :: reserved(6)
+    integer(c_int32_t) :: irs, irs_njacobi
+    integer(c_int32_t) :: reserved(4)
     real(c_double) :: mco, cfl, vnn, prandtl, sc, sct, prt
-    real(c_double) :: reserved_d(8)
+    real(c_double) :: irs_beta
+    real(c_double) :: reserved_d(7)
   end type mose_scheme
 
   type, bind(C) :: mose_tables

@@ -320,8 +320,16 @@ def scheme_from_ini(ini: dict, ndir: int, nsc: int):
     pimp = 1
     if turb == "SA":
         pimp = 1 if _truth(ini.get("MOSE-Turbulence", {}).get("point-implicit", "true")) else 0
+    # implicit residual smoothing: on with irs = true or irs-beta > 0, beta = irs-beta, two Jacobi sweeps
+    # (IO_Ini_Open.f90:138-146, Lib_IRS.f90:10-14)
+    try:
+        irs_beta = float(nu.get("irs-beta", "0.0"))
+    except ValueError:
+        irs_beta = 0.0
+    irs_on = irs_beta > 0.0 or _truth(nu.get("irs", "false"))
     return make_scheme(nsc=nsc, nrans=1 if turb == "SA" else 0, limiter=lim, riemann=rs, viscous=viscous, n_rk=rk,
-                       ndir=ndir, cfl=float(nu["cfl"]), mco=mco, point_implicit=pimp, integration=intvar, **kw)
+                       ndir=ndir, cfl=float(nu["cfl"]), mco=mco, point_implicit=pimp, integration=intvar,
+                       irs_beta=irs_beta if irs_on else None, **kw)
 
 
 def load_case(case_dir: str, nsteps: int | None = None) -> Case:

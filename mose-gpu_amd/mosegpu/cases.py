@@ -141,13 +141,18 @@ class Case:
 
 def make_scheme(nsc=1, nrans=0, limiter="minmod", riemann="HLLC", viscous=False, n_rk=3, ndir=3, cfl=0.5,
                 vnn=0.5, mco=1.0, prandtl=0.0, sc=0.5, sct=0.9, prt=0.9, point_implicit=1, guard=1,
-                integration="cons") -> Scheme:
+                integration="cons", irs_beta=None, irs_njacobi=2) -> Scheme:
     """integration: the state update's variables, "cons" (Newstate_Conservative_blk) or "prim"
-    (Newstate_Primitive_blk), as [MOSE-Numerics] integration-variables (Lib_Newstate.f90:43-60)."""
+    (Newstate_Primitive_blk), as [MOSE-Numerics] integration-variables (Lib_Newstate.f90:43-60).
+    irs_beta: implicit residual smoothing on with this coefficient (IRS_apply, Lib_IRS.f90:53-96), None = off;
+    irs_njacobi: its Jacobi sweeps per direction."""
     s = Scheme()
     s.nsc = nsc; s.nrans = nrans; s.limiter = LIM[limiter]; s.riemann = RS[riemann]; s.viscous = int(viscous)
     s.n_rk = n_rk; s.ndir = ndir; s.point_implicit = point_implicit; s.thermo_guard = guard
     s.integration = INTEGRATION[integration]
+    s.irs = 0 if irs_beta is None else 1
+    s.irs_beta = 0.0 if irs_beta is None else float(irs_beta)
+    s.irs_njacobi = irs_njacobi
     s.mco = mco; s.cfl = cfl; s.vnn = vnn; s.prandtl = prandtl; s.sc = sc; s.sct = sct; s.prt = prt
     return s
 

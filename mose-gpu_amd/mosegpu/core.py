@@ -61,9 +61,11 @@ class Scheme(C.Structure):
     _fields_ = [
         ("nsc", C.c_int32), ("nrans", C.c_int32), ("limiter", C.c_int32), ("riemann", C.c_int32),
         ("viscous", C.c_int32), ("n_rk", C.c_int32), ("ndir", C.c_int32), ("point_implicit", C.c_int32),
-        ("thermo_guard", C.c_int32), ("integration", C.c_int32), ("reserved", C.c_int32 * 6),
+        ("thermo_guard", C.c_int32), ("integration", C.c_int32), ("irs", C.c_int32), ("irs_njacobi", C.c_int32),
+        ("reserved", C.c_int32 * 4),
         ("mco", C.c_double), ("cfl", C.c_double), ("vnn", C.c_double), ("prandtl", C.c_double),
-        ("sc", C.c_double), ("sct", C.c_double), ("prt", C.c_double), ("reserved_d", C.c_double * 8),
+        ("sc", C.c_double), ("sct", C.c_double), ("prt", C.c_double), ("irs_beta", C.c_double),
+        ("reserved_d", C.c_double * 7),
     ]
 
 

@@ -1722,7 +1722,8 @@ static void o_newstate_cons(const o_case *C, double *P, const double *PO, const 
             out[np] = rho * E0;
             for (int s = np + 1; s < nprim; ++s) out[s] = q[s];
           }
-          for (int s = 0; s < nprim; ++s) res_l[s] = -res_l[s] / vol_l * dt * strang;
+          /* with IRS the residual arrives scaled and smoothed (newstate_cons_body.inc:47) */
+          if (!S->irs) for (int s = 0; s < nprim; ++s) res_l[s] = -res_l[s] / vol_l * dt * strang;
           for (int s = 0; s < nprim; ++s) cons[s] = consold[s] + (cons[s] - consold[s] + res_l[s]) * rkc;
           double rho = 0.0;
           for (int s = 0; s < nsc; ++s) rho = rho + cons[s];
@@ -1883,6 +1884,91 @@ void o_newstate(const o_case *C, double *P, const double *PO, const double *R, d
   else o_newstate_cons(C, P, PO, R, dt, rkc);
 }
 
+/* ------------------------------------------------------------------------------------------------ */
+/* implicit residual smoothing: IRS_apply (Lib_IRS.f90:53-96)                                         */
+/* ------------------------------------------------------------------------------------------------ */
+/* The conservative update's scaling R = -R/vol*dt*strang of every variable (irs_scale_blk :99-121; the primitive
+ * update smooths the raw residual), then per computational direction: the first ghost layer of the residual
+ * (irs_ghost_residual :125-172: zero gradient on every block face, then each connection record's ghost from its donor
+ * cell) and njacobi Jacobi sweeps R* <- (R + beta (R*(c-off) + R*(c+off))) / (1 + 2 beta) on the interior, started
+ * from R* = R over the whole block (irs_smooth_blk :194-235); R <- R*.  Only the flow variables 0..np are smoothed. */
+void o_irs(const o_case *C, double *R, double dt) {
+  const o_scheme *S = &C->sc;
+  const int64_t NC = C->NC;
+  const int nprim = S->nprim, nf = S->np + 1;
+  if (!S->prim) {
+    for (int b = 0; b < C->nb; ++b) {
+      const o_block *B = &C->blk[b];
+      for (int k = 1; k <= B->n3; ++k)
+        for (int j = 1; j <= B->n2; ++j)
+          for (int i = 1; i <= B->n1; ++i) {
+            const int64_t c0 = ROW(B, i, j, k);
+            const double vol_ = C->vol[c0];
+            for (int s = 0; s < nprim; ++s) PV(R, s, c0) = -PV(R, s, c0) / vol_ * dt * C->strang;
+          }
+    }
+  }
+  double *rs1 = (double *)malloc(sizeof(double) * (size_t)NC * nf);
+  double *rs2 = (double *)malloc(sizeof(double) * (size_t)NC * nf);
+  for (int d = 0; d < S->ndir; ++d) {
+    for (int b = 0; b < C->nb; ++b) {
+      const o_block *B = &C->blk[b];
+      const int n[3] = {B->n1, B->n2, B->n3};
+      for (int f = 1; f <= 6; ++f) {
+        const int a = (f - 1) / 2, lo = (f % 2) == 1;
+        const int u = a == 0 ? 1 : 0, v = a == 2 ? 1 : 2;   /* the two directions along the face */
+        for (int q = 1; q <= n[v]; ++q)
+          for (int p = 1; p <= n[u]; ++p) {
+            int cm[3], cg[3];
+            cm[a] = lo ? 1 : n[a]; cm[u] = p; cm[v] = q;
+            for (int e = 0; e < 3; ++e) cg[e] = cm[e] - guide[f - 1][e];
+            const int64_t rm = ROW(B, cm[0], cm[1], cm[2]), rg = ROW(B, cg[0], cg[1], cg[2]);
+            for (int s = 0; s < nf; ++s) PV(R, s, rg) = PV(R, s, rm);
+          }
+      }
+    }
+    for (int e = 0; e < C->nrec; ++e) {
+      const mose_bc_record *rc = &C->rec[e];
+      if (rc->typ != 1) continue;
+      const o_block *Bm = &C->blk[rc->b], *Bs = &C->blk[rc->bs];
+      const int64_t rg = ROW(Bm, rc->i - guide[rc->f - 1][0], rc->j - guide[rc->f - 1][1], rc->k - guide[rc->f - 1][2]);
+      const int64_t rd = ROW(Bs, rc->is, rc->js, rc->ks);
+      for (int s = 0; s < nf; ++s) PV(R, s, rg) = PV(R, s, rd);
+    }
+    for (int b = 0; b < C->nb; ++b) {
+      const o_block *B = &C->blk[b];
+      const int64_t off = d == 0 ? 1 : (d == 1 ? (int64_t)B->pi : (int64_t)B->pi * B->pj);
+      const int64_t nflat = (int64_t)B->pi * B->pj * (B->n3 + 2 * OGC);
+      for (int s = 0; s < nf; ++s)
+        for (int64_t c = 0; c < nflat; ++c) rs1[(int64_t)s * NC + B->boff + c] = PV(R, s, B->boff + c);
+      for (int jac = 0; jac < S->irs_njacobi; ++jac) {
+        for (int k = 1; k <= B->n3; ++k)
+          for (int j = 1; j <= B->n2; ++j)
+            for (int i = 1; i <= B->n1; ++i) {
+              const int64_t c0 = ROW(B, i, j, k);
+              for (int s = 0; s < nf; ++s)
+                rs2[(int64_t)s * NC + c0] = (PV(R, s, c0) + S->irs_beta * (rs1[(int64_t)s * NC + c0 - off] +
+                                             rs1[(int64_t)s * NC + c0 + off])) / (1.0 + 2.0 * S->irs_beta);
+            }
+        for (int k = 1; k <= B->n3; ++k)
+          for (int j = 1; j <= B->n2; ++j)
+            for (int i = 1; i <= B->n1; ++i) {
+              const int64_t c0 = ROW(B, i, j, k);
+              for (int s = 0; s < nf; ++s) rs1[(int64_t)s * NC + c0] = rs2[(int64_t)s * NC + c0];
+            }
+      }
+      for (int k = 1; k <= B->n3; ++k)
+        for (int j = 1; j <= B->n2; ++j)
+          for (int i = 1; i <= B->n1; ++i) {
+            const int64_t c0 = ROW(B, i, j, k);
+            for (int s = 0; s < nf; ++s) PV(R, s, c0) = rs1[(int64_t)s * NC + c0];
+          }
+    }
+  }
+  free(rs1);
+  free(rs2);
+}
+
 /* RKcoeff(3,3): src/lib/numerics/time/explicit/Lib_RK.f90:6-9 */
 static double rk_coeff(int n_rk, int irk) {
   static const double c[3][3] = {{1.0, 0.0, 0.0}, {1.0, 0.5, 0.0}, {1.0, 0.25, 2.0 / 3.0}};
@@ -1905,6 +1991,7 @@ int o_step(const o_case *C, double *P, double *PO, double *R, int nsteps, double
       for (int irk = 1; irk <= C->sc.n_rk; ++irk) {
         o_fill_ghosts(C, P);
         o_residual(C, P, R, dt);
+        if (C->sc.irs) o_irs(C, R, dt);
         o_newstate(C, P, PO, R, dt, rk_coeff(C->sc.n_rk, irk));
       }
       if (i_strang + n_strang == 3 && C->react) C->react(C, P, dt, C->react_ctx);

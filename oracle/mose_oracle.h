@@ -56,6 +56,8 @@ typedef struct {
   int point_implicit;           /* SA Patankar factor (sa_body.inc:113-116) */
   int guard;                    /* thermo guard on (Mod_Thermo_Cache.f90:323-335) */
   int prim;                     /* integration-variables = prim: Newstate_Primitive_blk (Lib_Newstate.f90:340-531) */
+  int irs, irs_njacobi;         /* implicit residual smoothing (Lib_IRS.f90) and its Jacobi sweeps */
+  double irs_beta;
 } o_scheme;
 
 typedef struct {

@@ -10,6 +10,7 @@
 
 void o_fill_ghosts(const o_case *C, double *P);
 void o_residual(const o_case *C, double *P, double *R, double dt);
+void o_irs(const o_case *C, double *R, double dt);
 double o_compute_dt(const o_case *C, const double *P);
 void o_newstate(const o_case *C, double *P, const double *PO, const double *R, double dt, double rkc);
 int o_step(const o_case *C, double *P, double *PO, double *R, int nsteps, double *dt_out);
@@ -76,6 +77,7 @@ void *oa_create(const mose_scheme *sc, const mose_tables *tb, int nb, const int 
   S->cfl = sc->cfl; S->vnn = sc->vnn; S->prl = sc->prandtl; S->sc = sc->sc; S->sct = sc->sct; S->prt = sc->prt;
   S->n_rk = sc->n_rk; S->ndir = sc->ndir; S->point_implicit = sc->point_implicit; S->guard = sc->thermo_guard;
   S->prim = sc->integration == MOSE_INT_PRIM;
+  S->irs = sc->irs != 0; S->irs_njacobi = sc->irs_njacobi; S->irs_beta = sc->irs_beta;
   a->tabs = (double *)calloc((size_t)4 * tb->nsc * MOSE_NTAB, sizeof(double));
   tables_init(&C->tab, tb, a->tabs);
   a->blk = (o_block *)calloc((size_t)nb, sizeof(o_block));
@@ -176,6 +178,8 @@ void oa_residual(void *h, double *P, double *R, double dt) {
   o_residual(&a->c, P, R, dt);
 }
 double oa_dt(void *h, const double *P) { return o_compute_dt(&((oa_t *)h)->c, P); }
+/* the implicit residual smoothing of a residual array alone (IRS_apply) */
+void oa_irs(void *h, double *R, double dt) { o_irs(&((oa_t *)h)->c, R, dt); }
 void oa_newstate(void *h, double *P, const double *PO, const double *R, double dt, double rkc) {
   o_newstate(&((oa_t *)h)->c, P, PO, R, dt, rkc);
 }

@@ -32,6 +32,7 @@ def oracle_lib():
     L.oa_dt.restype = C.c_double
     L.oa_dt.argtypes = [vp, dp]
     L.oa_newstate.argtypes = [vp, dp, dp, dp, C.c_double, C.c_double]
+    L.oa_irs.argtypes = [vp, dp, C.c_double]
     L.oa_step.restype = C.c_int
     L.oa_step.argtypes = [vp, dp, ip, dp]
     L.oa_limiter.restype = C.c_double
@@ -109,6 +110,11 @@ class OracleCase:
     def newstate(self, P: np.ndarray, PO: np.ndarray, R: np.ndarray, dt: float, rkc: float):
         """one RK stage update of P from the base PO and the residual R (Newstate_Conservative_blk)"""
         oracle_lib().oa_newstate(self.h, _d(P), _d(PO), _d(R), dt, rkc)
+
+    def irs(self, R: np.ndarray, dt: float):
+        """the implicit residual smoothing of R in place (IRS_apply): the scaling for the conservative update, the
+        residual's ghost layer and the Jacobi sweeps of every computational direction"""
+        oracle_lib().oa_irs(self.h, _d(R), dt)
 
     def dt(self, P: np.ndarray) -> float:
         return oracle_lib().oa_dt(self.h, _d(P))

@@ -33,7 +33,8 @@ def parity(case, nsteps, path, tile, what, bitwise=True):
     Pg, rep, chosen = run_device(case, nsteps, path, tile)
     Po = case.P.copy()
     dts = OracleCase(case, nthreads=16).step(Po, nsteps)
-    err = compare(case, Pg, Po, f"{what} (primitive update) path {path} tile {chosen} x {nsteps}")
+    upd = "primitive" if case.scheme.integration == mosegpu.INT_PRIM else "conservative"
+    err = compare(case, Pg, Po, f"{what} ({upd} update) path {path} tile {chosen} x {nsteps}")
     assert abs(rep.time - dts.sum()) <= (1e-15 if bitwise else 1e-13) * dts.sum()
     if bitwise:
         assert np.array_equal(interiors(case, Pg), interiors(case, Po))
