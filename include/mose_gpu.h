@@ -68,7 +68,7 @@ enum {
 
 /* limiter ids (Lib_Limiters.f90:21-61) and Riemann solver ids (Lib_Riemann.f90:23-107) */
 enum { MOSE_LIM_MINMOD = 1, MOSE_LIM_VANLEER = 2, MOSE_LIM_VANALBADA = 3, MOSE_LIM_MC = 4, MOSE_LIM_SUPERBEE = 5, MOSE_LIM_FIRST = 6 };
-enum { MOSE_RS_HLLC = 1, MOSE_RS_HLLE = 2, MOSE_RS_AUSMP = 4, MOSE_RS_LMROE = 12 };
+enum { MOSE_RS_HLLC = 1, MOSE_RS_HLLE = 2, MOSE_RS_SLAU = 3, MOSE_RS_AUSMP = 4, MOSE_RS_LMROE = 12 };
 /* integration variables of the state update (Assign_Integration_Variables, Lib_Newstate.f90:43-60): conservative
  * (Newstate_Conservative_blk) or primitive (Newstate_Primitive_blk, Lib_Newstate.f90:340-531, with Theta = (drho/dp)_T;
  * the preconditioned variant, integration-variables = prec, is not built) */

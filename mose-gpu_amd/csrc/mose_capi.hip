@@ -270,9 +270,9 @@ static int plan_case(const mose_scheme* sc, int nb, const int* dims, int nrec, c
   if (sc->nrans < 0 || sc->nrans > 1) { why = "nrans must be 0 or 1 (Spalart-Allmaras)"; return MOSE_ERR_ARG; }
   if (sc->nrans > 0 && !sc->viscous) { why = "Spalart-Allmaras needs the viscous fluxes"; return MOSE_ERR_ARG; }
   if (sc->limiter < 1 || sc->limiter > 6) { why = "limiter id must be 1..6"; return MOSE_ERR_ARG; }
-  if (!(sc->riemann == MOSE_RS_HLLC || sc->riemann == MOSE_RS_HLLE || sc->riemann == MOSE_RS_AUSMP ||
-        sc->riemann == MOSE_RS_LMROE)) {
-    why = "riemann solver must be HLLC (1), HLLE (2), AUSM+ (4) or LMRoe (12)";
+  if (!(sc->riemann == MOSE_RS_HLLC || sc->riemann == MOSE_RS_HLLE || sc->riemann == MOSE_RS_SLAU ||
+        sc->riemann == MOSE_RS_AUSMP || sc->riemann == MOSE_RS_LMROE)) {
+    why = "riemann solver must be HLLC (1), HLLE (2), SLAU (3), AUSM+ (4) or LMRoe (12)";
     return MOSE_ERR_ARG;
   }
   if (sc->n_rk < 1 || sc->n_rk > 3) { why = "n_rk must be 1..3"; return MOSE_ERR_ARG; }
@@ -960,6 +960,9 @@ extern "C" int mose_case_tune(mose_ctx* ctx, int id, const mose_tuning* t, mose_
   if (t->path != MOSE_PATH_FUSED && t->path != MOSE_PATH_PLAIN && t->path != MOSE_PATH_FAST)
     return fail(ctx, MOSE_ERR_ARG, "unknown path");
   if (c->launched || c->pending) return fail(ctx, MOSE_ERR_ARG, "mose_case_tune: a launch of the case awaits its poll");
+  if (t->path == MOSE_PATH_FAST && c->sc.riemann == MOSE_RS_SLAU)
+    return fail(ctx, MOSE_ERR_ARG, "the performance path's kernels take HLLC, HLLE, AUSM+ and LMRoe (SLAU: the parity "
+                                   "or plain path)");
   if (c->sc.irs && t->path != MOSE_PATH_PLAIN)
     return fail(ctx, MOSE_ERR_ARG, "implicit residual smoothing runs on the plain path (MOSE_PATH_PLAIN)");
   if (t->path == MOSE_PATH_FAST && c->sc.nsc == 9 && c->sc.integration == MOSE_INT_PRIM)

@@ -386,6 +386,53 @@ __device__ __forceinline__ F5 lmroe(const mose_dtab& T, double mco, const double
   return F;
 }
 
+// SLAU, the part common to the interior kernels (conv_physics.inc:126-167) and riemann_SLAU (Lib_Riemann.f90:140-198):
+// from the sound speeds and the total enthalpies h0l/h0r, which each computes its own way (riemann below)
+__device__ __forceinline__ F5 slau(double dlt, double drt, double ul, double vl, double wl, double pl, double al,
+                                   double ur, double vr, double wr, double pr, double ar, double nx, double ny, double nz,
+                                   double h0l, double h0r) {
+  const double unl = ul * nx + vl * ny + wl * nz, unr = ur * nx + vr * ny + wr * nz;
+  const double af = 0.5 * (al + ar);
+  const double ml = unl / af, mr = unr / af;
+  const double mt = dmin2(1.0, sqrt(0.5 * ((ul * ul + vl * vl + wl * wl) + (ur * ur + vr * vr + wr * wr))) / af);
+  const double chi = (1.0 - mt) * (1.0 - mt);
+  const double vnm = (dlt * fabs(unl) + drt * fabs(unr)) / (dlt + drt);
+  const double frho = -dmax2(dmin2(ml, 0.0), -1.0) * dmin2(dmax2(mr, 0.0), 1.0);
+  const double vnl = (1.0 - frho) * vnm + frho * fabs(unl);
+  const double vnr = (1.0 - frho) * vnm + frho * fabs(unr);
+  F5 F;
+  F.r = 0.5 * (dlt * (unl + vnl) + drt * (unr - vnr) - (chi / af) * (pr - pl));
+  double bl, br;
+  if (fabs(ml) < 1.0) bl = 0.25 * (2.0 - ml) * ((ml + 1.0) * (ml + 1.0));
+  else if (ml >= 0.0) bl = 1.0;
+  else bl = 0.0;
+  if (fabs(mr) < 1.0) br = 0.25 * (2.0 + mr) * ((mr - 1.0) * (mr - 1.0));
+  else if (mr >= 0.0) br = 0.0;
+  else br = 1.0;
+  const double ph = 0.5 * (pl + pr) + 0.5 * (bl - br) * (pl - pr) + (1.0 - chi) * (bl + br - 1.0) * 0.5 * (pl + pr);
+  if (F.r >= 0.0) {
+    F.u = F.r * ul + nx * ph; F.v = F.r * vl + ny * ph; F.w = F.r * wl + nz * ph; F.e = F.r * h0l;
+  } else {
+    F.u = F.r * ur + nx * ph; F.v = F.r * vr + ny * ph; F.w = F.r * wr + nz * ph; F.e = F.r * h0r;
+  }
+  return F;
+}
+
+// riemann_SLAU's total enthalpy of one state (Lib_Riemann.f90:152-166)
+template <int NS>
+__device__ __forceinline__ double slau_h0(const mose_dtab& T, const double* d, double dtot, double u, double v,
+                                          double w, double p) {
+  double R = 0.0, h0 = 0.0;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) R = R + T.Ri[s] * d[s] / dtot;
+  const double Tk = p / (dtot * R);
+  const int Ti = ftrunc(Tk);
+  const double dT = Tk - Ti;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) h0 = h0 + tabT(T.h, s, Ti, dT) * d[s] / dtot;
+  return h0 + 0.5 * (u * u + v * v + w * w);
+}
+
 // The Riemann dispatch (conv_physics.inc:103-183; Lib_Riemann.f90:114-136).  hllc_clamp: the interior
 // kernel clamps the table index of the HLLC Roe averages, the record kernels do not.  RS != 0 fixes the
 // solver at compile time (one kernel per solver keeps the register allocation of the other two out).
@@ -401,6 +448,19 @@ __device__ __forceinline__ F5 riemann(const mose_dtab& T, int rs_, double mco, b
     return lmroe<NS>(T, mco, dl, ul, vl, wl, pl, dlt, dr, ur, vr, wr, pr, drt, nx, ny, nz, ro);
   }
   if (rs == 4) return ausmp<NS>(T, dl, ul, vl, wl, pl, al, dlt, dr, ur, vr, wr, pr, ar, drt, nx, ny, nz);
+  if (rs == 3) {
+    // SLAU: the interior kernels take h0 of the inline Roe block (conv_physics.inc:57-78), the records
+    // riemann_SLAU's own
+    double h0l, h0r;
+    if (hllc_clamp) {
+      Roe ro = roe_avg<NS>(T, true, dl, dr, dlt, drt, ul, vl, wl, ur, vr, wr, pl, pr, Rl, Rr);
+      h0l = ro.h0l; h0r = ro.h0r;
+    } else {
+      h0l = slau_h0<NS>(T, dl, dlt, ul, vl, wl, pl);
+      h0r = slau_h0<NS>(T, dr, drt, ur, vr, wr, pr);
+    }
+    return slau(dlt, drt, ul, vl, wl, pl, al, ur, vr, wr, pr, ar, nx, ny, nz, h0l, h0r);
+  }
   Roe ro = roe_avg<NS>(T, hllc_clamp, dl, dr, dlt, drt, ul, vl, wl, ur, vr, wr, pl, pr, Rl, Rr);
   if (rs == 2) return hlle(dlt, drt, ul, vl, wl, pl, al, ur, vr, wr, pr, ar, nx, ny, nz, ro);
   return hllc(dlt, drt, ul, vl, wl, pl, al, ur, vr, wr, pr, ar, nx, ny, nz, ro);

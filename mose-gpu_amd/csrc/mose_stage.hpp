@@ -430,11 +430,13 @@ void launch_stage_shape(const CaseDev& D, const StageArgs& A, int rs, bool visc,
     if (rs == MOSE_RS_LMROE) k_stage<NS, NR, true, MOSE_RS_LMROE, BS, SH><<<grid, BS, lds, s>>>(D, A);
     else if (rs == MOSE_RS_AUSMP) k_stage<NS, NR, true, MOSE_RS_AUSMP, BS, SH><<<grid, BS, lds, s>>>(D, A);
     else if (rs == MOSE_RS_HLLE) k_stage<NS, NR, true, MOSE_RS_HLLE, BS, SH><<<grid, BS, lds, s>>>(D, A);
+    else if (rs == MOSE_RS_SLAU) k_stage<NS, NR, true, MOSE_RS_SLAU, BS, SH><<<grid, BS, lds, s>>>(D, A);
     else k_stage<NS, NR, true, MOSE_RS_HLLC, BS, SH><<<grid, BS, lds, s>>>(D, A);
   } else {
     if (rs == MOSE_RS_LMROE) k_stage<NS, NR, false, MOSE_RS_LMROE, BS, SH><<<grid, BS, lds, s>>>(D, A);
     else if (rs == MOSE_RS_AUSMP) k_stage<NS, NR, false, MOSE_RS_AUSMP, BS, SH><<<grid, BS, lds, s>>>(D, A);
     else if (rs == MOSE_RS_HLLE) k_stage<NS, NR, false, MOSE_RS_HLLE, BS, SH><<<grid, BS, lds, s>>>(D, A);
+    else if (rs == MOSE_RS_SLAU) k_stage<NS, NR, false, MOSE_RS_SLAU, BS, SH><<<grid, BS, lds, s>>>(D, A);
     else k_stage<NS, NR, false, MOSE_RS_HLLC, BS, SH><<<grid, BS, lds, s>>>(D, A);
   }
 }

@@ -194,9 +194,10 @@ contains
     select case (trim(rs))
     case ('HLLC'); sc%riemann = MOSE_RS_HLLC
     case ('HLLE'); sc%riemann = MOSE_RS_HLLE
+    case ('SLAU'); sc%riemann = MOSE_RS_SLAU
     case ('AUSM+'); sc%riemann = MOSE_RS_AUSMP
     case ('LMRoe'); sc%riemann = MOSE_RS_LMROE
-    case default; call case_stop('riemann-solver = ' // trim(rs) // ': HLLC, HLLE, AUSM+ or LMRoe on this path')
+    case default; call case_stop('riemann-solver = ' // trim(rs) // ': HLLC, HLLE, SLAU, AUSM+ or LMRoe on this path')
     end select
     sc%mco = num(ini_get(ini, 'MOSE-Numerics', 'riemann-options-Mco', '1.0'), 'riemann-options-Mco')
     if (.not. ini_has(ini, 'MOSE-Numerics', 'cfl')) call case_stop('[MOSE-Numerics] cfl is required')

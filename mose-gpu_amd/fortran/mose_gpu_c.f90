@@ -18,7 +18,7 @@ module mose_gpu_c
                                MOSE_BC_WALL_ADIABATIC = 5, MOSE_BC_WALL_ISOTHERMAL = 6, MOSE_BC_EXTRAPOLATION = 11
   integer(c_int), parameter :: MOSE_LIM_MINMOD = 1, MOSE_LIM_VANLEER = 2, MOSE_LIM_VANALBADA = 3, MOSE_LIM_MC = 4, &
                                MOSE_LIM_SUPERBEE = 5, MOSE_LIM_FIRST = 6
-  integer(c_int), parameter :: MOSE_RS_HLLC = 1, MOSE_RS_HLLE = 2, MOSE_RS_AUSMP = 4, MOSE_RS_LMROE = 12
+  integer(c_int), parameter :: MOSE_RS_HLLC = 1, MOSE_RS_HLLE = 2, MOSE_RS_SLAU = 3, MOSE_RS_AUSMP = 4, MOSE_RS_LMROE = 12
   integer(c_int), parameter :: MOSE_PATH_FUSED = 0, MOSE_PATH_PLAIN = 1, MOSE_PATH_FAST = 2
   integer(c_int), parameter :: MOSE_INT_CONS = 0, MOSE_INT_PRIM = 1
   integer(c_int), parameter :: MOSE_NTAB = 10001, MOSE_GC = 2, MOSE_MAX_SPECIES = 16

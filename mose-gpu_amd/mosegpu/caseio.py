@@ -303,8 +303,8 @@ def scheme_from_ini(ini: dict, ndir: int, nsc: int):
     if lim not in LIM:
         raise CaseFormatError(f"flux-limiter = {lim!r}: one of {sorted(LIM)}")
     rs = nu.get("riemann-solver", "HLLC")
-    if rs not in ("HLLC", "HLLE", "AUSM+", "LMRoe"):
-        raise CaseFormatError(f"riemann-solver = {rs}: HLLC, HLLE, AUSM+ or LMRoe on this path")
+    if rs not in ("HLLC", "HLLE", "SLAU", "AUSM+", "LMRoe"):
+        raise CaseFormatError(f"riemann-solver = {rs}: HLLC, HLLE, SLAU, AUSM+ or LMRoe on this path")
     mco = float(nu.get("riemann-options-Mco", "1.0"))
     if "cfl" not in nu:
         raise CaseFormatError("[MOSE-Numerics] cfl is required")

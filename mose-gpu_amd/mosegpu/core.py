@@ -26,7 +26,7 @@ ERRORS = {
 
 BC_NONE, BC_CONNECTION, BC_SYMMETRY, BC_FARFIELD, BC_WALL_ADIABATIC, BC_WALL_ISOTHERMAL, BC_EXTRAPOLATION = 0, 1, 3, 4, 5, 6, 11
 LIM = {"minmod": 1, "vanleer": 2, "vanalbada": 3, "mc": 4, "superbee": 5, "first-order": 6}
-RS = {"HLLC": 1, "HLLE": 2, "AUSM+": 4, "AUSMp": 4, "LMRoe": 12, "Roe": 12}
+RS = {"HLLC": 1, "HLLE": 2, "SLAU": 3, "AUSM+": 4, "AUSMp": 4, "LMRoe": 12, "Roe": 12}
 # integration-variables of the state update (Lib_Newstate.f90:43-60): conservative or primitive
 INTEGRATION = {"cons": 0, "prim": 1}
 INT_CONS, INT_PRIM = 0, 1

@@ -404,6 +404,50 @@ static void lmroe(const o_tables *t, double mco, const double *dl, double ul, do
   }
 }
 
+/* SLAU, the part common to the interior kernels (conv_physics.inc:126-167) and the record kernels' riemann_SLAU
+ * (Lib_Riemann.f90:140-198): from the sound speeds al/ar and the total enthalpies h0l/h0r, which each computes its
+ * own way (below). */
+static void slau(double dltot, double drtot, double ul, double vl, double wl, double pl, double al, double ur,
+                 double vr, double wr, double pr, double ar, double nx, double ny, double nz, double h0l, double h0r,
+                 double F[5]) {
+  const double unl = ul * nx + vl * ny + wl * nz, unr = ur * nx + vr * ny + wr * nz;
+  const double af = 0.5 * (al + ar);
+  const double ml = unl / af, mr = unr / af;
+  const double mt = min2(1.0, sqrt(0.5 * ((ul * ul + vl * vl + wl * wl) + (ur * ur + vr * vr + wr * wr))) / af);
+  const double chi = (1.0 - mt) * (1.0 - mt);
+  const double vnm = (dltot * fabs(unl) + drtot * fabs(unr)) / (dltot + drtot);
+  const double frho = -max2(min2(ml, 0.0), -1.0) * min2(max2(mr, 0.0), 1.0);
+  const double vnl = (1.0 - frho) * vnm + frho * fabs(unl);
+  const double vnr = (1.0 - frho) * vnm + frho * fabs(unr);
+  const double Fr = 0.5 * (dltot * (unl + vnl) + drtot * (unr - vnr) - (chi / af) * (pr - pl));
+  double bl, br;
+  if (fabs(ml) < 1.0) bl = 0.25 * (2.0 - ml) * ((ml + 1.0) * (ml + 1.0));
+  else if (ml >= 0.0) bl = 1.0;
+  else bl = 0.0;
+  if (fabs(mr) < 1.0) br = 0.25 * (2.0 + mr) * ((mr - 1.0) * (mr - 1.0));
+  else if (mr >= 0.0) br = 0.0;
+  else br = 1.0;
+  const double ph = 0.5 * (pl + pr) + 0.5 * (bl - br) * (pl - pr) + (1.0 - chi) * (bl + br - 1.0) * 0.5 * (pl + pr);
+  F[0] = Fr;
+  if (Fr >= 0.0) {
+    F[1] = Fr * ul + nx * ph; F[2] = Fr * vl + ny * ph; F[3] = Fr * wl + nz * ph; F[4] = Fr * h0l;
+  } else {
+    F[1] = Fr * ur + nx * ph; F[2] = Fr * vr + ny * ph; F[3] = Fr * wr + nz * ph; F[4] = Fr * h0r;
+  }
+}
+
+/* riemann_SLAU's total enthalpy of one state (Lib_Riemann.f90:152-166): R = sum Ri rho_i / rho, T = p / (rho R),
+ * the table rows of int(T), h0 = sum h_i rho_i / rho + |u|^2 / 2 */
+static double slau_h0(const o_tables *t, const double *d, double dtot, double u, double v, double w, double p) {
+  double R = 0.0, h0 = 0.0;
+  for (int s = 0; s < t->ns; ++s) R = R + t->Ri[s] * d[s] / dtot;
+  const double Tk = p / (dtot * R);
+  const int Ti = fint(Tk);
+  const double dT = Tk - Ti;
+  for (int s = 0; s < t->ns; ++s) h0 = h0 + tabT(t->h, t, s, Ti, Ti + 1, dT) * d[s] / dtot;
+  return h0 + 0.5 * (u * u + v * v + w * w);
+}
+
 /* riemann_open (rs 4 and 12) and the Riemann dispatch of the record kernels (Lib_Riemann.f90:114-136).
  * `clamp_hllc` selects the table-index clamp of the Roe averages used by HLLC: 1 in the interior kernel
  * (conv_physics.inc:73-74), 0 in the record kernels (Lib_Riemann_HLL.f90:244-247). */
@@ -418,6 +462,17 @@ static void riemann_any(const o_case *C, int rs, int clamp_hllc, const double *d
   } else if (rs == 2) {
     roe_averages(t, clamp_hllc, 0, ONTMAX - 1, dl, dr, dltot, drtot, ul, vl, wl, ur, vr, wr, pl, pr, Rl, Rr, &ro);
     hlle(dltot, drtot, ul, vl, wl, pl, al, ur, vr, wr, pr, ar, nx, ny, nz, &ro, F);
+  } else if (rs == 3) {
+    /* the interior kernels take h0 of the inline Roe block (conv_physics.inc:57-78), the records riemann_SLAU's own */
+    double h0l, h0r;
+    if (clamp_hllc) {
+      roe_averages(t, 1, 0, ONTMAX - 1, dl, dr, dltot, drtot, ul, vl, wl, ur, vr, wr, pl, pr, Rl, Rr, &ro);
+      h0l = ro.h0l; h0r = ro.h0r;
+    } else {
+      h0l = slau_h0(t, dl, dltot, ul, vl, wl, pl);
+      h0r = slau_h0(t, dr, drtot, ur, vr, wr, pr);
+    }
+    slau(dltot, drtot, ul, vl, wl, pl, al, ur, vr, wr, pr, ar, nx, ny, nz, h0l, h0r, F);
   } else if (rs == 4) {
     ausmp(t, dl, ul, vl, wl, pl, al, dltot, dr, ur, vr, wr, pr, ar, drtot, nx, ny, nz, F);
   } else if (rs == 12) {

@@ -12,7 +12,7 @@ from mosegpu.core import NTAB, RECORD_DTYPE, Scheme
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLD = os.path.join(HERE, "golden", "cases")
-CASES = ("isentropic-vortex", "shu-osher", "ein91", "noh87", "diff1")
+CASES = ("isentropic-vortex", "shu-osher", "ein91", "noh87", "diff1", "sod79")
 PROGRAM = mosegpu.LIB_PATH.parent / "mose_case_dir"
 
 

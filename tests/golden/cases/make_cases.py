@@ -9,6 +9,8 @@ container where the reference tree is at /root/reference; the GPU tests read onl
                                in x, no-flux faces (MOSE 0) in y and z: a Mach 3 shock into a density sine wave
   1D/Ein91                     Euler, HLLE + van Leer, SSP-RK3, CFL 0.9, Einfeldt's 1-2-0-3 double rarefaction
   1D/Noh87                     Euler, HLLE, SSP-RK3, CFL 0.5, 400 cells: two cold streams colliding at the centre
+  1D/Sod79                     Euler, SLAU + MC limiter, SSP-RK3, CFL 0.9, 400 x 1 x 1, extrapolation (MOSE 400) on
+                               every face: Sod's shock tube to t = 0.2
   1D/diff1                     Navier-Stokes, 4 species (CH4, H2O, N2, O2; Schmidt 1, Prandtl 1), HLLC + van Leer,
                                SSP-RK2, CFL and VNN 0.95, 200 x 1 x 1, periodic (MOSE 201) in x, symmetry (MOSE 300)
                                in y, no-flux faces (MOSE 0) in z: species layers at rest that diffuse
@@ -19,7 +21,8 @@ import shutil
 
 REF = "/root/reference/test"
 HERE = os.path.dirname(os.path.abspath(__file__))
-CASES = {"isentropic-vortex": "2D/euler/isentropic-vortex", "diff1": "1D/diff1", "shu-osher": "1D/Shu-Osher", "ein91": "1D/Ein91", "noh87": "1D/Noh87"}
+CASES = {"isentropic-vortex": "2D/euler/isentropic-vortex", "diff1": "1D/diff1", "shu-osher": "1D/Shu-Osher", "ein91": "1D/Ein91", "noh87": "1D/Noh87",
+         "sod79": "1D/Sod79"}
 COMPRESS = ("bc.txt", "ic.tec", "thermo.dat", "transport.dat")
 
 

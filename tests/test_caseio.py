@@ -85,7 +85,7 @@ def test_vortex_oracle_conserves_and_advects(vortex):
 
 @pytest.mark.parametrize("key,value,msg", [
     ("time-accurate", "false", "local time stepping"),
-    ("riemann-solver", "SLAU", "SLAU"),
+    ("riemann-solver", "AUSM+M", "AUSM"),
     ("integration-variables", "prec", "integration-variables"),
 ])
 def test_refusals(tmp_path, key, value, msg):

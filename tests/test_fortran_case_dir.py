@@ -42,7 +42,7 @@ def _edit(path, old, new):
 @pytest.mark.parametrize("what,file,old,new,msg", [
     ("boundary type", "INPUT/bc.txt", "201", "501", "boundary type 501"),
     ("local time stepping", "input.ini", "time-accurate = true", "time-accurate = false", "local time stepping"),
-    ("solver", "input.ini", "riemann-solver = HLLC", "riemann-solver = SLAU", "riemann-solver = SLAU"),
+    ("solver", "input.ini", "riemann-solver = HLLC", "riemann-solver = AUSM+M", "riemann-solver = AUSM+M"),
     ("preconditioned update", "input.ini", "integration-variables = cons", "integration-variables = prec",
      "integration-variables = prec"),
 ])
