@@ -9,9 +9,10 @@ that this library runs, and its solution written in the reference's output forma
   ic.tec         Tecplot BLOCK, one zone per block: the nodal x, y, z and the cell-centred primitives
   bc.txt         one record per boundary cell-face, 3-digit MOSE_open types (IO_BC_Open.f90:132-399)
 
-Only what this path runs is accepted; anything else is refused by name (local time stepping, prim/prec
-integration, Riemann solvers other than HLLC / HLLE / AUSM+ / LMRoe, boundary types outside 0/100, 101/201, 300
-(symmetry), 301 with zero heat flux, 302 and 400, axisymmetric meshes).  Files may be stored gzip-compressed
+Only what this path runs is accepted; anything else is refused by name (local time stepping, the preconditioned
+update integration-variables = prec, Riemann solvers other than HLLC / HLLE / SLAU / AUSM+ / LMRoe, boundary types
+outside 0/100, 101/201, 300 (symmetry), 301 with zero heat flux, 302 and 400, axisymmetric meshes).  The primitive
+update (integration-variables = prim) and the implicit residual smoothing (irs, irs-beta) are read into the scheme.  Files may be stored gzip-compressed
 (name + ".gz").  write_fortran_case / read_fortran_result are the raw case and result files of the Fortran driver
 mose_case_run.
 """
