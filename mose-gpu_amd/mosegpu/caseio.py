@@ -11,7 +11,8 @@ that this library runs, and its solution written in the reference's output forma
 
 Only what this path runs is accepted; anything else is refused by name (local time stepping, the preconditioned
 update integration-variables = prec, Riemann solvers other than HLLC / HLLE / SLAU / AUSM+ / LMRoe, boundary types
-outside 0/100, 101/201, 300 (symmetry), 301 with zero heat flux, 302 and 400, axisymmetric meshes).  The primitive
+outside 0/100, 101/201, 300 (symmetry), 301 with zero heat flux, 302, 400 and 405 (the supersonic inlet),
+axisymmetric meshes).  The primitive
 update (integration-variables = prim) and the implicit residual smoothing (irs, irs-beta) are read into the scheme.  Files may be stored gzip-compressed
 (name + ".gz").  write_fortran_case / read_fortran_result are the raw case and result files of the Fortran driver
 mose_case_run.
@@ -26,8 +27,8 @@ import os
 import numpy as np
 
 from .cases import Case, fill_ghosts_nearest, make_scheme
-from .core import (BC_CONNECTION, BC_EXTRAPOLATION, BC_NONE, BC_SYMMETRY, BC_WALL_ADIABATIC, BC_WALL_ISOTHERMAL, LIM, NTAB,
-                   RECORD_DTYPE, RS, Layout, ThermoTables, metrics_build)
+from .core import (BC_CONNECTION, BC_EXTRAPOLATION, BC_FARFIELD, BC_NONE, BC_SYMMETRY, BC_WALL_ADIABATIC, BC_WALL_ISOTHERMAL,
+                   FARFIELD_DTYPE, LIM, NTAB, RECORD_DTYPE, RS, Layout, ThermoTables, metrics_build)
 
 RUNIV = 8314.46261815324     # Mod_Convective.f90:30
 
@@ -214,13 +215,18 @@ def read_tecplot_block(path: str):
 # ------------------------------------------------------------------------------------------------- bc.txt
 # 3-digit MOSE_open types this path runs -> this library's record types (IO_BC_Open.f90:155-211)
 _OPEN_TYPES = {0: BC_NONE, 100: BC_NONE, 101: BC_CONNECTION, 201: BC_CONNECTION, 300: BC_SYMMETRY, 301: BC_WALL_ADIABATIC,
-               302: BC_WALL_ISOTHERMAL, 400: BC_EXTRAPOLATION}
+               302: BC_WALL_ISOTHERMAL, 400: BC_EXTRAPOLATION, 405: BC_FARFIELD}
+# IO_BC_Open.f90:40: the pamb marker of a 405 record, which BC_Inflow_Outflow (Lib_BC.f90:1574-1583) sends to the
+# supersonic closure at any Mach and closes on outward flow
+PAMB_FORCE_SUPERSONIC = -2.71
 
 
-def read_bc_open(path: str) -> np.ndarray:
+def read_bc_open(path: str, inlets: list | None = None) -> np.ndarray:
     """Records of a MOSE_open bc.txt, in file order, as RECORD_DTYPE with 0-based block ids.  A first line
     "b i j k f type" is followed by the type's property line: 101/201 "bs is js ks fs d11 d12 d21 d22",
-    301 "qw ks eps", 302 "Tw ks eps"; 0, 100, 300 and 400 have none."""
+    301 "qw ks eps", 302 "Tw ks eps", 405 "mach T p alpha beta rel_fac ci(species) ci(rans)" (IO_BC_Open.f90:286-300);
+    0, 100, 300 and 400 have none.  The tokens of each 405 line are appended to inlets as (record index, tokens):
+    its angles may name the face normal, which needs the metrics (inlet_rows)."""
     recs = []
     with _open(path) as f:
         lines = [ln for ln in (x.strip() for x in f) if ln]
@@ -250,6 +256,11 @@ def read_bc_open(path: str) -> np.ndarray:
                 raise CaseFormatError("bc.txt: a 301 wall with a nonzero heat flux is not on this path")
             if typ == 302:
                 r["tw"] = v[0]
+        elif typ == 405:
+            tok = [x for x in re.split(r"[,\s]+", lines[q]) if x]
+            q += 1
+            if inlets is not None:
+                inlets.append((len(recs), tok))
         recs.append(r)
     return np.array(recs, dtype=RECORD_DTYPE)
 
@@ -333,6 +344,53 @@ def scheme_from_ini(ini: dict, ndir: int, nsc: int):
                        irs_beta=irs_beta if irs_on else None, **kw)
 
 
+def _fortran_real(tok: str) -> float:
+    return float(tok.rstrip(",").replace("d", "e").replace("D", "e"))
+
+
+def inlet_rows(rec: np.ndarray, inlets: list, geom, layout: Layout, nsc: int, nrans: int) -> np.ndarray:
+    """The far-field table of the 405 records (one row per distinct set of values) and each record's row index
+    (rec["ff"]).  A row: the Mach number, the static T and p (BC_Supersonic_Inflow takes its p0 and T0 as static
+    values, IO_BC_Open.f90:286-300), pamb = PAMB_FORCE_SUPERSONIC, rel_fac, the species mass fractions and the RANS
+    values.  An angle token "normal" is the face normal's angle, alpha = atan(n2 / (n1 + 1e-20)),
+    beta = atan(n3 / (n1 + 1e-20)) (Resolve_Open_BC_Direction, IO_BC_Open.f90:411-451, with the face of
+    Face_Index, Lib_BC.f90:46-74); a number is an angle in radians, used as it is."""
+    rows, index = [], {}
+    for e, tok in inlets:
+        if len(tok) < 6 + nsc + nrans:
+            raise CaseFormatError(f"bc.txt: a 405 record needs {6 + nsc + nrans} values, has {len(tok)}")
+        r = rec[e]
+        b, f = int(r["b"]), int(r["f"])
+        n1, n2, n3 = layout.dims[b]
+        i, j, k = int(r["i"]), int(r["j"]), int(r["k"])
+        dr, m = (f + 1) // 2, f % 2
+        if dr == 1:
+            fm, NF, fi = geom.fm1, layout.NF1, layout.f1off[b] + (i - m) + (n1 + 1) * ((j - 1) + n2 * (k - 1))
+        elif dr == 2:
+            fm, NF, fi = geom.fm2, layout.NF2, layout.f2off[b] + (i - 1) + n1 * ((j - m) + (n2 + 1) * (k - 1))
+        else:
+            fm, NF, fi = geom.fm3, layout.NF3, layout.f3off[b] + (i - 1) + n1 * ((j - 1) + n2 * (k - m))
+        nx, ny, nz = fm[fi], fm[NF + fi], fm[2 * NF + fi]
+        ang = []
+        for t, comp in ((tok[3], ny), (tok[4], nz)):
+            ang.append(math.atan(comp / (nx + 1e-20)) if "normal" in t else _fortran_real(t))
+        row = np.zeros(1, dtype=FARFIELD_DTYPE)[0]
+        row["mach"], row["t0"], row["p0"] = (_fortran_real(x) for x in tok[:3])
+        row["pamb"] = PAMB_FORCE_SUPERSONIC
+        row["alpha"], row["beta"] = ang
+        row["rel_fac"] = _fortran_real(tok[5])
+        for s in range(nsc):
+            row["ci"][s] = _fortran_real(tok[6 + s])
+        for s in range(nrans):
+            row["rans"][s] = _fortran_real(tok[6 + nsc + s])
+        key = row.tobytes()
+        if key not in index:
+            index[key] = len(rows)
+            rows.append(row)
+        rec[e]["ff"] = index[key]
+    return np.array(rows, dtype=FARFIELD_DTYPE)
+
+
 def load_case(case_dir: str, nsteps: int | None = None) -> Case:
     """The case of a reference test directory (input.ini and INPUT/), ready for DeviceCase and the oracle.
     nsteps defaults to [MOSE-Parameters] iter-threshold (0 when the case ends on time-threshold)."""
@@ -355,10 +413,14 @@ def load_case(case_dir: str, nsteps: int | None = None) -> Case:
         if len(fl) != nprim:
             raise CaseFormatError(f"ic.tec zone {b + 1}: {len(fl)} cell variables, {nprim} expected")
     layout = Layout(dims)
-    rec = read_bc_open(os.path.join(inp, "bc.txt"))
+    inlets = []
+    rec = read_bc_open(os.path.join(inp, "bc.txt"), inlets)
     order = np.lexsort((rec["f"], rec["b"]))          # grouped by block, then face; file order inside a group
     rec = np.ascontiguousarray(rec[order])
     geom = metrics_build(layout, nodes, rec, ndir=ndir)
+    pos = np.empty_like(order)
+    pos[order] = np.arange(len(order))
+    ff = inlet_rows(rec, [(int(pos[e]), t) for e, t in inlets], geom, layout, tables.nsc, sch.nrans)
     P = np.zeros((nprim, layout.NC))
     for b, fl in enumerate(fields):
         V = layout.interior(P, b)
@@ -369,7 +431,8 @@ def load_case(case_dir: str, nsteps: int | None = None) -> Case:
     if nsteps is None:
         nsteps = int(float(par.get("iter-threshold", "0")))
     return Case(os.path.basename(os.path.normpath(case_dir)), tables, sch, layout, nodes, rec, geom, P, nsteps,
-                meta={"ini": ini, "ndir": ndir, "time_threshold": float(par.get("time-threshold", "0") or 0)})
+                meta={"ini": ini, "ndir": ndir, "time_threshold": float(par.get("time-threshold", "0") or 0)},
+                farfield=ff)
 
 
 # ------------------------------------------------------------------------------------------------- output files

@@ -22,7 +22,7 @@ import shutil
 REF = "/root/reference/test"
 HERE = os.path.dirname(os.path.abspath(__file__))
 CASES = {"isentropic-vortex": "2D/euler/isentropic-vortex", "diff1": "1D/diff1", "shu-osher": "1D/Shu-Osher", "ein91": "1D/Ein91", "noh87": "1D/Noh87",
-         "sod79": "1D/Sod79"}
+         "sod79": "1D/Sod79", "forward-step": "2D/euler/supersonic-forward-step"}
 COMPRESS = ("bc.txt", "ic.tec", "thermo.dat", "transport.dat")
 
 
