@@ -12,7 +12,8 @@
 !> int64 steps.  "dump" writes what was read instead and touches no GPU:
 !>   int32 nb, nsc, nrans, nrec, ndir; mose_scheme (raw); int32 dims(3, nb); float64 wm(nsc), cp(NTAB, nsc),
 !>   h(NTAB, nsc), [mu, k for a viscous case]; int64 NC, NF(3); float64 dl(NC, 3), m(NC, 9), vol(NC),
-!>   fm1(NF1, 4), fm2(NF2, 4), fm3(NF3, 4); mose_bc_record(nrec) (raw); float64 P(NC, nprim).
+!>   fm1(NF1, 4), fm2(NF2, 4), fm3(NF3, 4); mose_bc_record(nrec) (raw); int32 nff, mose_farfield(nff) (raw);
+!>   float64 P(NC, nprim).
 program mose_case_dir
   use iso_c_binding
   use mose_gpu_c
@@ -26,7 +27,7 @@ program mose_case_dir
   type(mose_tuning) :: tun, chosen
   type(mose_report) :: rep
   type(mose_bc_record), allocatable, target :: rec(:)
-  type(mose_farfield), target :: ff(1)
+  type(mose_farfield), allocatable, target :: ff(:)
   real(c_double), allocatable, target :: wm(:), cp(:, :), h(:, :), mu(:, :), kk(:, :), nodes(:), fields(:, :)
   real(c_double), allocatable, target :: dl(:), m(:), vol(:), yn(:), fm1(:), fm2(:), fm3(:), P(:, :)
   real(c_double) :: tmax
@@ -34,7 +35,7 @@ program mose_case_dir
   integer(c_int), allocatable, target :: cdims(:)
   integer(c_int64_t), allocatable :: boff(:)
   integer(c_int64_t) :: NC, NF(3), row0, c0
-  integer :: nb, nvcc, nrec, nsc, nprim, ndir, nsteps, b, i, j, k, s, u, ios, nargs
+  integer :: nb, nvcc, nrec, nsc, nprim, ndir, nsteps, b, i, j, k, s, u, ios, nargs, nff
   integer(c_int) :: case_id
   logical :: viscous, dump
   type(c_ptr) :: ctx
@@ -116,6 +117,7 @@ program mose_case_dir
   fm1 = 0.0_c_double; fm2 = 0.0_c_double; fm3 = 0.0_c_double
   if (mose_metrics_build(nb, cdims, nodes, nrec, rec, ndir, dl, m, vol, yn, fm1, fm2, fm3) /= MOSE_OK) &
     call case_stop('mose_metrics_build: a degenerate cell or a bad record')
+  call inlet_rows(nb, dims, NF, rec, nrec, fm1, fm2, fm3, nsc, sc%nrans, ff, nff)
 
   if (dump) then
     open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
@@ -127,6 +129,8 @@ program mose_case_dir
     write(u) NC, NF
     write(u) dl, m, vol, fm1, fm2, fm3
     if (nrec > 0) write(u) rec(1:nrec)
+    write(u) int(nff, c_int32_t)
+    if (nff > 0) write(u) ff(1:nff)
     write(u) P
     close(u)
   else
@@ -146,7 +150,7 @@ contains
       tabs%mu = c_null_ptr; tabs%k = c_null_ptr
     end if
     call mose_check(c_null_ptr, mose_ctx_create(0, tabs, ctx), 'mose_ctx_create')
-    call mose_check(ctx, mose_case_create(ctx, sc, nb, cdims, dl, m, vol, yn, fm1, fm2, fm3, nrec, rec, 0, ff, &
+    call mose_check(ctx, mose_case_create(ctx, sc, nb, cdims, dl, m, vol, yn, fm1, fm2, fm3, nrec, rec, nff, ff, &
                                           case_id), 'mose_case_create')
     call mose_check(ctx, mose_state_upload(ctx, case_id, P), 'mose_state_upload')
     call mose_check(ctx, mose_case_tune(ctx, case_id, tun, chosen), 'mose_case_tune')

@@ -14,9 +14,24 @@ module mose_case_files
   implicit none
   private
   public :: ini_t, ini_read, ini_get, ini_has, case_stop, read_tables, read_block_tec, read_bc_open, mesh_ndir, &
-            scheme_from_ini, fill_ghosts_nearest, sort_records, arena_sizes
+            scheme_from_ini, fill_ghosts_nearest, sort_records, arena_sizes, inlet_rows
 
   integer, parameter :: LL = 16384
+  !> IO_BC_Open.f90:40: the pamb marker of a 405 record, which BC_Inflow_Outflow (Lib_BC.f90:1574-1583) sends to the
+  !> supersonic closure at any Mach and closes on outward flow
+  real(c_double), parameter :: PAMB_FORCE_SUPERSONIC = -2.71_c_double
+
+  !> a 405 record's property line (IO_BC_Open.f90:286-300): Mach, static T and p, the two angles (radians, or the
+  !> token "normal": the face normal's angle, resolved after the metrics), rel_fac, then the mass fractions and the
+  !> RANS values
+  type :: inlet_t
+    real(c_double) :: mach = 0, t = 0, p = 0, alpha = 0, beta = 0, rel = 0
+    logical :: an = .false., bn = .false.
+    integer :: nv = 0
+    real(c_double) :: v(MOSE_MAX_SPECIES + 2) = 0
+  end type inlet_t
+  type(inlet_t), allocatable :: inlets(:)
+  integer :: n_inlets = 0
 
   type :: ini_t
     integer :: n = 0
@@ -570,7 +585,8 @@ contains
   ! ---------------------------------------------------------------------------------------------- bc.txt
   !> Records of a MOSE_open bc.txt in file order (IO_BC_Open.f90:132-399), 0-based block ids.  A line
   !> "b i j k f type" is followed by the type's property line: 101/201 "bs is js ks fs d11 d12 d21 d22",
-  !> 301 "qw ks eps", 302 "Tw ks eps"; 0, 100, 300 and 400 have none.
+  !> 301 "qw ks eps", 302 "Tw ks eps", 405 "mach T p alpha beta rel_fac ci(species) ci(rans)"; 0, 100, 300 and 400
+  !> have none.  A 405 record's ff holds, until inlet_rows, the 1-based index of its parsed line.
   subroutine read_bc_open(path, rec, nrec)
     character(len=*), intent(in) :: path
     type(mose_bc_record), allocatable, intent(out) :: rec(:)
@@ -608,6 +624,7 @@ contains
         case (301); r%typ = MOSE_BC_WALL_ADIABATIC
         case (302); r%typ = MOSE_BC_WALL_ISOTHERMAL
         case (400); r%typ = MOSE_BC_EXTRAPOLATION
+        case (405); r%typ = MOSE_BC_FARFIELD
         case default
           write(line, '(a,i0,a,i0,a,i0,a)') 'bc.txt: boundary type ', typ, ' (block ', r%b + 1, ', face ', r%f, &
                                             ') is not on this path'
@@ -627,6 +644,11 @@ contains
           if (typ == 301 .and. x(1) /= 0.0_c_double) &
             call case_stop('bc.txt: a 301 wall with a nonzero heat flux is not on this path')
           if (typ == 302) r%tw = x(1)
+        else if (typ == 405) then
+          call next_line(n)
+          if (n < 0) call case_stop('bc.txt: a 405 record without its property line')
+          call parse_inlet(line)
+          r%ff = n_inlets
         end if
       end associate
     end do
@@ -646,6 +668,126 @@ contains
       end do
     end subroutine next_line
   end subroutine read_bc_open
+
+  subroutine parse_inlet(line)
+    character(len=*), intent(in) :: line
+    character(len=64) :: w(MOSE_MAX_SPECIES + 8)
+    character(len=len(line)) :: s
+    type(inlet_t), allocatable :: g(:)
+    integer :: nw, p, q, ios, e
+    s = line
+    do p = 1, len_trim(s)
+      if (s(p:p) == ',' .or. s(p:p) == achar(9)) s(p:p) = ' '
+    end do
+    nw = 0
+    p = 1
+    do
+      do while (p <= len_trim(s))
+        if (s(p:p) /= ' ') exit
+        p = p + 1
+      end do
+      if (p > len_trim(s) .or. nw == size(w)) exit
+      q = index(s(p:), ' ') + p - 2
+      if (q < p) q = len_trim(s)
+      nw = nw + 1
+      w(nw) = s(p:q)
+      p = q + 1
+    end do
+    if (nw < 6) call case_stop('bc.txt: a 405 property line with fewer than 6 values')
+    if (.not. allocated(inlets)) allocate(inlets(64))
+    if (n_inlets == size(inlets)) then
+      allocate(g(2 * size(inlets)))
+      g(1:n_inlets) = inlets(1:n_inlets)
+      call move_alloc(g, inlets)
+    end if
+    n_inlets = n_inlets + 1
+    associate (t => inlets(n_inlets))
+      t%mach = value(w(1)); t%t = value(w(2)); t%p = value(w(3))
+      t%an = index(w(4), 'normal') > 0
+      t%bn = index(w(5), 'normal') > 0
+      t%alpha = 0.0_c_double; t%beta = 0.0_c_double
+      if (.not. t%an) t%alpha = value(w(4))
+      if (.not. t%bn) t%beta = value(w(5))
+      t%rel = value(w(6))
+      t%nv = nw - 6
+      do e = 1, t%nv
+        t%v(e) = value(w(6 + e))
+      end do
+    end associate
+  contains
+    real(c_double) function value(tok)
+      character(len=*), intent(in) :: tok
+      read(tok, *, iostat=ios) value
+      if (ios /= 0) call case_stop('bc.txt: unreadable 405 value "' // trim(tok) // '"')
+    end function value
+  end subroutine parse_inlet
+
+  !> The far-field table of the 405 records, one row per distinct set of values (bit patterns), and each record's
+  !> 0-based row in ff.  A row: the Mach number, the static T and p (BC_Supersonic_Inflow takes p0 and T0 as static
+  !> values), pamb = PAMB_FORCE_SUPERSONIC, rel_fac, the mass fractions and the RANS values.  An angle "normal" is
+  !> the face normal's, alpha = atan(n2 / (n1 + 1e-20)), beta = atan(n3 / (n1 + 1e-20)) (Resolve_Open_BC_Direction,
+  !> IO_BC_Open.f90:411-451, at the face of Face_Index, Lib_BC.f90:46-74).
+  subroutine inlet_rows(nb, dims, NF, rec, nrec, fm1, fm2, fm3, nsc, nrans, ff, nff)
+    integer, intent(in) :: nb, dims(3, nb), nrec, nsc, nrans
+    integer(c_int64_t), intent(in) :: NF(3)
+    type(mose_bc_record), intent(inout) :: rec(:)
+    real(c_double), intent(in) :: fm1(:), fm2(:), fm3(:)
+    type(mose_farfield), allocatable, intent(out) :: ff(:)
+    integer, intent(out) :: nff
+    integer(c_int64_t) :: off(3, nb), f
+    integer :: b, e, q, dr, md, n1, n2, n3
+    real(c_double) :: nx, ny, nz
+    type(mose_farfield) :: row
+    allocate(ff(max(n_inlets, 1)))
+    nff = 0
+    off(:, 1) = 0
+    do b = 2, nb
+      off(1, b) = off(1, b - 1) + int(dims(1, b - 1) + 1, c_int64_t) * dims(2, b - 1) * dims(3, b - 1)
+      off(2, b) = off(2, b - 1) + int(dims(1, b - 1), c_int64_t) * (dims(2, b - 1) + 1) * dims(3, b - 1)
+      off(3, b) = off(3, b - 1) + int(dims(1, b - 1), c_int64_t) * dims(2, b - 1) * (dims(3, b - 1) + 1)
+    end do
+    do e = 1, nrec
+      if (rec(e)%typ /= MOSE_BC_FARFIELD) cycle
+      associate (r => rec(e), t => inlets(rec(e)%ff))
+        if (t%nv < nsc + nrans) call case_stop('bc.txt: a 405 record without its mass fractions')
+        b = r%b + 1
+        n1 = dims(1, b); n2 = dims(2, b); n3 = dims(3, b)
+        dr = (r%f + 1) / 2
+        md = mod(r%f, 2)
+        if (dr == 1) then
+          f = off(1, b) + (r%i - md) + int(n1 + 1, c_int64_t) * ((r%j - 1) + int(n2, c_int64_t) * (r%k - 1))
+          nx = fm1(f + 1); ny = fm1(NF(1) + f + 1); nz = fm1(2 * NF(1) + f + 1)
+        else if (dr == 2) then
+          f = off(2, b) + (r%i - 1) + int(n1, c_int64_t) * ((r%j - md) + int(n2 + 1, c_int64_t) * (r%k - 1))
+          nx = fm2(f + 1); ny = fm2(NF(2) + f + 1); nz = fm2(2 * NF(2) + f + 1)
+        else
+          f = off(3, b) + (r%i - 1) + int(n1, c_int64_t) * ((r%j - 1) + int(n2, c_int64_t) * (r%k - md))
+          nx = fm3(f + 1); ny = fm3(NF(3) + f + 1); nz = fm3(2 * NF(3) + f + 1)
+        end if
+        row%mach = t%mach; row%t0 = t%t; row%p0 = t%p; row%pamb = PAMB_FORCE_SUPERSONIC
+        row%alpha = t%alpha; row%beta = t%beta
+        if (t%an) row%alpha = atan(ny / (nx + 1e-20_c_double))
+        if (t%bn) row%beta = atan(nz / (nx + 1e-20_c_double))
+        row%rel_fac = t%rel; row%reserved = 0.0_c_double
+        row%ci = 0.0_c_double; row%rans = 0.0_c_double
+        row%ci(1:nsc) = t%v(1:nsc)
+        if (nrans > 0) row%rans(1:nrans) = t%v(nsc + 1:nsc + nrans)
+        q = 0
+        do b = 1, nff
+          if (all(transfer(ff(b), [0_c_int64_t]) == transfer(row, [0_c_int64_t]))) then
+            q = b
+            exit
+          end if
+        end do
+        if (q == 0) then
+          nff = nff + 1
+          ff(nff) = row
+          q = nff
+        end if
+        r%ff = q - 1
+      end associate
+    end do
+  end subroutine inlet_rows
 
   !> grouped by block, then by face, file order inside a group (a stable sort)
   subroutine sort_records(rec, nrec)

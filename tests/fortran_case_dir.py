@@ -8,11 +8,11 @@ import subprocess
 import numpy as np
 
 import mosegpu
-from mosegpu.core import NTAB, RECORD_DTYPE, Scheme
+from mosegpu.core import FARFIELD_DTYPE, NTAB, RECORD_DTYPE, Scheme
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLD = os.path.join(HERE, "golden", "cases")
-CASES = ("isentropic-vortex", "shu-osher", "ein91", "noh87", "diff1", "sod79")
+CASES = ("isentropic-vortex", "shu-osher", "ein91", "noh87", "diff1", "sod79", "forward-step")
 PROGRAM = mosegpu.LIB_PATH.parent / "mose_case_dir"
 
 
@@ -67,6 +67,8 @@ def read_dump(path: str, viscous: bool, nprim: int) -> dict:
     d["fm2"] = take(np.float64, 4 * NF2)
     d["fm3"] = take(np.float64, 4 * NF3)
     d["rec"] = take(RECORD_DTYPE, nrec)
+    nff = int(take(np.int32, 1)[0])
+    d["ff"] = take(FARFIELD_DTYPE, nff)
     d["P"] = take(np.float64, NC * nprim)
     assert o == len(raw), "dump size"
     return d

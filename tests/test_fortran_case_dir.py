@@ -29,6 +29,7 @@ def test_fortran_reads_the_reference_case(tmp_path, name):
     for k in ("dl", "m", "vol", "fm1", "fm2", "fm3"):
         assert np.array_equal(got[k], getattr(g, k)), k
     assert got["rec"].tobytes() == np.ascontiguousarray(case.records).tobytes()
+    assert got["ff"].tobytes() == np.ascontiguousarray(case.farfield).tobytes()
     assert np.array_equal(got["P"], case.P.ravel())
     print(f"{name}: {case.layout.dims} blocks, {len(case.records)} records, nprim {case.nprim}: equal", flush=True)
 

@@ -112,7 +112,7 @@ def test_fortran_case_driver_reacting(gpu_ready, tmp_path):
 
 @pytest.mark.parametrize("name,nsteps,path", [("isentropic-vortex", 20, 0), ("isentropic-vortex", 20, 1),
                                               ("shu-osher", 330, 0), ("diff1", 100, 0), ("noh87", 100, 0),
-                                              ("sod79", 100, 0)])
+                                              ("sod79", 100, 0), ("forward-step", 100, 0)])
 def test_fortran_case_dir_runs_the_reference_case(gpu_ready, tmp_path, name, nsteps, path):
     """mose_case_dir reads the reference's own case directory (input.ini and INPUT/) in Fortran, builds the geometry,
     steps the case through MOSE_GPU_Explicit_Step and writes the state: the oracle's state and time on the same
