@@ -4,7 +4,8 @@ caseio.load_case, and its 405 inlet pinned by the oracle without the reference.
 
 - The reader: a 405 line "mach T p alpha beta rel_fac ci" (IO_BC_Open.f90:286-300) becomes one far-field row with
   the static T and p, the supersonic marker pamb = -2.71, and the angles of the token "normal" resolved from the
-  face normal (Resolve_Open_BC_Direction, IO_BC_Open.f90:411-451): 0 on the inlet faces, which are normal to x.
+  face normal (Resolve_Open_BC_Direction, IO_BC_Open.f90:411-451): 0 on the inlet faces, which are normal to x.  On a
+  sheared block, whose i-faces are tilted, the resolved angle is the normal's own, atan(-1/2).
 - The inlet holds its own state: with the whole field set to the row's static state at its Mach number (rho = p / (R
   T), |u| = M sqrt(gamma R T), the textbook definitions, gamma = cp / (cp - R) from the case's table), the inlet flux
   is the interior flux and no cell upstream of the step changes.  Reading T as a stagnation temperature, dropping the
@@ -54,3 +55,24 @@ def test_inlet_holds_its_own_supersonic_state():
     # the step is there: the cells in front of it do change
     d0 = np.abs(c.layout.interior(P, 0) - c.layout.interior(c.P, 0)) / scale
     assert np.max(d0[..., -2:]) > 1e-3
+
+
+def test_inlet_angles_from_the_face_normal():
+    """On a sheared block (x = X + Y / 2) the i-faces' normal is (1, -1/2, 0) / |.|: "normal" tokens give alpha =
+    atan(-1/2), beta = 0; a number is taken as it is (a Fortran d exponent included); equal lines share one row."""
+    from mosegpu import cases
+
+    n = (4, 3, 2)
+    nodes = cases.box_nodes(*n, (0.0, 0.0, 0.0), (1.0, 0.9, 0.4))
+    nodes[..., 0] += 0.5 * nodes[..., 1]
+    c = cases.box_case(n, {f: BC_FARFIELD for f in range(1, 7)}, viscous=False, nodes=nodes)
+    rec = np.ascontiguousarray(c.records.copy())
+    face1 = np.flatnonzero(rec["f"] == 1)
+    face3 = np.flatnonzero(rec["f"] == 3)[0]
+    inl = [(int(e), ["3.0,", "1.0,", "1.0,", "normal,", "normal,", "1.0,", "1.0,"]) for e in face1]
+    inl.append((int(face3), ["2.0", "1.0", "1.0", "0.25d0", "normal", "1.0", "1.0"]))
+    ff = caseio.inlet_rows(rec, inl, c.geom, c.layout, 1, 0)
+    assert len(ff) == 2 and np.all(ff["pamb"] == -2.71)
+    assert np.all(rec["ff"][face1] == 0) and rec["ff"][face3] == 1
+    assert abs(ff[0]["alpha"] - np.arctan(-0.5)) <= 1e-15 and abs(ff[0]["beta"]) <= 1e-15
+    assert (ff[1]["mach"], ff[1]["alpha"], ff[1]["beta"]) == (2.0, 0.25, 0.0)
